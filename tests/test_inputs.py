@@ -1,0 +1,52 @@
+"""Host-side inputs: basis order, interaction list rule, synthetic states."""
+import numpy as np
+
+from fock import Fock
+from sccm_amd import configs
+from sccm_amd.basis import make_basis
+from sccm_amd.interaction import list_indices, synthetic_interaction
+from sccm_amd.states import check_state, synthetic_state
+
+
+def test_list_counts_match_reference_binaries():
+    # record counts of Codes/int/{KB3G,GCN2850}/IMSRG_*_m1m2m3m4.dat and the sd value
+    assert len(list_indices(make_basis("pf"))) == 14442
+    assert len(list_indices(make_basis("f5pg9"))) == 10330
+    assert len(list_indices(make_basis("sd"))) == 2570
+
+
+def test_basis_order():
+    b = make_basis("pf")
+    assert b.n == 40
+    assert list(b.twom[:8]) == [7, 5, 3, 1, -1, -3, -5, -7]
+    assert (b.t[:20] == -1).all() and (b.t[20:] == 1).all()
+    assert [s for (_, s, _) in b.blocks()] == [8, 4, 6, 2] * 2
+
+
+def test_synthetic_interaction_is_rotation_invariant():
+    b = make_basis("p")
+    inter = synthetic_interaction(b, 1.0, 3, 4)
+    F = Fock(b)
+    H = F.hamiltonian(inter)
+    for J in (F.Jy, F.Jp):
+        C = H @ J - J @ H
+        assert abs(C).max() < 1e-12
+    assert abs(H - H.conj().T).max() < 1e-12
+
+
+def test_synthetic_states_are_bogoliubov():
+    b = make_basis("pf")
+    for seed in (1, 2):
+        U, V = synthetic_state(b, 2, 4, seed=seed)
+        unit, skew = check_state(U, V)
+        assert unit < 1e-13 and skew < 1e-10
+        rho = V @ V.T
+        assert 1.5 < np.trace(rho[:20, :20]) < 4 and 3 < np.trace(rho[20:, 20:]) < 6
+
+
+def test_workload_shapes():
+    w = configs.sd_plain()
+    assert w.mesh_points == 50176 and w.n == 24
+    w1 = configs.pf_gcm(nstates=20)
+    assert len(w1.pairs) == 210 and w1.mesh.n_points == 162000
+    assert w1.mesh_points == 34020000
