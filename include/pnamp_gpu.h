@@ -1,0 +1,136 @@
+/*
+ * pnamp_gpu.h -- C-ABI of the MI355X (gfx950) PNAMP projected-kernel library.
+ *
+ * The library replaces the per-pair kernel step of the reference PNAMP program
+ * (jmyao17/SCCM): for one or many (q, q') pairs of HFB states it evaluates, at
+ * every Euler (alpha, beta, gamma) x gauge (phi_p, phi_n) mesh point, the
+ * rotated norm overlap (Pfaffian), the transition densities, the particle
+ * numbers and the energy, sums the gauge points (particle-number projection)
+ * and projects onto angular momentum.  Everything runs in FP64 on one GPU.
+ *
+ * Conventions (all arrays owned by the caller; the library copies in/out):
+ *  - m-scheme basis of n states, protons first (t = -1) then neutrons (t = +1),
+ *    orbits contiguous, m = j .. -j inside an orbit (tnljm in the reference,
+ *    Codes/GCM/HFBA_Hamiltonian.f90:42-68).  n even, n <= PNAMP_NMAX.
+ *  - matrices are n x n, column-major (Fortran order), real U(i,k), V(i,k).
+ *  - interaction list: 1-based (a,b,c,d) with a<b, c<d, a+b<=c+d and the
+ *    value <ab|V|cd> (H%ka..kd, H%ME2BM of the reference).
+ *  - complex outputs are interleaved (re, im) doubles.
+ *  - kern layout: [PNAMP_NOBS][jmax+1][2 jmax+1][2 jmax+1] complex with
+ *    index (io, J, K1 + jmax, K2 + jmax); io = 0 norm, 1 Z, 2 N, 3 E_kin,
+ *    4 E + E0 (the ZRES slots 1, 2, 3, 4, 7 of Kernel4Triaxial), each the
+ *    projected *norm-weighted* sum, like ZsumJKK in the reference.
+ *  - zres layout: [n_euler][PNAMP_NOBS] complex, Euler points enumerated
+ *    beta outer, alpha, gamma inner (ieuler_ang of the reference).
+ *
+ * Return codes: 0 ok; < 0 API misuse or device error (see pnamp_last_error);
+ * > 0 a singular matrix was met (LAPACK-style INFO: U of a state not
+ * invertible in pnamp_set_states, or a zero pivot of T in a kernel call).
+ * The reference STOPs in those cases (Codes/PNAMP/PNAMP_Proj.f90:33-46,174-184).
+ *
+ * Threading: a context belongs to one host thread and one device; calls are
+ * ordered on the context's stream and return after the results are copied.
+ */
+#ifndef PNAMP_GPU_H
+#define PNAMP_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PNAMP_NOBS 5
+#define PNAMP_NMAX 44
+
+enum {
+  PNAMP_OK = 0,
+  PNAMP_E_ARG = -1,    /* bad argument */
+  PNAMP_E_SIZE = -2,   /* size beyond what the kernels support */
+  PNAMP_E_DEVICE = -3, /* HIP runtime error */
+  PNAMP_E_STATE = -4   /* call out of order (e.g. no states set) */
+};
+
+enum { PNAMP_FOLD_PLAIN = 0, PNAMP_FOLD_LITERAL = 1 };
+
+typedef struct pnamp_ctx pnamp_ctx;
+
+/* Single-particle basis (tnljm%t, n, l, twoj, twom of the reference). */
+typedef struct {
+  int n;
+  const int32_t *t, *nq, *l, *twoj, *twom;
+} pnamp_basis;
+
+/* Hamiltonian (H%ka..kd, H%ME2BM, H%zME1BM, H%E0 of the reference). */
+typedef struct {
+  int64_t nme;
+  const int32_t *ka, *kb, *kc, *kd;
+  const double *me2b;
+  const double *t1b; /* n x n column-major, real */
+  double e0;
+} pnamp_interaction;
+
+/* Projection mesh (AMP%alpha/beta/gamma, AMP%wwalp/wwbet/wwgam, PNP%MPhi,
+ * Input%Cscale, Input%iscale of the reference).  fold selects the
+ * J-projection: PNAMP_FOLD_PLAIN  N^J = wJ sum w D^{J*}_{K1K2} ZRES, all J;
+ *               PNAMP_FOLD_LITERAL the four-term mirror fold of
+ *               Codes/PNAMP/PNAMP_Kernel3D.f90:415-526 (even J). */
+typedef struct {
+  int nalp, nbet, ngam;
+  const double *alp, *walp, *bet, *wbet, *gam, *wgam;
+  int nphi, jmax, fold;
+  double cscale;
+  int iscale;
+} pnamp_mesh;
+
+/* Context: uploads basis, interaction and mesh, builds the Wigner tables on
+ * the device.  Replaces the setup of PNAMP_Main (Codes/PNAMP/PNAMP_Main.f90:24-50). */
+int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *inter, const pnamp_mesh *mesh,
+                 int device, pnamp_ctx **out);
+
+/* HFB states: U, V as [nstates][n*n] column-major.  Computes per state
+ * Z = V U^{-1} and the vacuum overlap det(1 + Z^T Z)^{-1/4} on the device
+ * (zbareover, Codes/PNAMP/PNAMP_Proj.f90:6-79); replaces the .wf reads of
+ * Kernel4Triaxial (PNAMP_Kernel3D.f90:83-122). */
+int pnamp_set_states(pnamp_ctx *ctx, int nstates, const double *U, const double *V);
+
+/* Vacuum overlaps <0|Phi_q> of the states set (Det of zbareover). */
+int pnamp_state_overlaps(pnamp_ctx *ctx, double *det);
+
+/* One pair: Kernel4Triaxial(lpr, Nphi1=q0, Nphi2=q1, NLEV)
+ * (Codes/PNAMP/PNAMP_Kernel3D.f90:1-761, observables norm/Z/N/Ekin/E).
+ * zres may be NULL; kern may be NULL. */
+int pnamp_kernel_pair(pnamp_ctx *ctx, int q0, int q1, int zval, int nval, double *zres,
+                      double *kern);
+
+/* Many pairs in one launch (replaces the pair loop of PNAMP_Main.f90:79-121).
+ * zres: [npairs][n_euler][PNAMP_NOBS] complex or NULL;
+ * kern: [npairs][PNAMP_NOBS][jmax+1][2jmax+1][2jmax+1] complex or NULL. */
+int pnamp_kernels_batched(pnamp_ctx *ctx, int npairs, const int32_t *q0, const int32_t *q1,
+                          int zval, int nval, double *zres, double *kern);
+
+/* Hill-Wheeler solve of one J block: HNDIAG (Codes/GCM/GCM_HNDiag.f90:2-103).
+ * N, H real symmetric dim x dim column-major (H norm-weighted); keeps norm
+ * eigenvalues with EN(i)/EN(1) >= eps, at most nos of them (nos <= 0: all).
+ * Outputs: EN[dim] norm eigenvalues (descending), E[*nkept] energies
+ * (ascending), F (dim x *nkept column-major, may be NULL) amplitudes.
+ * Returns 1 (IFL of the reference) if no norm eigenvalue survives. */
+int pnamp_hw_solve(pnamp_ctx *ctx, int dim, const double *N, const double *H, double eps, int nos,
+                   double *EN, double *E, double *F, int *nkept);
+
+/* Device time of the last kernel call, from HIP events on the context
+ * stream: the mesh-point kernel and the whole call (ms). */
+int pnamp_last_timing(pnamp_ctx *ctx, float *ms_points, float *ms_total);
+
+/* Number of Euler points of the context's mesh. */
+int pnamp_n_euler(pnamp_ctx *ctx);
+
+const char *pnamp_last_error(pnamp_ctx *ctx);
+
+void pnamp_destroy(pnamp_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PNAMP_GPU_H */

@@ -1,0 +1,1180 @@
+// pnamp_gpu.hip -- MI355X (gfx950) implementation of the PNAMP projected-kernel
+// path: FP64 rotation, Pfaffian norm overlap, transition densities, energy
+// kernel, gauge (particle-number) sum, J-projection and Hill-Wheeler solve,
+// behind the C-ABI of include/pnamp_gpu.h.
+//
+// Work decomposition (one GPU):
+//   k_dtables     Wigner d tables: rotation blocks d^j_{m m'}(beta) and
+//                 projection d^J_{K K'}(beta), d^J_{K K'}(pi - beta), per beta.
+//   k_state_prep  per HFB state: U^{-1} (Gauss-Jordan), Z = V U^{-1},
+//                 vacuum overlap det(1 + Z^T Z)^{-1/4} (Cholesky).
+//   k_point       one workgroup per (pair, Euler point); it loops over the
+//                 nphi^2 gauge points of that Euler point and keeps every
+//                 n x n complex intermediate in LDS (five slots of n^2
+//                 complex doubles = 151 KiB at n = 44).  Per gauge point:
+//                 Pfaffian of the 2n x 2n skew matrix (full pivoting, the
+//                 ZPfaffianF algorithm), T = V0^T A V1 + U0^T A* U1, T^{-1}
+//                 (Gauss-Jordan, partial pivoting), rho / kappa10 / kappa01,
+//                 Tr rho per species, E_kin and the two-body energy from the
+//                 m-scheme list.  Writes ZRES(pair, Euler, obs).
+//   k_project     one workgroup per (pair, observable): J-projection sums
+//                 over the Euler mesh (plain or literal fold).
+//   k_hw_*        Hill-Wheeler: Jacobi eigen-solves of N and N^-1/2 H N^-1/2.
+//
+// Everything is FP64.  No CPU fallback: every numerical step of a kernel
+// call runs on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pnamp_gpu.h"
+
+namespace {
+
+constexpr int NMAX = PNAMP_NMAX;      // largest n (m-states, both species)
+constexpr int HMAX = NMAX / 2;        // largest species half
+constexpr int RPACK_MAX = 256;        // packed rotation-block entries per species
+constexpr int NT = 512;               // threads of the mesh-point workgroup
+constexpr int NWAVE = NT / 64;
+constexpr int JMAX_MAX = 12;          // largest projected J supported
+constexpr int NOBS = PNAMP_NOBS;
+
+__constant__ double c_fact[40];
+
+// ------------------------------------------------------------------ complex
+__device__ __forceinline__ double2 cz() { return make_double2(0.0, 0.0); }
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cneg(double2 a) { return make_double2(-a.x, -a.y); }
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 cscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ void cfma(double2 &acc, double2 a, double2 b) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
+}
+__device__ __forceinline__ void rfma(double2 &acc, double a, double2 b) {  // acc += a * b, a real
+  acc.x = fma(a, b.x, acc.x);
+  acc.y = fma(a, b.y, acc.y);
+}
+__device__ __forceinline__ double2 crecip(double2 a) {
+  double d = a.x * a.x + a.y * a.y;
+  return make_double2(a.x / d, -a.y / d);
+}
+__device__ __forceinline__ double cabs1(double2 a) { return fabs(a.x) + fabs(a.y); }
+__device__ __forceinline__ double2 cexpi(double t) {
+  double s, c;
+  sincos(t, &s, &c);
+  return make_double2(c, s);
+}
+
+// d^j_{m1 m2}(beta) = <j m1|exp(-i beta J_y)|j m2>, doubled arguments.
+__device__ double wigner_d_dev(int tj, int tm1, int tm2, double beta) {
+  if (abs(tm1) > tj || abs(tm2) > tj || ((tj + tm1) & 1) || ((tj + tm2) & 1)) return 0.0;
+  const int jpm1 = (tj + tm1) / 2, jmm1 = (tj - tm1) / 2, jpm2 = (tj + tm2) / 2, jmm2 = (tj - tm2) / 2;
+  const double pre = sqrt(c_fact[jpm1] * c_fact[jmm1] * c_fact[jpm2] * c_fact[jmm2]);
+  double s, c;
+  sincos(0.5 * beta, &s, &c);
+  const int dm = (tm1 - tm2) / 2;
+  const int smin = dm < 0 ? -dm : 0;
+  const int smax = jpm2 < jmm1 ? jpm2 : jmm1;
+  double sum = 0.0;
+  for (int k = smin; k <= smax; ++k) {
+    const double den = c_fact[jpm2 - k] * c_fact[k] * c_fact[dm + k] * c_fact[jmm1 - k];
+    double term = 1.0;
+    for (int e = 0; e < tj - dm - 2 * k; ++e) term *= c;
+    for (int e = 0; e < dm + 2 * k; ++e) term *= s;
+    term /= den;
+    sum += ((dm + k) & 1) ? -term : term;
+  }
+  return pre * sum;
+}
+
+// ------------------------------------------------------------------ tables
+struct TableParams {
+  int nbet, h, npack, jmax, npj;
+  const double *bet;
+  const int *pack_a, *pack_b;  // local (species-half) indices of packed entries
+  const int *twoj, *twom;      // species-half basis
+  double *rtab;                // [nbet][npack]
+  double *ptab;                // [nbet][2][npj]
+};
+
+__global__ void k_dtables(TableParams p) {
+  const int ib = blockIdx.x;
+  const double bet = p.bet[ib];
+  for (int e = threadIdx.x; e < p.npack; e += blockDim.x) {
+    const int a = p.pack_a[e], b = p.pack_b[e];
+    p.rtab[(size_t)ib * p.npack + e] = wigner_d_dev(p.twoj[a], p.twom[a], p.twom[b], bet);
+  }
+  for (int e = threadIdx.x; e < p.npj; e += blockDim.x) {
+    int J = 0, off = 0;
+    while (off + (2 * J + 1) * (2 * J + 1) <= e) {
+      off += (2 * J + 1) * (2 * J + 1);
+      ++J;
+    }
+    const int r = e - off, K1 = r / (2 * J + 1) - J, K2 = r % (2 * J + 1) - J;
+    p.ptab[((size_t)ib * 2 + 0) * p.npj + e] = wigner_d_dev(2 * J, 2 * K1, 2 * K2, bet);
+    p.ptab[((size_t)ib * 2 + 1) * p.npj + e] = wigner_d_dev(2 * J, 2 * K1, 2 * K2, M_PI - bet);
+  }
+}
+
+// ------------------------------------------------------------------ state prep
+// One workgroup per state.  zbareover restated for real U, V:
+// Z = V U^{-1}; det(1 + Z^T Z)^{-1/4} equals exp(-sum log(1 + w)/4) over the
+// eigenvalues w of Z^T Z (real symmetric, non-negative).
+struct PrepParams {
+  int n;
+  const double *U, *V;
+  double *Z, *det;
+  int *status;  // [nstates]
+};
+
+__global__ __launch_bounds__(256) void k_state_prep(PrepParams p) {
+  __shared__ double A[NMAX * NMAX];
+  __shared__ double B[NMAX * NMAX];
+  __shared__ double fcol[NMAX];
+  __shared__ int piv;
+  __shared__ double red[4];
+  const int q = blockIdx.x, n = p.n, nn = n * n, tid = threadIdx.x;
+  const double *U = p.U + (size_t)q * nn, *V = p.V + (size_t)q * nn;
+  double *Z = p.Z + (size_t)q * nn;
+  int status = 0;
+  for (int e = tid; e < nn; e += blockDim.x) {
+    A[e] = U[e];
+    B[e] = ((e % n) == (e / n)) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  // Gauss-Jordan with partial pivoting: A -> 1, B -> U^{-1}
+  for (int k = 0; k < n; ++k) {
+    if (tid == 0) {
+      int pr = k;
+      double m = fabs(A[k + k * n]);
+      for (int i = k + 1; i < n; ++i)
+        if (fabs(A[i + k * n]) > m) { m = fabs(A[i + k * n]); pr = i; }
+      piv = pr;
+    }
+    __syncthreads();
+    const int pr = piv;
+    if (pr != k)
+      for (int j = tid; j < 2 * n; j += blockDim.x) {
+        double *M = j < n ? A : B;
+        const int jj = j < n ? j : j - n;
+        const double t = M[k + jj * n];
+        M[k + jj * n] = M[pr + jj * n];
+        M[pr + jj * n] = t;
+      }
+    __syncthreads();
+    const double pv = A[k + k * n];
+    if (pv == 0.0) status = k + 1;
+    const double ip = 1.0 / pv;
+    for (int i = tid; i < n; i += blockDim.x) fcol[i] = (i == k) ? 0.0 : A[i + k * n];
+    for (int j = tid; j < 2 * n; j += blockDim.x) {
+      double *M = j < n ? A : B;
+      const int jj = j < n ? j : j - n;
+      M[k + jj * n] *= ip;
+    }
+    __syncthreads();
+    for (int e = tid; e < 2 * nn; e += blockDim.x) {
+      const int i = e % n, jj = (e / n) % n;
+      double *M = e < nn ? A : B;
+      if (i != k) M[i + jj * n] -= fcol[i] * M[k + jj * n];
+    }
+    __syncthreads();
+  }
+  // Z = V U^{-1} -> global and A
+  for (int e = tid; e < nn; e += blockDim.x) {
+    const int i = e % n, j = e / n;
+    double s = 0.0;
+    for (int k = 0; k < n; ++k) s = fma(V[i + k * n], B[k + j * n], s);
+    Z[e] = s;
+    A[e] = s;
+  }
+  __syncthreads();
+  // B = 1 + Z^T Z
+  for (int e = tid; e < nn; e += blockDim.x) {
+    const int i = e % n, j = e / n;
+    double s = (i == j) ? 1.0 : 0.0;
+    for (int k = 0; k < n; ++k) s = fma(A[k + i * n], A[k + j * n], s);
+    B[e] = s;
+  }
+  __syncthreads();
+  // Cholesky (lower, in B), log det = 2 sum log L_kk
+  double logdet = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const double d = sqrt(B[k + k * n]);
+    logdet += 2.0 * log(d);
+    __syncthreads();
+    for (int i = k + 1 + tid; i < n; i += blockDim.x) B[i + k * n] /= d;
+    __syncthreads();
+    for (int e = tid; e < nn; e += blockDim.x) {
+      const int i = e % n, j = e / n;
+      if (j > k && i >= j) B[i + j * n] -= B[i + k * n] * B[j + k * n];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    p.det[q] = exp(-logdet / 4.0);
+    p.status[q] = status;
+  }
+}
+
+// ------------------------------------------------------------------ mesh points
+struct VEntry {
+  double me;
+  uint32_t idx;  // a | b << 8 | c << 16 | d << 24, 0-based
+  uint32_t pad;
+};
+
+struct PointParams {
+  int n, nalp, nbet, ngam, nphi, zval, nval, npack, NE;
+  int nme;
+  double cscale, vac_scale, snpfaf, e0;
+  const double *alp, *bet, *gam;
+  const double *U, *V, *Z, *det;
+  const double *rtab;               // [nbet][npack]
+  const int *pk_a, *pk_b;           // packed entry -> (a, b) local
+  const int *blk_off, *blk_sz, *blk_base;  // per local m-state
+  const int *twom;                  // per local m-state (doubled m)
+  const int *q0, *q1;               // [npairs]
+  const VEntry *vlist;
+  const double *t1b;
+  double2 *zres;                    // [npairs][NE][NOBS]
+  int *status;                      // [npairs][NE]
+};
+
+// Block-wide argmax of |S(i,j)|^2 over the upper triangle NR <= i < j < N2,
+// ties to the first (i, j) in row-major order, like ZPfaffianF's scan.
+__device__ __forceinline__ void wave_best(double &v, int &k) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(v, off);
+    const int ok = __shfl_xor(k, off);
+    if (ov > v || (ov == v && ok < k)) { v = ov; k = ok; }
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_point(PointParams p) {
+  __shared__ double2 sm[5 * NMAX * NMAX];
+  __shared__ double2 sR[RPACK_MAX];
+  __shared__ double2 fcol[NMAX];
+  __shared__ double redv[NWAVE * 8];
+  __shared__ int redk[NWAVE];
+  __shared__ short m_off[HMAX], m_sz[HMAX], m_base[HMAX], m_twom[HMAX];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int iang = blockIdx.x, ip = blockIdx.y;
+  const int n = p.n, h = n >> 1, nn = n * n, N2 = 2 * n;
+  const int ig = iang % p.ngam, ia = (iang / p.ngam) % p.nalp, ib = iang / (p.ngam * p.nalp);
+  const double alp = p.alp[ia], bet = p.bet[ib], gam = p.gam[ig];
+  const int q0 = p.q0[ip], q1 = p.q1[ip];
+  const double *U0 = p.U + (size_t)q0 * nn, *V0 = p.V + (size_t)q0 * nn;
+  const double *U1 = p.U + (size_t)q1 * nn, *V1 = p.V + (size_t)q1 * nn;
+  const double *Z0 = p.Z + (size_t)q0 * nn, *Z1 = p.Z + (size_t)q1 * nn;
+  const double vac = p.snpfaf * (p.det[q0] * p.vac_scale) * (p.det[q1] * p.vac_scale);
+  (void)bet;
+
+  for (int i = tid; i < h; i += NT) {
+    m_off[i] = (short)p.blk_off[i];
+    m_sz[i] = (short)p.blk_sz[i];
+    m_base[i] = (short)p.blk_base[i];
+    m_twom[i] = (short)p.twom[i];
+  }
+  // R(a, b) = exp(-i alp m_a) d(beta)_{ab} exp(-i gam m_b), packed blocks
+  for (int e = tid; e < p.npack; e += NT) {
+    const int a = p.pk_a[e], b = p.pk_b[e];
+    const double d = p.rtab[(size_t)ib * p.npack + e];
+    const double2 ph = cexpi(-alp * 0.5 * p.twom[a] - gam * 0.5 * p.twom[b]);
+    sR[e] = cscale(d, ph);
+  }
+  __syncthreads();
+
+  double2 *S = sm;                  // 2n x 2n skew matrix (Pfaffian phase)
+  double2 *s0 = sm, *s1 = sm + nn, *s2 = sm + 2 * nn, *s3 = sm + 3 * nn, *s4 = sm + 4 * nn;
+
+  double2 acc[NOBS];
+  for (int o = 0; o < NOBS; ++o) acc[o] = cz();
+  int status = 0;
+  const int nphi = p.nphi;
+  for (int g = 0; g < nphi * nphi; ++g) {
+    const int lp = g / nphi, ln = g % nphi;
+    const double phip = (nphi == 1) ? 0.0 : (2.0 * M_PI / nphi) * lp;
+    const double phin = (nphi == 1) ? 0.0 : (2.0 * M_PI / nphi) * ln;
+    const double2 eph[2] = {cexpi(phip), cexpi(phin)};
+
+    // ---------------------------------------------------------------- Pfaffian
+    // X = A Z1 -> s4 ; A = diag(e^{i phi_p} R, e^{i phi_n} R) (UVphiomega)
+    for (int e = tid; e < nn; e += NT) {
+      const int i = e % n, j = e / n;
+      const int sp = i >= h, il = i - sp * h;
+      const int off = m_off[il], sz = m_sz[il], base = m_base[il];
+      double2 s = cz();
+      for (int k = 0; k < sz; ++k) rfma(s, Z1[(sp * h + off + k) + j * n], sR[base + (il - off) + k * sz]);
+      s4[e] = cmul(eph[sp], s);
+    }
+    __syncthreads();
+    // S = cscale [[A Z1 A^T, -1], [1, -Z0]] (upper triangle; Pfaf_Over with
+    // conj(Vbar) conj(Ubar)^{-1} = A (V1 U1^{-1}) A^T since A is unitary)
+    for (int e = tid; e < N2 * N2; e += NT) {
+      const int i = e % N2, j = e / N2;
+      if (i >= j) continue;
+      double2 v;
+      if (j < n) {
+        const int sp = j >= h, jl = j - sp * h;
+        const int off = m_off[jl], sz = m_sz[jl], base = m_base[jl];
+        double2 s = cz();
+        for (int k = 0; k < sz; ++k) cfma(s, s4[i + (sp * h + off + k) * n], sR[base + (jl - off) + k * sz]);
+        v = cmul(eph[sp], s);
+      } else if (i < n) {
+        v = make_double2((j - n == i) ? -1.0 : 0.0, 0.0);
+      } else {
+        v = make_double2(-Z0[(i - n) + (j - n) * n], 0.0);
+      }
+      S[i + j * N2] = cscale(p.cscale, v);
+    }
+    __syncthreads();
+    double2 pf = make_double2(1.0, 0.0);
+    for (int IB = 0; IB < n - 1; ++IB) {
+      const int NR = 2 * IB, NC = NR + 1, m = N2 - NR;
+      double bv = -1.0;
+      int bk = 0x7fffffff;
+      for (int e = tid; e < m * m; e += NT) {
+        const int i = NR + e % m, j = NR + e / m;
+        if (i < j) {
+          const double2 v = S[i + j * N2];
+          const double a = v.x * v.x + v.y * v.y;
+          const int key = i * N2 + j;
+          if (a > bv || (a == bv && key < bk)) { bv = a; bk = key; }
+        }
+      }
+      wave_best(bv, bk);
+      if (lane == 0) { redv[wave] = bv; redk[wave] = bk; }
+      __syncthreads();
+      bv = redv[0];
+      bk = redk[0];
+      for (int w = 1; w < NWAVE; ++w)
+        if (redv[w] > bv || (redv[w] == bv && redk[w] < bk)) { bv = redv[w]; bk = redk[w]; }
+      const int I1 = bk / N2, I2 = bk % N2;
+      double phas = 1.0;
+      if (I1 == NR) phas = -phas;
+      if (I2 == NC) phas = -phas;
+      // ZExch(NR, I1) then ZExch(NC, I2) on the upper triangle (rows < NR are done)
+      for (int pass = 0; pass < 2; ++pass) {
+        const int i1 = pass ? NC : NR, i2 = pass ? I2 : I1;
+        if (i1 != i2) {
+          for (int i = tid; i < N2; i += NT) {
+            if (i >= NR && i < i1) {
+              const double2 t = S[i + i1 * N2]; S[i + i1 * N2] = S[i + i2 * N2]; S[i + i2 * N2] = t;
+            } else if (i > i2) {
+              const double2 t = S[i1 + i * N2]; S[i1 + i * N2] = S[i2 + i * N2]; S[i2 + i * N2] = t;
+            } else if (i > i1 && i < i2) {
+              const double2 t = S[i1 + i * N2];
+              S[i1 + i * N2] = cneg(S[i + i2 * N2]);
+              S[i + i2 * N2] = cneg(t);
+            } else if (i == i1) {
+              S[i1 + i2 * N2] = cneg(S[i1 + i2 * N2]);
+            }
+          }
+        }
+        __syncthreads();
+      }
+      const double2 ss = S[NR + NC * N2];
+      if (sqrt(ss.x * ss.x + ss.y * ss.y) <= 1.0e-13) {
+        pf = cz();
+        break;
+      }
+      const double2 inv = crecip(ss);
+      const int r0 = NR + 2, m2 = N2 - r0;
+      for (int e = tid; e < m2 * m2; e += NT) {
+        const int i = r0 + e % m2, j = r0 + e / m2;
+        if (i < j) {
+          double2 t = cmul(S[NC + i * N2], S[NR + j * N2]);
+          t = csub(t, cmul(S[NR + i * N2], S[NC + j * N2]));
+          S[i + j * N2] = cadd(S[i + j * N2], cmul(t, inv));
+        }
+      }
+      pf = cscale(phas, cmul(pf, ss));
+      __syncthreads();
+    }
+    if (pf.x != 0.0 || pf.y != 0.0) pf = cmul(pf, S[(N2 - 2) + (N2 - 1) * N2]);
+    const double2 gphase = cexpi(-phip * p.zval - phin * p.nval);
+    const double2 znorm = cmul(cscale(vac, pf), gphase);
+    __syncthreads();
+
+    // ---------------------------------------------------------------- densities
+    // s0 = A V1, s1 = A* U1 (UVphiomega / ROKAPPAphiomega)
+    for (int e = tid; e < nn; e += NT) {
+      const int i = e % n, j = e / n;
+      const int sp = i >= h, il = i - sp * h;
+      const int off = m_off[il], sz = m_sz[il], base = m_base[il];
+      double2 a = cz(), b = cz();
+      for (int k = 0; k < sz; ++k) {
+        const double2 r = sR[base + (il - off) + k * sz];
+        const int row = sp * h + off + k;
+        rfma(a, V1[row + j * n], r);
+        rfma(b, U1[row + j * n], cconj(r));
+      }
+      s0[e] = cmul(eph[sp], a);
+      s1[e] = cmul(cconj(eph[sp]), b);
+    }
+    __syncthreads();
+    // s2 = T = V0^T s0 + U0^T s1 ; s3 = 1
+    for (int t = tid; t < (n / 2) * (n / 2); t += NT) {
+      const int i = 2 * (t % (n / 2)), j = 2 * (t / (n / 2));
+      double2 c00 = cz(), c10 = cz(), c01 = cz(), c11 = cz();
+      for (int k = 0; k < n; ++k) {
+        const double v0 = V0[k + i * n], v1 = V0[k + (i + 1) * n];
+        const double u0 = U0[k + i * n], u1 = U0[k + (i + 1) * n];
+        const double2 a0 = s0[k + j * n], a1 = s0[k + (j + 1) * n];
+        const double2 b0 = s1[k + j * n], b1 = s1[k + (j + 1) * n];
+        rfma(c00, v0, a0); rfma(c00, u0, b0);
+        rfma(c10, v1, a0); rfma(c10, u1, b0);
+        rfma(c01, v0, a1); rfma(c01, u0, b1);
+        rfma(c11, v1, a1); rfma(c11, u1, b1);
+      }
+      s2[i + j * n] = c00; s2[i + 1 + j * n] = c10;
+      s2[i + (j + 1) * n] = c01; s2[i + 1 + (j + 1) * n] = c11;
+    }
+    for (int e = tid; e < nn; e += NT) s3[e] = make_double2((e % n) == (e / n) ? 1.0 : 0.0, 0.0);
+    __syncthreads();
+    // Gauss-Jordan: s2 -> 1, s3 -> T^{-1} (pivot = max |re|+|im|, first maximum)
+    for (int k = 0; k < n; ++k) {
+      if (wave == 0) {
+        double bv = -1.0;
+        int bk = 0x7fffffff;
+        for (int i = k + lane; i < n; i += 64) {
+          const double a = cabs1(s2[i + k * n]);
+          if (a > bv || (a == bv && i < bk)) { bv = a; bk = i; }
+        }
+        wave_best(bv, bk);
+        if (lane == 0) redk[0] = bk;
+      }
+      __syncthreads();
+      const int pr = redk[0];
+      if (pr != k)
+        for (int j = tid; j < 2 * n; j += NT) {
+          double2 *M = j < n ? s2 : s3;
+          const int jj = j < n ? j : j - n;
+          const double2 t = M[k + jj * n];
+          M[k + jj * n] = M[pr + jj * n];
+          M[pr + jj * n] = t;
+        }
+      __syncthreads();
+      const double2 pv = s2[k + k * n];
+      if (pv.x == 0.0 && pv.y == 0.0 && status == 0) status = k + 1;
+      const double2 ipv = crecip(pv);
+      for (int i = tid; i < n; i += NT) fcol[i] = (i == k) ? cz() : s2[i + k * n];
+      for (int j = tid; j < 2 * n; j += NT) {
+        double2 *M = j < n ? s2 : s3;
+        const int jj = j < n ? j : j - n;
+        if (j >= k) M[k + jj * n] = cmul(M[k + jj * n], ipv);
+      }
+      __syncthreads();
+      for (int e = tid; e < 2 * nn; e += NT) {
+        const int i = e % n, jj = (e / n) % n;
+        if (i == k) continue;
+        if (e < nn) {
+          if (jj > k) s2[i + jj * n] = csub(s2[i + jj * n], cmul(fcol[i], s2[k + jj * n]));
+        } else {
+          s3[i + jj * n] = csub(s3[i + jj * n], cmul(fcol[i], s3[k + jj * n]));
+        }
+      }
+      __syncthreads();
+    }
+    // s2 = W1 = (A V1) T^{-1}, s4 = W2 = (A* U1) T^{-1}
+    for (int t = tid; t < (n / 2) * (n / 2); t += NT) {
+      const int i = 2 * (t % (n / 2)), j = 2 * (t / (n / 2));
+      double2 a00 = cz(), a10 = cz(), a01 = cz(), a11 = cz();
+      double2 b00 = cz(), b10 = cz(), b01 = cz(), b11 = cz();
+      for (int k = 0; k < n; ++k) {
+        const double2 x0 = s0[i + k * n], x1 = s0[i + 1 + k * n];
+        const double2 y0 = s1[i + k * n], y1 = s1[i + 1 + k * n];
+        const double2 t0 = s3[k + j * n], t1 = s3[k + (j + 1) * n];
+        cfma(a00, x0, t0); cfma(a10, x1, t0); cfma(a01, x0, t1); cfma(a11, x1, t1);
+        cfma(b00, y0, t0); cfma(b10, y1, t0); cfma(b01, y0, t1); cfma(b11, y1, t1);
+      }
+      s2[i + j * n] = a00; s2[i + 1 + j * n] = a10; s2[i + (j + 1) * n] = a01; s2[i + 1 + (j + 1) * n] = a11;
+      s4[i + j * n] = b00; s4[i + 1 + j * n] = b10; s4[i + (j + 1) * n] = b01; s4[i + 1 + (j + 1) * n] = b11;
+    }
+    __syncthreads();
+    // rho = W1 V0^T -> s0, kappa10 = W1 U0^T -> s1, kappa01 = -W2 V0^T -> s3
+    for (int t = tid; t < (n / 2) * (n / 2); t += NT) {
+      const int i = 2 * (t % (n / 2)), j = 2 * (t / (n / 2));
+      double2 r00 = cz(), r10 = cz(), r01 = cz(), r11 = cz();
+      double2 k00 = cz(), k10 = cz(), k01 = cz(), k11 = cz();
+      double2 l00 = cz(), l10 = cz(), l01 = cz(), l11 = cz();
+      for (int k = 0; k < n; ++k) {
+        const double2 w0 = s2[i + k * n], w1 = s2[i + 1 + k * n];
+        const double2 x0 = s4[i + k * n], x1 = s4[i + 1 + k * n];
+        const double v0 = V0[j + k * n], v1 = V0[j + 1 + k * n];
+        const double u0 = U0[j + k * n], u1 = U0[j + 1 + k * n];
+        rfma(r00, v0, w0); rfma(r10, v0, w1); rfma(r01, v1, w0); rfma(r11, v1, w1);
+        rfma(k00, u0, w0); rfma(k10, u0, w1); rfma(k01, u1, w0); rfma(k11, u1, w1);
+        rfma(l00, v0, x0); rfma(l10, v0, x1); rfma(l01, v1, x0); rfma(l11, v1, x1);
+      }
+      s0[i + j * n] = r00; s0[i + 1 + j * n] = r10; s0[i + (j + 1) * n] = r01; s0[i + 1 + (j + 1) * n] = r11;
+      s1[i + j * n] = k00; s1[i + 1 + j * n] = k10; s1[i + (j + 1) * n] = k01; s1[i + 1 + (j + 1) * n] = k11;
+      s3[i + j * n] = cneg(l00); s3[i + 1 + j * n] = cneg(l10);
+      s3[i + (j + 1) * n] = cneg(l01); s3[i + 1 + (j + 1) * n] = cneg(l11);
+    }
+    __syncthreads();
+
+    // ---------------------------------------------------------------- observables
+    const double2 *ro = s0, *k10 = s1, *k01 = s3;
+    double2 e2 = cz(), ek = cz(), zp = cz(), zn = cz();
+    for (int q = tid; q < p.nme; q += NT) {
+      const VEntry ve = p.vlist[q];
+      const int i = ve.idx & 255, j = (ve.idx >> 8) & 255, k = (ve.idx >> 16) & 255, l = ve.idx >> 24;
+      const double2 rki = ro[k + i * n], rlj = ro[l + j * n], rli = ro[l + i * n], rkj = ro[k + j * n];
+      const double2 rik = ro[i + k * n], rjl = ro[j + l * n], rjk = ro[j + k * n], ril = ro[i + l * n];
+      const double2 pij = k01[i + j * n], pji = k01[j + i * n], pkl = k01[k + l * n], plk = k01[l + k * n];
+      const double2 qkl = k10[k + l * n], qlk = k10[l + k * n], qij = k10[i + j * n], qji = k10[j + i * n];
+      // HFB_Energy_COMPLEX, PNAMP_Energy_OpenMP.f90:40-78
+      const double2 r_ijkl = cadd(csub(cmul(rki, rlj), cmul(rli, rkj)), cmul(pij, qkl));
+      const double2 r_jikl = cadd(csub(cmul(rkj, rli), cmul(rlj, rki)), cmul(pji, qkl));
+      const double2 r_jilk = cadd(csub(cmul(rlj, rki), cmul(rkj, rli)), cmul(pji, qlk));
+      const double2 r_ijlk = cadd(csub(cmul(rli, rkj), cmul(rki, rlj)), cmul(pij, qlk));
+      const double2 r_klij = cadd(csub(cmul(rik, rjl), cmul(rjk, ril)), cmul(pkl, qij));
+      const double2 r_klji = cadd(csub(cmul(rjk, ril), cmul(rik, rjl)), cmul(pkl, qji));
+      const double2 r_lkij = cadd(csub(cmul(ril, rjk), cmul(rjl, rik)), cmul(plk, qij));
+      const double2 r_lkji = cadd(csub(cmul(rjl, rik), cmul(ril, rjk)), cmul(plk, qji));
+      const double d = (k + l == i + j) ? 0.0 : 1.0;
+      double2 s = csub(r_ijkl, r_jikl);
+      s = csub(s, r_ijlk);
+      s = cadd(s, r_jilk);
+      double2 t = csub(r_klij, r_klji);
+      t = csub(t, r_lkij);
+      t = cadd(t, r_lkji);
+      s = cadd(s, cscale(d, t));
+      e2 = cadd(e2, cscale(0.25 * ve.me, s));
+    }
+    for (int e = tid; e < nn; e += NT) {
+      const int a = e % n, c = e / n;  // E_kin = sum rho(c,a) t(a,c)
+      ek = cadd(ek, cscale(p.t1b[a + c * n], ro[c + a * n]));
+    }
+    for (int i = tid; i < h; i += NT) {
+      zp = cadd(zp, ro[i + i * n]);
+      zn = cadd(zn, ro[(i + h) + (i + h) * n]);
+    }
+    double vals[8] = {e2.x, e2.y, ek.x, ek.y, zp.x, zp.y, zn.x, zn.y};
+    for (int v = 0; v < 8; ++v)
+      for (int off = 32; off > 0; off >>= 1) vals[v] += __shfl_xor(vals[v], off);
+    if (lane == 0)
+      for (int v = 0; v < 8; ++v) redv[wave * 8 + v] = vals[v];
+    __syncthreads();
+    if (tid == 0) {
+      double tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int w = 0; w < NWAVE; ++w)
+        for (int v = 0; v < 8; ++v) tot[v] += redv[w * 8 + v];
+      const double2 E2 = make_double2(tot[0], tot[1]), EK = make_double2(tot[2], tot[3]);
+      const double2 ZP = make_double2(tot[4], tot[5]), ZN = make_double2(tot[6], tot[7]);
+      acc[0] = cadd(acc[0], znorm);
+      acc[1] = cadd(acc[1], cmul(znorm, ZP));
+      acc[2] = cadd(acc[2], cmul(znorm, ZN));
+      acc[3] = cadd(acc[3], cmul(znorm, EK));
+      acc[4] = cadd(acc[4], cmul(znorm, cadd(cadd(E2, EK), make_double2(p.e0, 0.0))));
+    }
+    __syncthreads();
+  }
+  // the singular-T flag is per thread; combine through LDS
+  if (lane == 0) redk[wave] = 0;
+  __syncthreads();
+  if (status) redk[wave] = status;  // any nonzero value marks the point
+  __syncthreads();
+  if (tid == 0) {
+    int st = 0;
+    for (int w = 0; w < NWAVE; ++w) st = st ? st : redk[w];
+    const double inv = 1.0 / (double)(nphi * nphi);
+    double2 *out = p.zres + ((size_t)ip * p.NE + iang) * NOBS;
+    for (int o = 0; o < NOBS; ++o) out[o] = cscale(inv, acc[o]);
+    p.status[(size_t)ip * p.NE + iang] = st;
+  }
+}
+
+// ------------------------------------------------------------------ projection
+struct ProjParams {
+  int nalp, nbet, ngam, NE, jmax, fold, npj;
+  const double *alp, *walp, *wbet, *gam, *wgam;
+  const double *ptab;        // [nbet][2][npj]
+  const double2 *zres;       // [npairs][NE][NOBS]
+  const int *status;         // [npairs][NE]
+  double2 *kern;             // [npairs][NOBS][jmax+1][2jmax+1][2jmax+1]
+  int *pair_status;          // [npairs]
+};
+
+__global__ __launch_bounds__(256) void k_project(ProjParams p) {
+  const int ip = blockIdx.x, io = blockIdx.y, tid = threadIdx.x;
+  const int J2m = 2 * p.jmax + 1;
+  const double2 *z = p.zres + (size_t)ip * p.NE * NOBS;
+  double2 *kout = p.kern + ((size_t)ip * NOBS + io) * (p.jmax + 1) * J2m * J2m;
+  const bool onept = (p.nalp * p.ngam == 1);
+  const int ntot = (p.jmax + 1) * J2m * J2m;
+  for (int o = tid; o < ntot; o += blockDim.x) {
+    const int J = o / (J2m * J2m), K1 = (o / J2m) % J2m - p.jmax, K2 = o % J2m - p.jmax;
+    if (K1 < -J || K1 > J || K2 < -J || K2 > J) {
+      kout[o] = cz();
+      continue;
+    }
+    // compact d-table index: sum_{J'<J} (2J'+1)^2 = J (4J^2 - 1) / 3
+    const int e = J * (4 * J * J - 1) / 3 + (K1 + J) * (2 * J + 1) + (K2 + J);
+    double2 acc = cz();
+    if (!(p.fold == PNAMP_FOLD_LITERAL && (J & 1))) {
+      const double wJ = onept ? (2.0 * J + 1.0) / 2.0
+                              : (p.fold == PNAMP_FOLD_LITERAL ? 2.0 : 1.0) * (2.0 * J + 1.0) / (8.0 * M_PI * M_PI);
+      const double d0 = (K1 == 0 ? 2.0 : 1.0) * (K2 == 0 ? 2.0 : 1.0);
+      for (int ib = 0; ib < p.nbet; ++ib) {
+        const double dp = p.ptab[((size_t)ib * 2 + 0) * p.npj + e];
+        const double dm = p.ptab[((size_t)ib * 2 + 1) * p.npj + e];
+        for (int ia = 0; ia < p.nalp; ++ia)
+          for (int ig = 0; ig < p.ngam; ++ig) {
+            const double w = p.walp[ia] * p.wbet[ib] * p.wgam[ig];
+            const int i1 = (ib * p.nalp + ia) * p.ngam + ig;
+            if (p.fold == PNAMP_FOLD_PLAIN) {
+              const double2 D = cscale(dp * w * wJ, cexpi(K1 * p.alp[ia] + K2 * p.gam[ig]));
+              cfma(acc, D, z[(size_t)i1 * NOBS + io]);
+            } else {
+              const int i2 = (ib * p.nalp + ia) * p.ngam + (p.ngam - 1 - ig);
+              const int i3 = (ib * p.nalp + (p.nalp - 1 - ia)) * p.ngam + ig;
+              const int i4 = (ib * p.nalp + (p.nalp - 1 - ia)) * p.ngam + (p.ngam - 1 - ig);
+              const double2 zap = cexpi(K1 * p.alp[ia]), zam = cexpi(-K1 * p.alp[ia]);
+              const double2 zgp = cexpi(K2 * p.gam[ig]);
+              const double2 za1 = cscale(dp, cmul(zap, zgp)), za2 = cscale(dp, cmul(zam, zgp));
+              const double2 za3 = cscale(dm, cmul(zap, zgp)), za4 = cscale(dm, cmul(zam, zgp));
+              double2 s = cmul(za1, z[(size_t)i1 * NOBS + io]);
+              s = cadd(s, cmul(za2, cconj(z[(size_t)i2 * NOBS + io])));
+              s = cadd(s, cmul(za3, cconj(z[(size_t)i3 * NOBS + io])));
+              s = cadd(s, cmul(za4, z[(size_t)i4 * NOBS + io]));
+              acc = cadd(acc, cscale(wJ * w / d0, s));
+            }
+          }
+      }
+    }
+    kout[o] = acc;
+  }
+  if (io == 0) {
+    __shared__ int st[256];
+    int s = 0;
+    for (int i = tid; i < p.NE; i += blockDim.x) {
+      const int v = p.status[(size_t)ip * p.NE + i];
+      if (v && !s) s = v;
+    }
+    st[tid] = s;
+    __syncthreads();
+    if (tid == 0) {
+      int r = 0;
+      for (int i = 0; i < (int)blockDim.x; ++i)
+        if (st[i]) { r = st[i]; break; }
+      p.pair_status[ip] = r;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ Hill-Wheeler
+// Cyclic Jacobi on a real symmetric matrix in global memory, one workgroup,
+// round-robin pairing so that dim/2 rotations run in parallel per step.
+// Eigenvectors accumulate in Vv.  Ends after a sweep whose off-diagonal norm
+// is below 1e-30 of the total, or after 60 sweeps.
+__global__ __launch_bounds__(256) void k_jacobi(double *A, double *Vv, int dim, int *nsweeps) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  __shared__ double cs[512], sn[512];
+  __shared__ int pp[512], qq[512];
+  __shared__ double red[256];
+  __shared__ int done;
+  const int m = dim + (dim & 1);  // even number of slots, slot dim is a dummy
+  for (int e = tid; e < dim * dim; e += nt) Vv[e] = ((e % dim) == (e / dim)) ? 1.0 : 0.0;
+  __syncthreads();
+  int sweep = 0;
+  for (sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int e = tid; e < dim * dim; e += nt) {
+      const double x = A[e] * A[e];
+      tot += x;
+      if ((e % dim) != (e / dim)) off += x;
+    }
+    red[tid] = off;
+    __syncthreads();
+    if (tid == 0) {
+      double o = 0.0;
+      for (int i = 0; i < nt; ++i) o += red[i];
+      red[0] = o;
+    }
+    __syncthreads();
+    const double offs = red[0];
+    __syncthreads();
+    red[tid] = tot;
+    __syncthreads();
+    if (tid == 0) {
+      double o = 0.0;
+      for (int i = 0; i < nt; ++i) o += red[i];
+      done = (offs <= 1e-30 * o || offs == 0.0);
+    }
+    __syncthreads();
+    if (done) break;
+    for (int step = 0; step < m - 1; ++step) {
+      // round-robin pairs: slot 0 fixed, others rotate
+      for (int k = tid; k < m / 2; k += nt) {
+        // circle method: slot 0 fixed, slots 1..m-1 on a circle of odd length
+        int a, b;
+        if (k == 0) {
+          a = 0;
+          b = 1 + step % (m - 1);
+        } else {
+          a = 1 + (step + k) % (m - 1);
+          b = 1 + (step - k + (m - 1)) % (m - 1);
+        }
+        const int P = a < b ? a : b, Q = a < b ? b : a;
+        double c = 1.0, s = 0.0;
+        if (Q < dim) {
+          const double apq = A[P + Q * dim];
+          if (fabs(apq) > 1e-300) {
+            const double th = (A[Q + Q * dim] - A[P + P * dim]) / (2.0 * apq);
+            const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+            c = 1.0 / sqrt(t * t + 1.0);
+            s = t * c;
+          }
+        }
+        pp[k] = P; qq[k] = Q; cs[k] = c; sn[k] = s;
+      }
+      __syncthreads();
+      // columns: A = A J, V = V J
+      for (int e = tid; e < (m / 2) * dim; e += nt) {
+        const int k = e / dim, r = e % dim, P = pp[k], Q = qq[k];
+        if (Q >= dim) continue;
+        const double c = cs[k], s = sn[k];
+        const double ap = A[r + P * dim], aq = A[r + Q * dim];
+        A[r + P * dim] = c * ap - s * aq;
+        A[r + Q * dim] = s * ap + c * aq;
+        const double vp = Vv[r + P * dim], vq = Vv[r + Q * dim];
+        Vv[r + P * dim] = c * vp - s * vq;
+        Vv[r + Q * dim] = s * vp + c * vq;
+      }
+      __syncthreads();
+      // rows: A = J^T A
+      for (int e = tid; e < (m / 2) * dim; e += nt) {
+        const int k = e / dim, r = e % dim, P = pp[k], Q = qq[k];
+        if (Q >= dim) continue;
+        const double c = cs[k], s = sn[k];
+        const double ap = A[P + r * dim], aq = A[Q + r * dim];
+        A[P + r * dim] = c * ap - s * aq;
+        A[Q + r * dim] = s * ap + c * aq;
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0) *nsweeps = sweep;
+}
+
+// W = R^T H R for R (dim x M) and symmetric H (dim x dim): one thread per (j, k)
+__global__ void k_congruence(const double *H, const double *R, double *W, int dim, int M) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= M * M) return;
+  const int j = e % M, k = e / M;
+  double s = 0.0;
+  for (int l = 0; l < dim; ++l) {
+    double hr = 0.0;
+    for (int i = 0; i < dim; ++i) hr = fma(H[l + i * dim], R[i + k * dim], hr);
+    s = fma(R[l + j * dim], hr, s);
+  }
+  W[j + k * M] = s;
+}
+
+__global__ void k_scale_cols(const double *D, const double *w, double *R, int dim, int M) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= dim * M) return;
+  const int i = e % dim, k = e / dim;
+  R[i + k * dim] = D[i + k * dim] / sqrt(w[k]);
+}
+
+__global__ void k_matmul(const double *A, const double *B, double *C, int m, int kk, int nc) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m * nc) return;
+  const int i = e % m, j = e / m;
+  double s = 0.0;
+  for (int l = 0; l < kk; ++l) s = fma(A[i + l * m], B[l + j * kk], s);
+  C[i + j * m] = s;
+}
+
+}  // namespace
+
+// ====================================================================== host
+struct pnamp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  float ms_points = 0.f, ms_total = 0.f;
+  std::string err;
+  // basis
+  int n = 0, h = 0, npack = 0;
+  std::vector<int> twom_h;
+  // mesh
+  int nalp = 0, nbet = 0, ngam = 0, nphi = 0, jmax = 0, fold = 0, NE = 0, npj = 0;
+  double cscale = 1.0, vac_scale = 1.0, snpfaf = 1.0, e0 = 0.0;
+  int nme = 0;
+  int nstates = 0;
+  int pair_cap = 0;
+  // device buffers
+  double *d_alp = nullptr, *d_walp = nullptr, *d_bet = nullptr, *d_wbet = nullptr, *d_gam = nullptr,
+         *d_wgam = nullptr;
+  double *d_rtab = nullptr, *d_ptab = nullptr;
+  int *d_pk_a = nullptr, *d_pk_b = nullptr, *d_blk_off = nullptr, *d_blk_sz = nullptr,
+      *d_blk_base = nullptr, *d_twoj = nullptr, *d_twom = nullptr;
+  VEntry *d_vlist = nullptr;
+  double *d_t1b = nullptr;
+  double *d_U = nullptr, *d_V = nullptr, *d_Z = nullptr, *d_det = nullptr;
+  int *d_state_status = nullptr;
+  int *d_q0 = nullptr, *d_q1 = nullptr;
+  double2 *d_zres = nullptr, *d_kern = nullptr;
+  int *d_status = nullptr, *d_pair_status = nullptr;
+};
+
+#define HIPCHK(ctx, x)                                                       \
+  do {                                                                       \
+    hipError_t e_ = (x);                                                     \
+    if (e_ != hipSuccess) {                                                  \
+      (ctx)->err = std::string(#x) + ": " + hipGetErrorString(e_);           \
+      return PNAMP_E_DEVICE;                                                 \
+    }                                                                        \
+  } while (0)
+
+static int fail(pnamp_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+template <class T>
+static int dalloc(pnamp_ctx *c, T **p, size_t count) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (count == 0) count = 1;
+  HIPCHK(c, hipMalloc((void **)p, count * sizeof(T)));
+  return 0;
+}
+
+template <class T>
+static int dupload(pnamp_ctx *c, T **p, const T *src, size_t count) {
+  int r = dalloc(c, p, count);
+  if (r) return r;
+  if (count) HIPCHK(c, hipMemcpy(*p, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *inter,
+                            const pnamp_mesh *mesh, int device, pnamp_ctx **out) {
+  if (!out) return PNAMP_E_ARG;
+  *out = nullptr;
+  if (!basis || !inter || !mesh) return PNAMP_E_ARG;
+  pnamp_ctx *c = new pnamp_ctx();
+  auto bail = [&](int code, const std::string &m) {
+    c->err = m;
+    *out = c;  // keep the context so the caller can read the message
+    return code;
+  };
+  const int n = basis->n;
+  if (n <= 0 || (n & 1)) return bail(PNAMP_E_ARG, "n must be positive and even");
+  if (n > NMAX) return bail(PNAMP_E_SIZE, "n exceeds PNAMP_NMAX");
+  const int h = n / 2;
+  for (int i = 0; i < n; ++i)
+    if (basis->t[i] != (i < h ? -1 : 1)) return bail(PNAMP_E_ARG, "basis must list protons (t=-1) then neutrons (t=+1)");
+  for (int i = 0; i < h; ++i)
+    if (basis->nq[i] != basis->nq[i + h] || basis->l[i] != basis->l[i + h] || basis->twoj[i] != basis->twoj[i + h] ||
+        basis->twom[i] != basis->twom[i + h])
+      return bail(PNAMP_E_ARG, "proton and neutron halves must have the same orbits");
+  // rotation blocks: contiguous runs of equal (n, l, j), m = j..-j
+  std::vector<int> off(h), sz(h), base(h), pa, pb;
+  for (int i = 0; i < h;) {
+    const int tj = basis->twoj[i];
+    const int s = tj + 1;
+    if (i + s > h) return bail(PNAMP_E_ARG, "incomplete orbit block");
+    for (int k = 0; k < s; ++k) {
+      if (basis->twoj[i + k] != tj || basis->nq[i + k] != basis->nq[i] || basis->l[i + k] != basis->l[i] ||
+          basis->twom[i + k] != tj - 2 * k)
+        return bail(PNAMP_E_ARG, "orbit block must run m = j .. -j");
+      off[i + k] = i;
+      sz[i + k] = s;
+      base[i + k] = (int)pa.size();
+    }
+    for (int bcol = 0; bcol < s; ++bcol)
+      for (int arow = 0; arow < s; ++arow) {
+        pa.push_back(i + arow);
+        pb.push_back(i + bcol);
+      }
+    i += s;
+  }
+  if ((int)pa.size() > RPACK_MAX) return bail(PNAMP_E_SIZE, "rotation blocks exceed the packed LDS table");
+  if (mesh->nalp < 1 || mesh->nbet < 1 || mesh->ngam < 1 || mesh->nphi < 1 || mesh->jmax < 0)
+    return bail(PNAMP_E_ARG, "bad mesh sizes");
+  if (mesh->jmax > JMAX_MAX) return bail(PNAMP_E_SIZE, "jmax too large");
+  if (mesh->fold != PNAMP_FOLD_PLAIN && mesh->fold != PNAMP_FOLD_LITERAL) return bail(PNAMP_E_ARG, "bad fold");
+  if (inter->nme < 0 || inter->nme > (1 << 30)) return bail(PNAMP_E_ARG, "bad interaction size");
+  c->device = device;
+  HIPCHK(c, hipSetDevice(device));
+  HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto &e : c->ev) HIPCHK(c, hipEventCreate(&e));
+  double fact[40];
+  fact[0] = 1.0;
+  for (int i = 1; i < 40; ++i) fact[i] = fact[i - 1] * i;
+  HIPCHK(c, hipMemcpyToSymbol(HIP_SYMBOL(c_fact), fact, sizeof(fact)));
+  c->n = n;
+  c->h = h;
+  c->npack = (int)pa.size();
+  c->twom_h.assign(basis->twom, basis->twom + h);
+  c->nalp = mesh->nalp;
+  c->nbet = mesh->nbet;
+  c->ngam = mesh->ngam;
+  c->nphi = mesh->nphi;
+  c->jmax = mesh->jmax;
+  c->fold = mesh->fold;
+  c->NE = mesh->nalp * mesh->nbet * mesh->ngam;
+  c->npj = 0;
+  for (int J = 0; J <= c->jmax; ++J) c->npj += (2 * J + 1) * (2 * J + 1);
+  c->cscale = mesh->cscale;
+  c->vac_scale = std::pow(10.0, (double)(-(n * mesh->iscale) / 2));
+  const long aaa = (long)n * (n + 1) / 2;
+  c->snpfaf = (aaa % 2) ? -1.0 : 1.0;
+  c->e0 = inter->e0;
+  c->nme = (int)inter->nme;
+  int r = 0;
+  std::vector<double> bet(mesh->bet, mesh->bet + mesh->nbet), wbet(mesh->wbet, mesh->wbet + mesh->nbet);
+  if ((r = dupload(c, &c->d_alp, mesh->alp, c->nalp)) || (r = dupload(c, &c->d_walp, mesh->walp, c->nalp)) ||
+      (r = dupload(c, &c->d_bet, bet.data(), c->nbet)) || (r = dupload(c, &c->d_wbet, wbet.data(), c->nbet)) ||
+      (r = dupload(c, &c->d_gam, mesh->gam, c->ngam)) || (r = dupload(c, &c->d_wgam, mesh->wgam, c->ngam)))
+    return bail(r, c->err);
+  if ((r = dupload(c, &c->d_pk_a, pa.data(), pa.size())) || (r = dupload(c, &c->d_pk_b, pb.data(), pb.size())) ||
+      (r = dupload(c, &c->d_blk_off, off.data(), h)) || (r = dupload(c, &c->d_blk_sz, sz.data(), h)) ||
+      (r = dupload(c, &c->d_blk_base, base.data(), h)) ||
+      (r = dupload(c, &c->d_twoj, (const int *)basis->twoj, h)) ||
+      (r = dupload(c, &c->d_twom, (const int *)basis->twom, h)))
+    return bail(r, c->err);
+  std::vector<VEntry> vl(inter->nme);
+  for (int64_t q = 0; q < inter->nme; ++q) {
+    const int a = inter->ka[q] - 1, b = inter->kb[q] - 1, cc = inter->kc[q] - 1, d = inter->kd[q] - 1;
+    if (a < 0 || b < 0 || cc < 0 || d < 0 || a >= n || b >= n || cc >= n || d >= n)
+      return bail(PNAMP_E_ARG, "interaction index out of range");
+    vl[q].me = inter->me2b[q];
+    vl[q].idx = (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)cc << 16) | ((uint32_t)d << 24);
+    vl[q].pad = 0;
+  }
+  if ((r = dupload(c, &c->d_vlist, vl.data(), vl.size())) || (r = dupload(c, &c->d_t1b, inter->t1b, (size_t)n * n)))
+    return bail(r, c->err);
+  if ((r = dalloc(c, &c->d_rtab, (size_t)c->nbet * c->npack)) || (r = dalloc(c, &c->d_ptab, (size_t)c->nbet * 2 * c->npj)))
+    return bail(r, c->err);
+  TableParams tp{c->nbet, h, c->npack, c->jmax, c->npj, c->d_bet, c->d_pk_a, c->d_pk_b, c->d_twoj, c->d_twom,
+                 c->d_rtab, c->d_ptab};
+  hipLaunchKernelGGL(k_dtables, dim3(c->nbet), dim3(256), 0, c->stream, tp);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *out = c;
+  return PNAMP_OK;
+}
+
+extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, const double *V) {
+  if (!c || !U || !V || nstates <= 0) return PNAMP_E_ARG;
+  const size_t nn = (size_t)c->n * c->n;
+  int r = 0;
+  if ((r = dupload(c, &c->d_U, U, nn * nstates)) || (r = dupload(c, &c->d_V, V, nn * nstates)) ||
+      (r = dalloc(c, &c->d_Z, nn * nstates)) || (r = dalloc(c, &c->d_det, (size_t)nstates)) ||
+      (r = dalloc(c, &c->d_state_status, (size_t)nstates)))
+    return r;
+  PrepParams pp{c->n, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status};
+  hipLaunchKernelGGL(k_state_prep, dim3(nstates), dim3(256), 0, c->stream, pp);
+  HIPCHK(c, hipGetLastError());
+  std::vector<int> st(nstates);
+  HIPCHK(c, hipMemcpyAsync(st.data(), c->d_state_status, sizeof(int) * nstates, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->nstates = nstates;
+  for (int q = 0; q < nstates; ++q)
+    if (st[q]) {
+      c->nstates = 0;
+      return fail(c, st[q], "U of state " + std::to_string(q) + " is singular (zbareover ZGETRF INFO)");
+    }
+  return PNAMP_OK;
+}
+
+extern "C" int pnamp_state_overlaps(pnamp_ctx *c, double *det) {
+  if (!c || !det) return PNAMP_E_ARG;
+  if (!c->nstates) return fail(c, PNAMP_E_STATE, "no states set");
+  HIPCHK(c, hipMemcpy(det, c->d_det, sizeof(double) * c->nstates, hipMemcpyDeviceToHost));
+  return PNAMP_OK;
+}
+
+static int ensure_pairs(pnamp_ctx *c, int np) {
+  if (np <= c->pair_cap) return 0;
+  int r = 0;
+  const size_t ko = (size_t)NOBS * (c->jmax + 1) * (2 * c->jmax + 1) * (2 * c->jmax + 1);
+  if ((r = dalloc(c, &c->d_q0, np)) || (r = dalloc(c, &c->d_q1, np)) ||
+      (r = dalloc(c, &c->d_zres, (size_t)np * c->NE * NOBS)) || (r = dalloc(c, &c->d_kern, (size_t)np * ko)) ||
+      (r = dalloc(c, &c->d_status, (size_t)np * c->NE)) || (r = dalloc(c, &c->d_pair_status, (size_t)np)))
+    return r;
+  c->pair_cap = np;
+  return 0;
+}
+
+extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0, const int32_t *q1, int zval,
+                                     int nval, double *zres, double *kern) {
+  if (!c || npairs <= 0 || !q0 || !q1) return PNAMP_E_ARG;
+  if (!c->nstates) return fail(c, PNAMP_E_STATE, "no states set");
+  for (int i = 0; i < npairs; ++i)
+    if (q0[i] < 0 || q1[i] < 0 || q0[i] >= c->nstates || q1[i] >= c->nstates)
+      return fail(c, PNAMP_E_ARG, "pair index out of range");
+  if (npairs > 65535) return fail(c, PNAMP_E_SIZE, "at most 65535 pairs per call");
+  int r = ensure_pairs(c, npairs);
+  if (r) return r;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(c->d_q0, q0, sizeof(int) * npairs, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_q1, q1, sizeof(int) * npairs, hipMemcpyHostToDevice, c->stream));
+  PointParams pp;
+  pp.n = c->n; pp.nalp = c->nalp; pp.nbet = c->nbet; pp.ngam = c->ngam; pp.nphi = c->nphi;
+  pp.zval = zval; pp.nval = nval; pp.npack = c->npack; pp.NE = c->NE; pp.nme = c->nme;
+  pp.cscale = c->cscale; pp.vac_scale = c->vac_scale; pp.snpfaf = c->snpfaf; pp.e0 = c->e0;
+  pp.alp = c->d_alp; pp.bet = c->d_bet; pp.gam = c->d_gam;
+  pp.U = c->d_U; pp.V = c->d_V; pp.Z = c->d_Z; pp.det = c->d_det;
+  pp.rtab = c->d_rtab; pp.pk_a = c->d_pk_a; pp.pk_b = c->d_pk_b;
+  pp.blk_off = c->d_blk_off; pp.blk_sz = c->d_blk_sz; pp.blk_base = c->d_blk_base; pp.twom = c->d_twom;
+  pp.q0 = c->d_q0; pp.q1 = c->d_q1; pp.vlist = c->d_vlist; pp.t1b = c->d_t1b;
+  pp.zres = c->d_zres; pp.status = c->d_status;
+  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  hipLaunchKernelGGL(k_point, dim3(c->NE, npairs), dim3(NT), 0, c->stream, pp);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  ProjParams pr{c->nalp, c->nbet, c->ngam, c->NE, c->jmax, c->fold, c->npj, c->d_alp, c->d_walp, c->d_wbet,
+                c->d_gam, c->d_wgam, c->d_ptab, c->d_zres, c->d_status, c->d_kern, c->d_pair_status};
+  hipLaunchKernelGGL(k_project, dim3(npairs, NOBS), dim3(256), 0, c->stream, pr);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  const size_t kper = (size_t)NOBS * (c->jmax + 1) * (2 * c->jmax + 1) * (2 * c->jmax + 1);
+  if (kern)  // k_project writes every slot of the dense layout, zeros where |K| > J
+    HIPCHK(c, hipMemcpyAsync(kern, c->d_kern, sizeof(double2) * kper * npairs, hipMemcpyDeviceToHost, c->stream));
+  if (zres)
+    HIPCHK(c, hipMemcpyAsync(zres, c->d_zres, sizeof(double2) * (size_t)npairs * c->NE * NOBS,
+                             hipMemcpyDeviceToHost, c->stream));
+  std::vector<int> pst(npairs);
+  HIPCHK(c, hipMemcpyAsync(pst.data(), c->d_pair_status, sizeof(int) * npairs, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipEventElapsedTime(&c->ms_points, c->ev[0], c->ev[1]));
+  HIPCHK(c, hipEventElapsedTime(&c->ms_total, c->ev[0], c->ev[3]));
+  for (int i = 0; i < npairs; ++i)
+    if (pst[i]) return fail(c, pst[i], "singular T (ROKAPPAphiomega) in pair " + std::to_string(i));
+  return PNAMP_OK;
+}
+
+extern "C" int pnamp_kernel_pair(pnamp_ctx *c, int q0, int q1, int zval, int nval, double *zres, double *kern) {
+  const int32_t a = q0, b = q1;
+  return pnamp_kernels_batched(c, 1, &a, &b, zval, nval, zres, kern);
+}
+
+extern "C" int pnamp_hw_solve(pnamp_ctx *c, int dim, const double *N, const double *H, double eps, int nos,
+                              double *EN, double *E, double *F, int *nkept) {
+  if (!c || dim <= 0 || !N || !H || !EN || !E || !nkept) return PNAMP_E_ARG;
+  if (dim > 1022) return fail(c, PNAMP_E_SIZE, "HW dimension above 1022");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t dd = (size_t)dim * dim;
+  double *dA = nullptr, *dV = nullptr, *dH = nullptr, *dR = nullptr, *dW = nullptr, *dG = nullptr, *dw = nullptr,
+         *dF = nullptr;
+  int *dsw = nullptr;
+  struct Guard {
+    std::vector<void *> p;
+    ~Guard() {
+      for (void *x : p)
+        if (x) (void)hipFree(x);
+    }
+  } guard;
+  auto al = [&](double **p, size_t cnt) -> int {
+    hipError_t e = hipMalloc((void **)p, cnt * sizeof(double));
+    guard.p.push_back(*p);
+    return e == hipSuccess ? 0 : PNAMP_E_DEVICE;
+  };
+  if (al(&dA, dd) || al(&dV, dd) || al(&dH, dd) || al(&dR, dd) || al(&dW, dd) || al(&dG, dd) || al(&dw, dim) ||
+      al(&dF, dd))
+    return fail(c, PNAMP_E_DEVICE, "hipMalloc failed in pnamp_hw_solve");
+  HIPCHK(c, hipMalloc((void **)&dsw, sizeof(int)));
+  guard.p.push_back(dsw);
+  // HNDIAG: diagonalise -N
+  std::vector<double> negN(dd);
+  for (size_t i = 0; i < dd; ++i) negN[i] = -N[i];
+  HIPCHK(c, hipMemcpyAsync(dA, negN.data(), sizeof(double) * dd, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dH, H, sizeof(double) * dd, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(256), 0, c->stream, dA, dV, dim, dsw);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> A(dd), Vv(dd);
+  HIPCHK(c, hipMemcpyAsync(A.data(), dA, sizeof(double) * dd, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(Vv.data(), dV, sizeof(double) * dd, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // order: ascending eigenvalues of -N == descending EN
+  std::vector<int> ord(dim);
+  for (int i = 0; i < dim; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return A[a + (size_t)a * dim] < A[b + (size_t)b * dim]; });
+  for (int i = 0; i < dim; ++i) EN[i] = -A[ord[i] + (size_t)ord[i] * dim];
+  int M = 0;
+  while (M < dim && EN[M] / EN[0] >= eps) ++M;
+  if (nos > 0 && nos < M) M = nos;
+  *nkept = M;
+  if (M <= 0) return 1;
+  std::vector<double> D((size_t)dim * M), w(M);
+  for (int k = 0; k < M; ++k) {
+    w[k] = EN[k];
+    for (int i = 0; i < dim; ++i) D[i + (size_t)k * dim] = Vv[i + (size_t)ord[k] * dim];
+  }
+  HIPCHK(c, hipMemcpyAsync(dR, D.data(), sizeof(double) * D.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dw, w.data(), sizeof(double) * M, hipMemcpyHostToDevice, c->stream));
+  // RR = DD / sqrt(EN)  (in place into dW), GG = RR^T H RR
+  hipLaunchKernelGGL(k_scale_cols, dim3((dim * M + 255) / 256), dim3(256), 0, c->stream, dR, dw, dW, dim, M);
+  hipLaunchKernelGGL(k_congruence, dim3((M * M + 255) / 256), dim3(256), 0, c->stream, dH, dW, dG, dim, M);
+  hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(256), 0, c->stream, dG, dA, M, dsw);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> G((size_t)M * M), gv((size_t)M * M);
+  HIPCHK(c, hipMemcpyAsync(G.data(), dG, sizeof(double) * G.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(gv.data(), dA, sizeof(double) * gv.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<int> o2(M);
+  for (int i = 0; i < M; ++i) o2[i] = i;
+  std::stable_sort(o2.begin(), o2.end(), [&](int a, int b) { return G[a + (size_t)a * M] < G[b + (size_t)b * M]; });
+  for (int k = 0; k < M; ++k) E[k] = G[o2[k] + (size_t)o2[k] * M];
+  if (F) {
+    std::vector<double> gs((size_t)M * M);
+    for (int k = 0; k < M; ++k)
+      for (int l = 0; l < M; ++l) gs[l + (size_t)k * M] = gv[l + (size_t)o2[k] * M];
+    HIPCHK(c, hipMemcpyAsync(dG, gs.data(), sizeof(double) * gs.size(), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_matmul, dim3((dim * M + 255) / 256), dim3(256), 0, c->stream, dW, dG, dF, dim, M, M);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(F, dF, sizeof(double) * (size_t)dim * M, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return PNAMP_OK;
+}
+
+extern "C" int pnamp_last_timing(pnamp_ctx *c, float *ms_points, float *ms_total) {
+  if (!c) return PNAMP_E_ARG;
+  if (ms_points) *ms_points = c->ms_points;
+  if (ms_total) *ms_total = c->ms_total;
+  return PNAMP_OK;
+}
+
+extern "C" int pnamp_n_euler(pnamp_ctx *c) { return c ? c->NE : PNAMP_E_ARG; }
+
+extern "C" const char *pnamp_last_error(pnamp_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" void pnamp_destroy(pnamp_ctx *c) {
+  if (!c) return;
+  void *bufs[] = {c->d_alp, c->d_walp, c->d_bet, c->d_wbet, c->d_gam, c->d_wgam, c->d_rtab, c->d_ptab,
+                  c->d_pk_a, c->d_pk_b, c->d_blk_off, c->d_blk_sz, c->d_blk_base, c->d_twoj, c->d_twom,
+                  c->d_vlist, c->d_t1b, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status, c->d_q0,
+                  c->d_q1, c->d_zres, c->d_kern, c->d_status, c->d_pair_status};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  for (auto e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}

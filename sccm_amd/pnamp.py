@@ -1,0 +1,201 @@
+"""Host front end of the HIP library ``libpnamp_gpu.so`` (C-ABI in
+``include/pnamp_gpu.h``).
+
+Mirrors the reference's per-pair kernel step: ``Kernel4Triaxial``
+(``Codes/PNAMP/PNAMP_Kernel3D.f90:1``) for one pair, the pair loop of
+``PNAMP_Main`` (``Codes/PNAMP/PNAMP_Main.f90:79-121``) as one batched call, and
+``HNDIAG`` (``Codes/GCM/GCM_HNDiag.f90:2``).  There is no CPU path: if the
+library is missing or fails to load, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpnamp_gpu.so")
+NOBS = 5
+OBS = ("norm", "Z", "N", "Ekin", "E")
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+
+class _Basis(C.Structure):
+    _fields_ = [("n", C.c_int), ("t", _i32p), ("nq", _i32p), ("l", _i32p), ("twoj", _i32p), ("twom", _i32p)]
+
+
+class _Inter(C.Structure):
+    _fields_ = [("nme", C.c_int64), ("ka", _i32p), ("kb", _i32p), ("kc", _i32p), ("kd", _i32p),
+                ("me2b", _f64p), ("t1b", _f64p), ("e0", C.c_double)]
+
+
+class _Mesh(C.Structure):
+    _fields_ = [("nalp", C.c_int), ("nbet", C.c_int), ("ngam", C.c_int),
+                ("alp", _f64p), ("walp", _f64p), ("bet", _f64p), ("wbet", _f64p), ("gam", _f64p), ("wgam", _f64p),
+                ("nphi", C.c_int), ("jmax", C.c_int), ("fold", C.c_int), ("cscale", C.c_double), ("iscale", C.c_int)]
+
+
+class PnampError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"pnamp error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} not built (run __graft_entry__.build())")
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.pnamp_create.argtypes = [C.POINTER(_Basis), C.POINTER(_Inter), C.POINTER(_Mesh), C.c_int, C.POINTER(vp)]
+    L.pnamp_create.restype = C.c_int
+    L.pnamp_set_states.argtypes = [vp, C.c_int, _f64p, _f64p]
+    L.pnamp_set_states.restype = C.c_int
+    L.pnamp_state_overlaps.argtypes = [vp, _f64p]
+    L.pnamp_state_overlaps.restype = C.c_int
+    L.pnamp_kernel_pair.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, _f64p, _f64p]
+    L.pnamp_kernel_pair.restype = C.c_int
+    L.pnamp_kernels_batched.argtypes = [vp, C.c_int, _i32p, _i32p, C.c_int, C.c_int, _f64p, _f64p]
+    L.pnamp_kernels_batched.restype = C.c_int
+    L.pnamp_hw_solve.argtypes = [vp, C.c_int, _f64p, _f64p, C.c_double, C.c_int, _f64p, _f64p, _f64p,
+                                 C.POINTER(C.c_int)]
+    L.pnamp_hw_solve.restype = C.c_int
+    L.pnamp_last_timing.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.pnamp_last_timing.restype = C.c_int
+    L.pnamp_n_euler.argtypes = [vp]
+    L.pnamp_n_euler.restype = C.c_int
+    L.pnamp_last_error.argtypes = [vp]
+    L.pnamp_last_error.restype = C.c_char_p
+    L.pnamp_destroy.argtypes = [vp]
+    L.pnamp_destroy.restype = None
+    _lib = L
+    return L
+
+
+def _p(a, ct=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+class PNAMP:
+    """One device context: basis, interaction, mesh and HFB states live in HBM."""
+
+    def __init__(self, basis, inter, mesh, device: int = 0):
+        L = load_library()
+        self._L = L
+        self._keep = []
+
+        def i32(x):
+            a = np.ascontiguousarray(x, dtype=np.int32)
+            self._keep.append(a)
+            return _p(a, C.c_int32)
+
+        def f64(x, order="C"):
+            a = np.asarray(x, dtype=np.float64, order=order).copy(order=order)
+            self._keep.append(a)
+            return _p(a)
+
+        self.basis, self.inter, self.mesh = basis, inter, mesh
+        b = _Basis(basis.n, i32(basis.t), i32(basis.nq), i32(basis.l), i32(basis.twoj), i32(basis.twom))
+        it = _Inter(inter.nme, i32(inter.ka), i32(inter.kb), i32(inter.kc), i32(inter.kd), f64(inter.me),
+                    f64(inter.t1b, "F"), float(inter.e0))
+        m = _Mesh(mesh.alp.size, mesh.bet.size, mesh.gam.size, f64(mesh.alp), f64(mesh.walp), f64(mesh.bet),
+                  f64(mesh.wbet), f64(mesh.gam), f64(mesh.wgam), mesh.nphi, mesh.jmax, mesh.fold,
+                  float(mesh.cscale), int(mesh.iscale))
+        h = C.c_void_p()
+        rc = L.pnamp_create(C.byref(b), C.byref(it), C.byref(m), device, C.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = L.pnamp_last_error(h).decode() if h.value else "create failed"
+            self.close()
+            raise PnampError(rc, msg)
+        self.nstates = 0
+
+    def _check(self, rc):
+        if rc != 0:
+            raise PnampError(rc, self._L.pnamp_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.pnamp_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n_euler(self) -> int:
+        return self._L.pnamp_n_euler(self._h)
+
+    def set_states(self, U: np.ndarray, V: np.ndarray):
+        """U, V: [nstates, n, n] with U[q][i, k] (single-particle i, quasiparticle k)."""
+        U = np.asarray(U, dtype=np.float64)
+        V = np.asarray(V, dtype=np.float64)
+        if U.ndim == 2:
+            U, V = U[None], V[None]
+        n = self.basis.n
+        if U.shape[1:] != (n, n) or V.shape != U.shape:
+            raise ValueError("U, V must be [nstates, n, n]")
+        Uf = np.ascontiguousarray(np.transpose(U, (0, 2, 1)))  # per-state column-major
+        Vf = np.ascontiguousarray(np.transpose(V, (0, 2, 1)))
+        self._check(self._L.pnamp_set_states(self._h, U.shape[0], _p(Uf), _p(Vf)))
+        self.nstates = U.shape[0]
+
+    def state_overlaps(self) -> np.ndarray:
+        out = np.zeros(self.nstates)
+        self._check(self._L.pnamp_state_overlaps(self._h, _p(out)))
+        return out
+
+    def _kshape(self):
+        j = self.mesh.jmax
+        return (NOBS, j + 1, 2 * j + 1, 2 * j + 1)
+
+    def kernel_pair(self, q0: int, q1: int, zval: int, nval: int, want_zres: bool = False):
+        kern = np.zeros(self._kshape(), dtype=np.complex128)
+        zres = np.zeros((self.n_euler, NOBS), dtype=np.complex128) if want_zres else None
+        self._check(self._L.pnamp_kernel_pair(self._h, int(q0), int(q1), int(zval), int(nval),
+                                              _p(zres) if zres is not None else None, _p(kern)))
+        return (zres, kern) if want_zres else kern
+
+    def kernels_batched(self, pairs, zval: int, nval: int, want_zres: bool = False, want_kern: bool = True):
+        pairs = np.asarray(pairs, dtype=np.int32).reshape(-1, 2)
+        q0 = np.ascontiguousarray(pairs[:, 0])
+        q1 = np.ascontiguousarray(pairs[:, 1])
+        npairs = len(pairs)
+        kern = np.zeros((npairs,) + self._kshape(), dtype=np.complex128) if want_kern else None
+        zres = np.zeros((npairs, self.n_euler, NOBS), dtype=np.complex128) if want_zres else None
+        self._check(self._L.pnamp_kernels_batched(self._h, npairs, _p(q0, C.c_int32), _p(q1, C.c_int32),
+                                                  int(zval), int(nval),
+                                                  _p(zres) if zres is not None else None,
+                                                  _p(kern) if kern is not None else None))
+        return (zres, kern) if want_zres else kern
+
+    def timing(self) -> tuple[float, float]:
+        a, b = C.c_float(), C.c_float()
+        self._check(self._L.pnamp_last_timing(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def hw_solve(self, N: np.ndarray, H: np.ndarray, eps: float, nos: int = 0, want_f: bool = True):
+        dim = N.shape[0]
+        Nf = np.asfortranarray(N, dtype=np.float64)
+        Hf = np.asfortranarray(H, dtype=np.float64)
+        EN = np.zeros(dim)
+        E = np.zeros(dim)
+        F = np.zeros((dim, dim), order="F") if want_f else None
+        m = C.c_int()
+        rc = self._L.pnamp_hw_solve(self._h, dim, _p(Nf), _p(Hf), float(eps), int(nos), _p(EN), _p(E),
+                                    _p(F) if F is not None else None, C.byref(m))
+        if rc not in (0, 1):
+            self._check(rc)
+        M = m.value
+        return dict(ifl=rc, m=M, EN=EN, E=E[:M], F=None if F is None else np.array(F[:, :M]))

@@ -1,0 +1,161 @@
+"""HIP path (through the C-ABI) against the oracle and the reference goldens.
+
+Tolerances: the plain FP64 path must match the oracle to 1e-10 relative
+(normwise per observable block), the Ge76 reference kernels to their 8 printed
+digits, and exact Fock-space projections to 1e-12."""
+import numpy as np
+import pytest
+
+import ge76
+from sccm_amd import configs
+from sccm_amd.basis import make_basis
+from sccm_amd.interaction import synthetic_interaction
+from sccm_amd.mesh import Mesh, make_mesh
+from sccm_amd.states import synthetic_state
+
+pytestmark = pytest.mark.gpu
+REL = 1e-10
+
+
+def _gpu(*a, **k):
+    from sccm_amd.pnamp import PNAMP
+    return PNAMP(*a, **k)
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _blockwise(k_gpu, k_or):
+    """max over (obs, J) of ||gpu - oracle|| / ||oracle|| on each J block"""
+    worst = 0.0
+    for io in range(k_or.shape[0]):
+        for J in range(k_or.shape[1]):
+            ref = k_or[io, J]
+            nr = np.abs(ref).max()
+            if nr == 0:
+                assert np.abs(k_gpu[io, J]).max() == 0
+                continue
+            worst = max(worst, np.abs(k_gpu[io, J] - ref).max() / nr)
+    return worst
+
+
+@pytest.mark.parametrize("space,zn,mesh_args,fold", [
+    ("sd", (2, 2), (3, 4, 3, 3, 4), 0),
+    ("sd", (2, 2), (4, 4, 4, 3, 4), 1),
+    ("pf", (2, 4), (2, 3, 2, 3, 6), 0),
+    ("f5pg9", (4, 2), (2, 2, 3, 2, 8), 0),
+])
+def test_pair_matches_oracle(oracle_mod, space, zn, mesh_args, fold):
+    b = make_basis(space)
+    inter = synthetic_interaction(b, 1.2, 101, 102)
+    mesh = make_mesh(*mesh_args, fold=fold)
+    Us, Vs = [], []
+    for s in (11, 12):
+        U, V = synthetic_state(b, *zn, seed=s)
+        Us.append(U)
+        Vs.append(V)
+    g = _gpu(b, inter, mesh)
+    g.set_states(np.array(Us), np.array(Vs))
+    o = oracle_mod.Oracle(b, inter, mesh, *zn)
+    for (p, q) in ((0, 1), (1, 1)):
+        zr_g, k_g = g.kernel_pair(p, q, *zn, want_zres=True)
+        zr_o, k_o = o.kernel_pair(Us[p], Vs[p], Us[q], Vs[q])
+        for io in range(5):
+            assert _rel(zr_g[:, io], zr_o[:, io]) < REL, (space, p, q, io)
+        assert _blockwise(k_g, k_o) < REL
+    g.close()
+
+
+def test_batched_equals_single(oracle_mod):
+    w = configs.pf_gcm(nstates=4, nalp=2, nbet=2, nphi=3, jmax=4)
+    g = _gpu(w.basis, w.inter, w.mesh)
+    g.set_states(w.U, w.V)
+    kb = g.kernels_batched(w.pairs, w.zval, w.nval)
+    for i, (p, q) in enumerate(w.pairs):
+        ks = g.kernel_pair(p, q, w.zval, w.nval)
+        assert np.array_equal(ks, kb[i])
+    g.close()
+
+
+def test_ge76_goldens_on_gpu(oracle_mod):
+    b, inter, U, V, betas, kern = ge76.load()
+    mesh = ge76.golden_mesh()
+    g = _gpu(b, inter, mesh)
+    g.set_states(U, V)
+    dets = g.state_overlaps()
+    for q in range(len(betas)):
+        _, d = oracle_mod.zbareover(U[q], V[q])
+        assert abs(dets[q] / d - 1) < 1e-12
+    strict = [i for i, x in enumerate(betas) if x != -0.2]
+    pairs = [(p, q) for p in strict for q in strict]
+    kb = g.kernels_batched(pairs, ge76.ZVAL, ge76.NVAL)
+    jm = mesh.jmax
+    for (p, q), k in zip(pairs, kb):
+        gold = kern[f"{betas[p]:+.1f},{betas[q]:+.1f}"]
+        n0, h0 = gold["0"][0], gold["0"][2]
+        for J in (0, 2, 4, 6):
+            N, H = k[0, J, jm, jm].real, k[4, J, jm, jm].real
+            assert abs(N - gold[str(J)][0]) <= 2e-7 * max(abs(gold[str(J)][0]), 1e-6 * abs(n0))
+            assert abs(H - gold[str(J)][2]) <= 2e-7 * max(abs(gold[str(J)][2]), 1e-6 * abs(h0))
+    o = oracle_mod.Oracle(b, inter, mesh, ge76.ZVAL, ge76.NVAL)
+    _, ko = o.kernel_pair(U[1], V[1], U[3], V[3])
+    assert _blockwise(kb[pairs.index((1, 3))], ko) < REL
+    g.close()
+
+
+def test_exact_projection_on_gpu():
+    from fock import Fock
+    b = make_basis("p")
+    inter = synthetic_interaction(b, 1.0, 11, 12)
+    U0, V0 = synthetic_state(b, 2, 2, seed=5, sigma=0.4, sigma_pair=0.3)
+    U1, V1 = synthetic_state(b, 2, 2, seed=6, sigma=0.4, sigma_pair=0.3)
+    na, jmax = 12, 4
+    alp = np.arange(na) * 2 * np.pi / na
+    w = np.full(na, 2 * np.pi / na)
+    x, wb = np.polynomial.legendre.leggauss(12)
+    m = Mesh(alp, w, np.arccos(x), wb, alp.copy(), w.copy(), 7, jmax, 0, 0)
+    g = _gpu(b, inter, m)
+    g.set_states(np.array([U0, U1]), np.array([V0, V1]))
+    kern = g.kernel_pair(0, 1, 2, 2)
+    F = Fock(b)
+    v0, v1 = F.hfb_state(U0, V0), F.hfb_state(U1, V1)
+    sec, P = F.projectors(2, 2, jmax)
+    H = F.hamiltonian(inter)[sec][:, sec].toarray()
+    for (J, K1, K2), Pm in P.items():
+        assert abs(np.vdot(v0[sec], Pm @ v1[sec]) - kern[0, J, K1 + jmax, K2 + jmax]) < 1e-12
+        assert abs(np.vdot(v0[sec], H @ (Pm @ v1[sec])) - kern[4, J, K1 + jmax, K2 + jmax]) < 1e-11
+    g.close()
+
+
+def test_hw_solve_matches_oracle(oracle_mod):
+    rng = np.random.default_rng(4)
+    dim = 30
+    G = rng.normal(size=(dim, 12))
+    N = G @ G.T + 1e-10 * np.eye(dim)
+    Hs = rng.normal(size=(dim, dim))
+    H = Hs + Hs.T
+    b = make_basis("sd")
+    g = _gpu(b, synthetic_interaction(b, 1.0, 1, 2), make_mesh(1, 1, 1, 1, 0))
+    r = g.hw_solve(N, H, 1e-6)
+    ro = oracle_mod.hndiag(N, H, 1e-6)
+    assert r["ifl"] == 0 and r["m"] == ro["m"] == 12
+    assert np.abs(r["E"] - ro["E"]).max() < 1e-10 * np.abs(ro["E"]).max()
+    assert np.abs(r["EN"] - ro["EN"]).max() < 1e-10 * ro["EN"][0]
+    for k in range(r["m"]):  # amplitudes up to sign
+        a, c = r["F"][:, k], ro["F"][:, k]
+        s = np.sign(a @ c)
+        assert np.abs(a - s * c).max() < 1e-8 * np.abs(c).max()
+    g.close()
+
+
+def test_singular_state_is_reported():
+    from sccm_amd.pnamp import PnampError
+    b = make_basis("sd")
+    U, V = synthetic_state(b, 2, 2, seed=3)
+    U[:, 5] = 0.0
+    g = _gpu(b, synthetic_interaction(b, 1.0, 1, 2), make_mesh(1, 1, 1, 1, 0))
+    with pytest.raises(PnampError) as e:
+        g.set_states(U, V)
+    assert e.value.code > 0
+    g.close()
