@@ -53,6 +53,15 @@ enum {
 
 enum { PNAMP_FOLD_PLAIN = 0, PNAMP_FOLD_LITERAL = 1 };
 
+/* Evaluation forms of pnamp_kernel_pair / pnamp_kernels_batched.
+ * PLAIN     the general path: every gauge point of every Euler point in full
+ *           (n x n densities, 2n x 2n Pfaffian), any U, V.
+ * FACTORED  FP64, for states without proton-neutron mixing: per Euler point
+ *           nphi proton and nphi neutron Pfaffians / density sets (h = n/2),
+ *           the proton-neutron energy for all nphi^2 gauge pairs from the pn
+ *           part of the interaction list.  Same results as PLAIN to rounding. */
+enum { PNAMP_FORM_PLAIN = 0, PNAMP_FORM_FACTORED = 1 };
+
 typedef struct pnamp_ctx pnamp_ctx;
 
 /* Single-particle basis (tnljm%t, n, l, twoj, twom of the reference). */
@@ -93,6 +102,15 @@ int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *inter, const
  * (zbareover, Codes/PNAMP/PNAMP_Proj.f90:6-79); replaces the .wf reads of
  * Kernel4Triaxial (PNAMP_Kernel3D.f90:83-122). */
 int pnamp_set_states(pnamp_ctx *ctx, int nstates, const double *U, const double *V);
+
+/* Selects the evaluation form (default PNAMP_FORM_PLAIN).  No counterpart in
+ * the reference, which has one form; callers choosing FACTORED must set
+ * states for which pnamp_states_separable returns 1. */
+int pnamp_set_form(pnamp_ctx *ctx, int form);
+
+/* 1 if every state set has no proton-neutron mixing (U, V block-diagonal by
+ * species up to the order of quasiparticles), 0 otherwise. */
+int pnamp_states_separable(pnamp_ctx *ctx);
 
 /* Vacuum overlaps <0|Phi_q> of the states set (Det of zbareover). */
 int pnamp_state_overlaps(pnamp_ctx *ctx, double *det);

@@ -40,19 +40,39 @@ static double cabs1(zc z) { return fabs(creal(z)) + fabs(cimag(z)); }
 
 /* ---------------------------------------------------------------- BLAS-like */
 
-/* C = alpha * op(A) * op(B) + beta * C, column-major, ta/tb in {'n','t'}. */
+/* C = alpha * op(A) * op(B) + beta * C, column-major, ta/tb in {'n','t'}
+ * (ZGEMM with the loop order of the reference BLAS: column j of C is
+ * accumulated from columns of A for op(A) = A, by dot products for A^T). */
 static void zgemm(char ta, char tb, int m, int n, int k, zc alpha, const zc *A, int lda,
                   const zc *B, int ldb, zc beta, zc *C, int ldc) {
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < m; ++i) {
-      zc s = 0;
+  for (int j = 0; j < n; ++j) {
+    zc *c = C + (size_t)j * ldc;
+    if (beta == 0)
+      for (int i = 0; i < m; ++i) c[i] = 0;
+    else if (beta != 1)
+      for (int i = 0; i < m; ++i) c[i] *= beta;
+    if (ta == 'n') {
       for (int l = 0; l < k; ++l) {
-        zc a = (ta == 'n') ? A[IX(i, l, lda)] : A[IX(l, i, lda)];
         zc b = (tb == 'n') ? B[IX(l, j, ldb)] : B[IX(j, l, ldb)];
-        s += a * b;
+        if (b == 0) continue;
+        b *= alpha;
+        const zc *a = A + (size_t)l * lda;
+        for (int i = 0; i < m; ++i) c[i] += a[i] * b;
       }
-      C[IX(i, j, ldc)] = alpha * s + (beta == 0 ? 0 : beta * C[IX(i, j, ldc)]);
+    } else {
+      for (int i = 0; i < m; ++i) {
+        const zc *a = A + (size_t)i * lda;
+        zc s = 0;
+        if (tb == 'n') {
+          const zc *bb = B + (size_t)j * ldb;
+          for (int l = 0; l < k; ++l) s += a[l] * bb[l];
+        } else {
+          for (int l = 0; l < k; ++l) s += a[l] * B[IX(j, l, ldb)];
+        }
+        c[i] += alpha * s;
+      }
     }
+  }
 }
 
 /* ZGETRF: LU with partial pivoting (pivot = max |re|+|im|, first maximum). */
@@ -400,8 +420,8 @@ int or_rokappa(int n, const zc *A, const double *U0, const double *V0, const dou
   size_t nn = (size_t)n * n;
   zc *Ac = malloc(sizeof(zc) * nn), *U0z = malloc(sizeof(zc) * nn), *V0z = malloc(sizeof(zc) * nn);
   zc *U1z = malloc(sizeof(zc) * nn), *V1z = malloc(sizeof(zc) * nn);
-  zc *a1 = malloc(sizeof(zc) * nn), *a2 = malloc(sizeof(zc) * nn), *a3 = malloc(sizeof(zc) * nn),
-     *a4 = malloc(sizeof(zc) * nn), *T = malloc(sizeof(zc) * nn);
+  zc *a1 = calloc(nn, sizeof(zc)), *a2 = calloc(nn, sizeof(zc)), *a3 = calloc(nn, sizeof(zc)),
+     *a4 = calloc(nn, sizeof(zc)), *T = calloc(nn, sizeof(zc));
   for (size_t i = 0; i < nn; ++i) {
     Ac[i] = conj(A[i]); U0z[i] = U0[i]; V0z[i] = V0[i]; U1z[i] = U1[i]; V1z[i] = V1[i];
   }

@@ -229,6 +229,145 @@ __global__ __launch_bounds__(256) void k_state_prep(PrepParams p) {
   }
 }
 
+// ------------------------------------------------------------------ block-level LA
+// Block-wide argmax helper: ties go to the smaller key.
+__device__ __forceinline__ void wave_best(double &v, int &k) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(v, off);
+    const int ok = __shfl_xor(k, off);
+    if (ov > v || (ov == v && ok < k)) { v = ov; k = ok; }
+  }
+}
+
+// Pfaffian of the N2 x N2 skew matrix whose upper triangle is in S
+// (column-major, leading dimension N2), by ZPfaffianF's algorithm
+// (Codes/PNAMP/PNAMP_Proj.f90:218-350): at step IB the largest |S(i,j)| of
+// the remaining upper triangle (first in row-major order on ties) is moved to
+// (NR, NC) with the two ZExch exchanges, the rank-2 Schur update follows, and
+// a pivot of modulus <= 1e-13 gives Pf = 0.  S is destroyed.  All NTH threads
+// of the block must call it; redv/redk need NTH/64 entries.
+template <int NTH>
+__device__ double2 block_pfaffian(double2 *S, const int N2, double *redv, int *redk) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int NW = NTH / 64;
+  double2 pf = make_double2(1.0, 0.0);
+  for (int IB = 0; IB < N2 / 2 - 1; ++IB) {
+    const int NR = 2 * IB, NC = NR + 1, m = N2 - NR;
+    double bv = -1.0;
+    int bk = 0x7fffffff;
+    for (int j = NR + 1 + tid / 64; j < N2; j += NW)
+      for (int i = NR + lane; i < j; i += 64) {
+        const double2 v = S[i + j * N2];
+        const double a = v.x * v.x + v.y * v.y;
+        const int key = i * N2 + j;
+        if (a > bv || (a == bv && key < bk)) { bv = a; bk = key; }
+      }
+    (void)m;
+    wave_best(bv, bk);
+    if (lane == 0) { redv[wave] = bv; redk[wave] = bk; }
+    __syncthreads();
+    bv = redv[0];
+    bk = redk[0];
+    for (int w = 1; w < NW; ++w)
+      if (redv[w] > bv || (redv[w] == bv && redk[w] < bk)) { bv = redv[w]; bk = redk[w]; }
+    const int I1 = bk / N2, I2 = bk % N2;
+    double phas = 1.0;
+    if (I1 == NR) phas = -phas;
+    if (I2 == NC) phas = -phas;
+    for (int pass = 0; pass < 2; ++pass) {
+      const int i1 = pass ? NC : NR, i2 = pass ? I2 : I1;
+      if (i1 != i2) {
+        for (int i = tid; i < N2; i += NTH) {
+          if (i >= NR && i < i1) {
+            const double2 t = S[i + i1 * N2]; S[i + i1 * N2] = S[i + i2 * N2]; S[i + i2 * N2] = t;
+          } else if (i > i2) {
+            const double2 t = S[i1 + i * N2]; S[i1 + i * N2] = S[i2 + i * N2]; S[i2 + i * N2] = t;
+          } else if (i > i1 && i < i2) {
+            const double2 t = S[i1 + i * N2];
+            S[i1 + i * N2] = cneg(S[i + i2 * N2]);
+            S[i + i2 * N2] = cneg(t);
+          } else if (i == i1) {
+            S[i1 + i2 * N2] = cneg(S[i1 + i2 * N2]);
+          }
+        }
+      }
+      __syncthreads();
+    }
+    const double2 ss = S[NR + NC * N2];
+    if (sqrt(ss.x * ss.x + ss.y * ss.y) <= 1.0e-13) {
+      __syncthreads();
+      return cz();
+    }
+    const double2 inv = crecip(ss);
+    const int r0 = NR + 2;
+    for (int j = r0 + 1 + tid / 64; j < N2; j += NW) {
+      const double2 snr_j = S[NR + j * N2], snc_j = S[NC + j * N2];
+      for (int i = r0 + lane; i < j; i += 64) {
+        double2 t = cmul(S[NC + i * N2], snr_j);
+        t = csub(t, cmul(S[NR + i * N2], snc_j));
+        S[i + j * N2] = cadd(S[i + j * N2], cmul(t, inv));
+      }
+    }
+    pf = cscale(phas, cmul(pf, ss));
+    __syncthreads();
+  }
+  const double2 last = S[(N2 - 2) + (N2 - 1) * N2];
+  __syncthreads();
+  return cmul(pf, last);
+}
+
+// Gauss-Jordan with partial pivoting (pivot = max |re|+|im|, first maximum,
+// as ZGETRF): A (n x n) -> 1, B -> A^{-1} B.  Returns the LAPACK-style INFO
+// of the first exactly-zero pivot (0 if none).  fcol needs n entries.
+template <int NTH>
+__device__ int block_gauss_jordan(double2 *A, double2 *B, const int n, double2 *fcol, int *redk) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int info = 0;
+  for (int k = 0; k < n; ++k) {
+    if (wave == 0) {
+      double bv = -1.0;
+      int bk = 0x7fffffff;
+      for (int i = k + lane; i < n; i += 64) {
+        const double a = cabs1(A[i + k * n]);
+        if (a > bv || (a == bv && i < bk)) { bv = a; bk = i; }
+      }
+      wave_best(bv, bk);
+      if (lane == 0) redk[0] = bk;
+    }
+    __syncthreads();
+    const int pr = redk[0];
+    if (pr != k)
+      for (int j = tid; j < 2 * n; j += NTH) {
+        double2 *M = j < n ? A : B;
+        const int jj = j < n ? j : j - n;
+        const double2 t = M[k + jj * n];
+        M[k + jj * n] = M[pr + jj * n];
+        M[pr + jj * n] = t;
+      }
+    __syncthreads();
+    const double2 pv = A[k + k * n];
+    if (pv.x == 0.0 && pv.y == 0.0 && info == 0) info = k + 1;
+    const double2 ipv = crecip(pv);
+    for (int i = tid; i < n; i += NTH) fcol[i] = (i == k) ? cz() : A[i + k * n];
+    for (int j = tid; j < 2 * n; j += NTH) {
+      double2 *M = j < n ? A : B;
+      const int jj = j < n ? j : j - n;
+      if (j >= k) M[k + jj * n] = cmul(M[k + jj * n], ipv);
+    }
+    __syncthreads();
+    for (int jj = tid / 64; jj < 2 * n; jj += NTH / 64) {
+      if (jj < n && jj <= k) continue;
+      double2 *M = jj < n ? A : B;
+      const int c = jj < n ? jj : jj - n;
+      const double2 rk = M[k + c * n];
+      for (int i = lane; i < n; i += 64)
+        if (i != k) M[i + c * n] = csub(M[i + c * n], cmul(fcol[i], rk));
+    }
+    __syncthreads();
+  }
+  return info;
+}
+
 // ------------------------------------------------------------------ mesh points
 struct VEntry {
   double me;
@@ -253,15 +392,6 @@ struct PointParams {
   int *status;                      // [npairs][NE]
 };
 
-// Block-wide argmax of |S(i,j)|^2 over the upper triangle NR <= i < j < N2,
-// ties to the first (i, j) in row-major order, like ZPfaffianF's scan.
-__device__ __forceinline__ void wave_best(double &v, int &k) {
-  for (int off = 32; off > 0; off >>= 1) {
-    const double ov = __shfl_xor(v, off);
-    const int ok = __shfl_xor(k, off);
-    if (ov > v || (ov == v && ok < k)) { v = ov; k = ok; }
-  }
-}
 
 __global__ __launch_bounds__(NT) void k_point(PointParams p) {
   __shared__ double2 sm[5 * NMAX * NMAX];
@@ -342,70 +472,7 @@ __global__ __launch_bounds__(NT) void k_point(PointParams p) {
       S[i + j * N2] = cscale(p.cscale, v);
     }
     __syncthreads();
-    double2 pf = make_double2(1.0, 0.0);
-    for (int IB = 0; IB < n - 1; ++IB) {
-      const int NR = 2 * IB, NC = NR + 1, m = N2 - NR;
-      double bv = -1.0;
-      int bk = 0x7fffffff;
-      for (int e = tid; e < m * m; e += NT) {
-        const int i = NR + e % m, j = NR + e / m;
-        if (i < j) {
-          const double2 v = S[i + j * N2];
-          const double a = v.x * v.x + v.y * v.y;
-          const int key = i * N2 + j;
-          if (a > bv || (a == bv && key < bk)) { bv = a; bk = key; }
-        }
-      }
-      wave_best(bv, bk);
-      if (lane == 0) { redv[wave] = bv; redk[wave] = bk; }
-      __syncthreads();
-      bv = redv[0];
-      bk = redk[0];
-      for (int w = 1; w < NWAVE; ++w)
-        if (redv[w] > bv || (redv[w] == bv && redk[w] < bk)) { bv = redv[w]; bk = redk[w]; }
-      const int I1 = bk / N2, I2 = bk % N2;
-      double phas = 1.0;
-      if (I1 == NR) phas = -phas;
-      if (I2 == NC) phas = -phas;
-      // ZExch(NR, I1) then ZExch(NC, I2) on the upper triangle (rows < NR are done)
-      for (int pass = 0; pass < 2; ++pass) {
-        const int i1 = pass ? NC : NR, i2 = pass ? I2 : I1;
-        if (i1 != i2) {
-          for (int i = tid; i < N2; i += NT) {
-            if (i >= NR && i < i1) {
-              const double2 t = S[i + i1 * N2]; S[i + i1 * N2] = S[i + i2 * N2]; S[i + i2 * N2] = t;
-            } else if (i > i2) {
-              const double2 t = S[i1 + i * N2]; S[i1 + i * N2] = S[i2 + i * N2]; S[i2 + i * N2] = t;
-            } else if (i > i1 && i < i2) {
-              const double2 t = S[i1 + i * N2];
-              S[i1 + i * N2] = cneg(S[i + i2 * N2]);
-              S[i + i2 * N2] = cneg(t);
-            } else if (i == i1) {
-              S[i1 + i2 * N2] = cneg(S[i1 + i2 * N2]);
-            }
-          }
-        }
-        __syncthreads();
-      }
-      const double2 ss = S[NR + NC * N2];
-      if (sqrt(ss.x * ss.x + ss.y * ss.y) <= 1.0e-13) {
-        pf = cz();
-        break;
-      }
-      const double2 inv = crecip(ss);
-      const int r0 = NR + 2, m2 = N2 - r0;
-      for (int e = tid; e < m2 * m2; e += NT) {
-        const int i = r0 + e % m2, j = r0 + e / m2;
-        if (i < j) {
-          double2 t = cmul(S[NC + i * N2], S[NR + j * N2]);
-          t = csub(t, cmul(S[NR + i * N2], S[NC + j * N2]));
-          S[i + j * N2] = cadd(S[i + j * N2], cmul(t, inv));
-        }
-      }
-      pf = cscale(phas, cmul(pf, ss));
-      __syncthreads();
-    }
-    if (pf.x != 0.0 || pf.y != 0.0) pf = cmul(pf, S[(N2 - 2) + (N2 - 1) * N2]);
+    const double2 pf = block_pfaffian<NT>(S, N2, redv, redk);
     const double2 gphase = cexpi(-phip * p.zval - phin * p.nval);
     const double2 znorm = cmul(cscale(vac, pf), gphase);
     __syncthreads();
@@ -446,49 +513,10 @@ __global__ __launch_bounds__(NT) void k_point(PointParams p) {
     }
     for (int e = tid; e < nn; e += NT) s3[e] = make_double2((e % n) == (e / n) ? 1.0 : 0.0, 0.0);
     __syncthreads();
-    // Gauss-Jordan: s2 -> 1, s3 -> T^{-1} (pivot = max |re|+|im|, first maximum)
-    for (int k = 0; k < n; ++k) {
-      if (wave == 0) {
-        double bv = -1.0;
-        int bk = 0x7fffffff;
-        for (int i = k + lane; i < n; i += 64) {
-          const double a = cabs1(s2[i + k * n]);
-          if (a > bv || (a == bv && i < bk)) { bv = a; bk = i; }
-        }
-        wave_best(bv, bk);
-        if (lane == 0) redk[0] = bk;
-      }
-      __syncthreads();
-      const int pr = redk[0];
-      if (pr != k)
-        for (int j = tid; j < 2 * n; j += NT) {
-          double2 *M = j < n ? s2 : s3;
-          const int jj = j < n ? j : j - n;
-          const double2 t = M[k + jj * n];
-          M[k + jj * n] = M[pr + jj * n];
-          M[pr + jj * n] = t;
-        }
-      __syncthreads();
-      const double2 pv = s2[k + k * n];
-      if (pv.x == 0.0 && pv.y == 0.0 && status == 0) status = k + 1;
-      const double2 ipv = crecip(pv);
-      for (int i = tid; i < n; i += NT) fcol[i] = (i == k) ? cz() : s2[i + k * n];
-      for (int j = tid; j < 2 * n; j += NT) {
-        double2 *M = j < n ? s2 : s3;
-        const int jj = j < n ? j : j - n;
-        if (j >= k) M[k + jj * n] = cmul(M[k + jj * n], ipv);
-      }
-      __syncthreads();
-      for (int e = tid; e < 2 * nn; e += NT) {
-        const int i = e % n, jj = (e / n) % n;
-        if (i == k) continue;
-        if (e < nn) {
-          if (jj > k) s2[i + jj * n] = csub(s2[i + jj * n], cmul(fcol[i], s2[k + jj * n]));
-        } else {
-          s3[i + jj * n] = csub(s3[i + jj * n], cmul(fcol[i], s3[k + jj * n]));
-        }
-      }
-      __syncthreads();
+    // Gauss-Jordan: s2 -> 1, s3 -> T^{-1}
+    {
+      const int info = block_gauss_jordan<NT>(s2, s3, n, fcol, redk);
+      if (info && !status) status = info;
     }
     // s2 = W1 = (A V1) T^{-1}, s4 = W2 = (A* U1) T^{-1}
     for (int t = tid; t < (n / 2) * (n / 2); t += NT) {
@@ -597,6 +625,306 @@ __global__ __launch_bounds__(NT) void k_point(PointParams p) {
     double2 *out = p.zres + ((size_t)ip * p.NE + iang) * NOBS;
     for (int o = 0; o < NOBS; ++o) out[o] = cscale(inv, acc[o]);
     p.status[(size_t)ip * p.NE + iang] = st;
+  }
+}
+
+// ------------------------------------------------------------------ factored form
+// For states without proton-neutron mixing (U, V block-diagonal by species
+// after ordering the quasiparticles) every matrix of the path is
+// block-diagonal: the 2n x 2n Pfaffian is (-1)^h Pf(S_p) Pf(S_n) with
+// S_s = [[e^{2 i phi_s} R Z1s R^T, -1], [1, -Z0s]] (2h x 2h), and the species-s
+// densities depend only on phi_s, through T_s = e^{i phi_s} X_s + e^{-i phi_s} Y_s
+// with X_s = V0s^T R V1s, Y_s = U0s^T R* U1s fixed per Euler point.  So each
+// Euler point needs nphi Pfaffians / inverses / density sets per species
+// instead of nphi^2 full ones.  The energy splits into E_pp(phi_p) + E_nn(phi_n)
+// (species lists) + E_pn(phi_p, phi_n), a bilinear form in rho_p and rho_n
+// evaluated for all nphi^2 gauge pairs by k_fac_combine.
+constexpr int NTF = 256;     // threads of the species workgroup
+constexpr int NTC = 288;     // threads of the combine workgroup (9 groups of 32)
+constexpr int NPHI_MAX = 16;
+
+struct FacParams {
+  int n, h, nalp, nbet, ngam, nphi, npack, NE;
+  double cscale;
+  const double *alp, *bet, *gam;
+  const double *Uf, *Vf;            // [nstates][2][h*h], species blocks, column-major
+  const double *Z;                  // [nstates][n*n]
+  const double *rtab;
+  const int *pk_a, *pk_b, *blk_off, *blk_sz, *blk_base, *twom;
+  const int *q0, *q1;
+  const VEntry *vl0, *vl1;          // species lists with local indices (p, n)
+  int nme0, nme1;
+  const double *t1b;                // n x n
+  double2 *out;                     // [npairs][NE][2][nphi][4]: Pf, E2, Ekin, Tr rho
+  double2 *rho;                     // [npairs][NE][2][nphi][h*h]
+  int *status;                      // [npairs][NE][2]
+};
+
+__global__ __launch_bounds__(NTF) void k_fac_species(FacParams p) {
+  extern __shared__ double2 sm[];  // 10 h^2 complex
+  __shared__ double2 sR[RPACK_MAX];
+  __shared__ double2 fcol[HMAX];
+  __shared__ double redv[NTF / 64 * 8];
+  __shared__ int redk[NTF / 64];
+  __shared__ short m_off[HMAX], m_sz[HMAX], m_base[HMAX];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int iang = blockIdx.x, ip = blockIdx.y, sp = blockIdx.z;
+  const int n = p.n, h = p.h, hh = h * h, nn = n * n, N2 = 2 * h;
+  const int ig = iang % p.ngam, ia = (iang / p.ngam) % p.nalp, ib = iang / (p.ngam * p.nalp);
+  const double alp = p.alp[ia], gam = p.gam[ig];
+  const int q0 = p.q0[ip], q1 = p.q1[ip];
+  const double *U0 = p.Uf + ((size_t)q0 * 2 + sp) * hh, *V0 = p.Vf + ((size_t)q0 * 2 + sp) * hh;
+  const double *U1 = p.Uf + ((size_t)q1 * 2 + sp) * hh, *V1 = p.Vf + ((size_t)q1 * 2 + sp) * hh;
+  const double *Z0 = p.Z + (size_t)q0 * nn + (size_t)sp * h * (n + 1);  // species block, ld n
+  const double *Z1 = p.Z + (size_t)q1 * nn + (size_t)sp * h * (n + 1);
+  const VEntry *vl = sp ? p.vl1 : p.vl0;
+  const int nme = sp ? p.nme1 : p.nme0;
+
+  for (int i = tid; i < h; i += NTF) {
+    m_off[i] = (short)p.blk_off[i];
+    m_sz[i] = (short)p.blk_sz[i];
+    m_base[i] = (short)p.blk_base[i];
+  }
+  for (int e = tid; e < p.npack; e += NTF) {
+    const int a = p.pk_a[e], b = p.pk_b[e];
+    const double d = p.rtab[(size_t)ib * p.npack + e];
+    sR[e] = cscale(d, cexpi(-alp * 0.5 * p.twom[a] - gam * 0.5 * p.twom[b]));
+  }
+  __syncthreads();
+  double2 *RV1 = sm, *RcU1 = sm + hh, *X = sm + 2 * hh, *Y = sm + 3 * hh, *M0 = sm + 4 * hh;
+  double2 *SC = sm + 5 * hh;  // 5 h^2 scratch; the Pfaffian uses its first 4 h^2
+  // RV1 = R V1s, RcU1 = R* U1s, SC = R Z1s
+  for (int e = tid; e < hh; e += NTF) {
+    const int i = e % h, j = e / h;
+    const int off = m_off[i], sz = m_sz[i], base = m_base[i];
+    double2 a = cz(), b = cz(), c = cz();
+    for (int k = 0; k < sz; ++k) {
+      const double2 r = sR[base + (i - off) + k * sz];
+      rfma(a, V1[(off + k) + j * h], r);
+      rfma(b, U1[(off + k) + j * h], cconj(r));
+      rfma(c, Z1[(off + k) + (size_t)j * n], r);
+    }
+    RV1[e] = a;
+    RcU1[e] = b;
+    SC[e] = c;
+  }
+  __syncthreads();
+  // X = V0s^T RV1, Y = U0s^T RcU1, M0 = (R Z1s) R^T
+  for (int e = tid; e < hh; e += NTF) {
+    const int i = e % h, j = e / h;
+    double2 x = cz(), y = cz();
+    for (int k = 0; k < h; ++k) {
+      rfma(x, V0[k + i * h], RV1[k + j * h]);
+      rfma(y, U0[k + i * h], RcU1[k + j * h]);
+    }
+    X[e] = x;
+    Y[e] = y;
+    const int off = m_off[j], sz = m_sz[j], base = m_base[j];
+    double2 m = cz();
+    for (int k = 0; k < sz; ++k) cfma(m, SC[i + (off + k) * h], sR[base + (j - off) + k * sz]);
+    M0[e] = m;
+  }
+  __syncthreads();
+  int status = 0;
+  const int nphi = p.nphi;
+  for (int f = 0; f < nphi; ++f) {
+    const double phi = (nphi == 1) ? 0.0 : (2.0 * M_PI / nphi) * f;
+    const double2 e1 = cexpi(phi), e2 = cexpi(2.0 * phi);
+    // S = cscale [[e^{2 i phi} M0, -1], [1, -Z0s]] (upper triangle, ld 2h)
+    for (int j = tid / 64; j < N2; j += NTF / 64)
+      for (int i = lane; i < j; i += 64) {
+        double2 v;
+        if (j < h) v = cmul(e2, M0[i + j * h]);
+        else if (i < h) v = make_double2((j - h == i) ? -1.0 : 0.0, 0.0);
+        else v = make_double2(-Z0[(i - h) + (size_t)(j - h) * n], 0.0);
+        SC[i + j * N2] = cscale(p.cscale, v);
+      }
+    __syncthreads();
+    const double2 pf = block_pfaffian<NTF>(SC, N2, redv, redk);
+    // T = e^{i phi} X + e^{-i phi} Y -> SC[0], 1 -> SC[1]
+    double2 *T = SC, *Ti = SC + hh, *W1 = SC + 2 * hh, *W2 = SC + 3 * hh;
+    for (int e = tid; e < hh; e += NTF) {
+      T[e] = cadd(cmul(e1, X[e]), cmul(cconj(e1), Y[e]));
+      Ti[e] = make_double2((e % h) == (e / h) ? 1.0 : 0.0, 0.0);
+    }
+    __syncthreads();
+    {
+      const int info = block_gauss_jordan<NTF>(T, Ti, h, fcol, redk);
+      if (info && !status) status = info;
+    }
+    // W1 = e^{i phi} RV1 T^{-1}, W2 = e^{-i phi} RcU1 T^{-1}
+    for (int e = tid; e < hh; e += NTF) {
+      const int i = e % h, j = e / h;
+      double2 a = cz(), b = cz();
+      for (int k = 0; k < h; ++k) {
+        const double2 t = Ti[k + j * h];
+        cfma(a, RV1[i + k * h], t);
+        cfma(b, RcU1[i + k * h], t);
+      }
+      W1[e] = cmul(e1, a);
+      W2[e] = cmul(cconj(e1), b);
+    }
+    __syncthreads();
+    // rho = W1 V0s^T -> SC[0], kappa10 = W1 U0s^T -> SC[1], kappa01 = -W2 V0s^T -> SC[4]
+    double2 *ro = SC, *k10 = SC + hh, *k01 = SC + 4 * hh;
+    for (int e = tid; e < hh; e += NTF) {
+      const int i = e % h, j = e / h;
+      double2 r = cz(), a = cz(), b = cz();
+      for (int k = 0; k < h; ++k) {
+        const double2 w1 = W1[i + k * h], w2 = W2[i + k * h];
+        const double v0 = V0[j + k * h], u0 = U0[j + k * h];
+        rfma(r, v0, w1);
+        rfma(a, u0, w1);
+        rfma(b, v0, w2);
+      }
+      ro[e] = r;
+      k10[e] = a;
+      k01[e] = cneg(b);
+    }
+    __syncthreads();
+    double2 e2b = cz(), ek = cz(), tr = cz();
+    for (int q = tid; q < nme; q += NTF) {
+      const VEntry ve = vl[q];
+      const int i = ve.idx & 255, j = (ve.idx >> 8) & 255, k = (ve.idx >> 16) & 255, l = ve.idx >> 24;
+      const double2 rki = ro[k + i * h], rlj = ro[l + j * h], rli = ro[l + i * h], rkj = ro[k + j * h];
+      const double2 rik = ro[i + k * h], rjl = ro[j + l * h], rjk = ro[j + k * h], ril = ro[i + l * h];
+      const double2 pij = k01[i + j * h], pji = k01[j + i * h], pkl = k01[k + l * h], plk = k01[l + k * h];
+      const double2 qkl = k10[k + l * h], qlk = k10[l + k * h], qij = k10[i + j * h], qji = k10[j + i * h];
+      const double2 r_ijkl = cadd(csub(cmul(rki, rlj), cmul(rli, rkj)), cmul(pij, qkl));
+      const double2 r_jikl = cadd(csub(cmul(rkj, rli), cmul(rlj, rki)), cmul(pji, qkl));
+      const double2 r_jilk = cadd(csub(cmul(rlj, rki), cmul(rkj, rli)), cmul(pji, qlk));
+      const double2 r_ijlk = cadd(csub(cmul(rli, rkj), cmul(rki, rlj)), cmul(pij, qlk));
+      const double2 r_klij = cadd(csub(cmul(rik, rjl), cmul(rjk, ril)), cmul(pkl, qij));
+      const double2 r_klji = cadd(csub(cmul(rjk, ril), cmul(rik, rjl)), cmul(pkl, qji));
+      const double2 r_lkij = cadd(csub(cmul(ril, rjk), cmul(rjl, rik)), cmul(plk, qij));
+      const double2 r_lkji = cadd(csub(cmul(rjl, rik), cmul(ril, rjk)), cmul(plk, qji));
+      const double d = (k + l == i + j) ? 0.0 : 1.0;
+      double2 sa = cadd(csub(csub(r_ijkl, r_jikl), r_ijlk), r_jilk);
+      double2 sb = cadd(csub(csub(r_klij, r_klji), r_lkij), r_lkji);
+      e2b = cadd(e2b, cscale(0.25 * ve.me, cadd(sa, cscale(d, sb))));
+    }
+    for (int e = tid; e < hh; e += NTF) {
+      const int a = e % h, c = e / h;
+      ek = cadd(ek, cscale(p.t1b[(sp * h + a) + (size_t)(sp * h + c) * n], ro[c + a * h]));
+    }
+    for (int i = tid; i < h; i += NTF) tr = cadd(tr, ro[i + i * h]);
+    double vals[6] = {e2b.x, e2b.y, ek.x, ek.y, tr.x, tr.y};
+    for (int v = 0; v < 6; ++v)
+      for (int off = 32; off > 0; off >>= 1) vals[v] += __shfl_xor(vals[v], off);
+    if (lane == 0)
+      for (int v = 0; v < 6; ++v) redv[wave * 8 + v] = vals[v];
+    const size_t obase = ((((size_t)ip * p.NE + iang) * 2 + sp) * nphi + f);
+    double2 *rout = p.rho + obase * hh;
+    for (int e = tid; e < hh; e += NTF) rout[e] = ro[e];
+    __syncthreads();
+    if (tid == 0) {
+      double t[6] = {0, 0, 0, 0, 0, 0};
+      for (int w = 0; w < NTF / 64; ++w)
+        for (int v = 0; v < 6; ++v) t[v] += redv[w * 8 + v];
+      double2 *o = p.out + obase * 4;
+      o[0] = pf;
+      o[1] = make_double2(t[0], t[1]);
+      o[2] = make_double2(t[2], t[3]);
+      o[3] = make_double2(t[4], t[5]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) p.status[((size_t)ip * p.NE + iang) * 2 + sp] = status;
+}
+
+struct CombParams {
+  int h, nphi, NE, zval, nval;
+  double vac_scale, snpfaf, sgn_h, e0;
+  const double *det;
+  const int *q0, *q1;
+  const VEntry *vpn;                // pn list: a,c proton-local; b,d neutron-local; pad = partner flag
+  int nme_pn;
+  const double2 *out, *rho;
+  const int *fstatus;
+  double2 *zres;                    // [npairs][NE][NOBS]
+  int *status;                      // [npairs][NE]
+};
+
+// One workgroup per (pair, Euler point): E_pn for all gauge pairs from the
+// pn entries of the list (for a in p, b in n the eight-term sum of
+// HFB_Energy_COMPLEX reduces to V [rho_p(c,a) rho_n(d,b) + d rho_p(a,c) rho_n(b,d)]),
+// then the nphi^2 gauge points are summed into ZRES.
+__global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
+  extern __shared__ double2 dsm[];  // rho_p [nphi][h^2], rho_n [nphi][h^2]
+  __shared__ double2 epn[NPHI_MAX * NPHI_MAX];
+  __shared__ double red[NTC / 64 * 10];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int iang = blockIdx.x, ip = blockIdx.y;
+  const int h = p.h, hh = h * h, nphi = p.nphi;
+  const size_t base = ((size_t)ip * p.NE + iang) * 2;
+  double2 *rp = dsm, *rn = dsm + (size_t)nphi * hh;
+  for (int e = tid; e < nphi * hh; e += NTC) {
+    rp[e] = p.rho[base * nphi * hh + e];
+    rn[e] = p.rho[(base + 1) * nphi * hh + e];
+  }
+  __syncthreads();
+  const int grp = tid >> 5, gl = tid & 31;
+  for (int fp = grp; fp < nphi; fp += NTC / 32) {
+    double2 acc[NPHI_MAX];
+    for (int f = 0; f < NPHI_MAX; ++f) acc[f] = cz();
+    const double2 *rpf = rp + (size_t)fp * hh;
+    for (int q = gl; q < p.nme_pn; q += 32) {
+      const VEntry ve = p.vpn[q];
+      const int a = ve.idx & 255, b = (ve.idx >> 8) & 255, c = (ve.idx >> 16) & 255, d = ve.idx >> 24;
+      const double2 x = cscale(ve.me, rpf[c + a * h]);
+      const double2 xp = cscale(ve.pad ? ve.me : 0.0, rpf[a + c * h]);
+      for (int f = 0; f < NPHI_MAX; ++f) {
+        if (f >= nphi) break;
+        cfma(acc[f], x, rn[(size_t)f * hh + d + b * h]);
+        cfma(acc[f], xp, rn[(size_t)f * hh + b + d * h]);
+      }
+    }
+    for (int f = 0; f < nphi; ++f) {
+      double re = acc[f].x, im = acc[f].y;
+      for (int off = 16; off > 0; off >>= 1) {
+        re += __shfl_xor(re, off);
+        im += __shfl_xor(im, off);
+      }
+      if (gl == 0) epn[fp * nphi + f] = make_double2(re, im);
+    }
+  }
+  __syncthreads();
+  const double2 *op = p.out + base * nphi * 4, *on = p.out + (base + 1) * nphi * 4;
+  const double vac = p.snpfaf * p.sgn_h * (p.det[p.q0[ip]] * p.vac_scale) * (p.det[p.q1[ip]] * p.vac_scale);
+  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int t = tid; t < nphi * nphi; t += NTC) {
+    const int fp = t / nphi, fn = t % nphi;
+    const double phip = (nphi == 1) ? 0.0 : (2.0 * M_PI / nphi) * fp;
+    const double phin = (nphi == 1) ? 0.0 : (2.0 * M_PI / nphi) * fn;
+    const double2 *a = op + fp * 4, *b = on + fn * 4;
+    const double2 zn = cmul(cscale(vac, cmul(a[0], b[0])), cexpi(-phip * p.zval - phin * p.nval));
+    const double2 ekin = cadd(a[2], b[2]);
+    const double2 E = cadd(cadd(cadd(a[1], b[1]), epn[fp * nphi + fn]), cadd(ekin, make_double2(p.e0, 0.0)));
+    const double2 o[5] = {zn, cmul(zn, a[3]), cmul(zn, b[3]), cmul(zn, ekin), cmul(zn, E)};
+    for (int k = 0; k < 5; ++k) {
+      v[2 * k] += o[k].x;
+      v[2 * k + 1] += o[k].y;
+    }
+  }
+  for (int k = 0; k < 10; ++k)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
+  if (lane == 0)
+    for (int k = 0; k < 10; ++k) red[wave * 10 + k] = v[k];
+  __syncthreads();
+  if (tid == 0) {
+    const double inv = 1.0 / (double)(nphi * nphi);
+    double2 *z = p.zres + ((size_t)ip * p.NE + iang) * NOBS;
+    for (int k = 0; k < 5; ++k) {
+      double re = 0, im = 0;
+      for (int w = 0; w < NTC / 64; ++w) {
+        re += red[w * 10 + 2 * k];
+        im += red[w * 10 + 2 * k + 1];
+      }
+      z[k] = make_double2(inv * re, inv * im);
+    }
+    const int s0 = p.fstatus[base], s1 = p.fstatus[base + 1];
+    p.status[(size_t)ip * p.NE + iang] = s0 ? s0 : s1;
   }
 }
 
@@ -834,6 +1162,15 @@ struct pnamp_ctx {
   int *d_q0 = nullptr, *d_q1 = nullptr;
   double2 *d_zres = nullptr, *d_kern = nullptr;
   int *d_status = nullptr, *d_pair_status = nullptr;
+  // factored form
+  int form = PNAMP_FORM_PLAIN;
+  bool separable = false;
+  int nme_s[3] = {0, 0, 0};
+  VEntry *d_vs[3] = {nullptr, nullptr, nullptr};  // pp, nn (local indices), pn
+  double *d_Uf = nullptr, *d_Vf = nullptr;
+  double2 *d_fout = nullptr, *d_frho = nullptr;
+  int *d_fstatus = nullptr;
+  size_t fac_cap = 0;  // pairs the factored scratch holds
 };
 
 #define HIPCHK(ctx, x)                                                       \
@@ -967,6 +1304,33 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
   }
   if ((r = dupload(c, &c->d_vlist, vl.data(), vl.size())) || (r = dupload(c, &c->d_t1b, inter->t1b, (size_t)n * n)))
     return bail(r, c->err);
+  // species lists of the factored form: pp and nn with species-local indices,
+  // pn (a, c proton; b, d neutron) with the (ab) <-> (cd) partner flag in pad
+  {
+    std::vector<VEntry> ls[3];
+    for (int64_t q = 0; q < inter->nme; ++q) {
+      const int a = inter->ka[q] - 1, b = inter->kb[q] - 1, cc = inter->kc[q] - 1, d = inter->kd[q] - 1;
+      const int ta = a >= h, tb = b >= h, tc = cc >= h, td = d >= h;
+      VEntry e{inter->me2b[q], 0u, 0u};
+      if (!ta && !tb && !tc && !td) {
+        e.idx = (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)cc << 16) | ((uint32_t)d << 24);
+        ls[0].push_back(e);
+      } else if (ta && tb && tc && td) {
+        e.idx = (uint32_t)(a - h) | ((uint32_t)(b - h) << 8) | ((uint32_t)(cc - h) << 16) | ((uint32_t)(d - h) << 24);
+        ls[1].push_back(e);
+      } else if (!ta && tb && !tc && td) {
+        e.idx = (uint32_t)a | ((uint32_t)(b - h) << 8) | ((uint32_t)cc << 16) | ((uint32_t)(d - h) << 24);
+        e.pad = (a + b == cc + d) ? 0u : 1u;
+        ls[2].push_back(e);
+      } else {
+        return bail(PNAMP_E_ARG, "interaction entry violates isospin-projection conservation");
+      }
+    }
+    for (int k = 0; k < 3; ++k) {
+      c->nme_s[k] = (int)ls[k].size();
+      if ((r = dupload(c, &c->d_vs[k], ls[k].data(), ls[k].size()))) return bail(r, c->err);
+    }
+  }
   if ((r = dalloc(c, &c->d_rtab, (size_t)c->nbet * c->npack)) || (r = dalloc(c, &c->d_ptab, (size_t)c->nbet * 2 * c->npj)))
     return bail(r, c->err);
   TableParams tp{c->nbet, h, c->npack, c->jmax, c->npj, c->d_bet, c->d_pk_a, c->d_pk_b, c->d_twoj, c->d_twom,
@@ -982,6 +1346,7 @@ extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, cons
   if (!c || !U || !V || nstates <= 0) return PNAMP_E_ARG;
   const size_t nn = (size_t)c->n * c->n;
   int r = 0;
+  c->separable = false;
   if ((r = dupload(c, &c->d_U, U, nn * nstates)) || (r = dupload(c, &c->d_V, V, nn * nstates)) ||
       (r = dalloc(c, &c->d_Z, nn * nstates)) || (r = dalloc(c, &c->d_det, (size_t)nstates)) ||
       (r = dalloc(c, &c->d_state_status, (size_t)nstates)))
@@ -998,8 +1363,52 @@ extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, cons
       c->nstates = 0;
       return fail(c, st[q], "U of state " + std::to_string(q) + " is singular (zbareover ZGETRF INFO)");
     }
+  // Species separability: every quasiparticle column of U and V has support
+  // in one species only, h columns per species.  Then the species blocks
+  // (columns reordered, protons first) are kept for the factored form.
+  const int n = c->n, h = c->h;
+  std::vector<double> Uf((size_t)nstates * 2 * h * h), Vf((size_t)nstates * 2 * h * h);
+  bool sep = true;
+  for (int q = 0; q < nstates && sep; ++q) {
+    const double *Uq = U + (size_t)q * nn, *Vq = V + (size_t)q * nn;
+    double scale = 0.0;
+    for (size_t e = 0; e < nn; ++e) scale = std::max(scale, std::max(std::fabs(Uq[e]), std::fabs(Vq[e])));
+    const double tol = 1e-14 * scale;
+    int cnt[2] = {0, 0};
+    for (int k = 0; k < n && sep; ++k) {
+      bool inp = false, inn = false;
+      for (int i = 0; i < n; ++i) {
+        const bool nz = std::fabs(Uq[i + (size_t)k * n]) > tol || std::fabs(Vq[i + (size_t)k * n]) > tol;
+        if (nz) (i < h ? inp : inn) = true;
+      }
+      if (inp && inn) { sep = false; break; }
+      const int spc = inn ? 1 : 0;
+      if (cnt[spc] >= h) { sep = false; break; }
+      const int col = cnt[spc]++;
+      for (int i = 0; i < h; ++i) {
+        Uf[((size_t)q * 2 + spc) * h * h + i + (size_t)col * h] = Uq[(spc * h + i) + (size_t)k * n];
+        Vf[((size_t)q * 2 + spc) * h * h + i + (size_t)col * h] = Vq[(spc * h + i) + (size_t)k * n];
+      }
+    }
+    if (sep && (cnt[0] != h || cnt[1] != h)) sep = false;
+  }
+  c->separable = sep;
+  if (sep) {
+    int r2 = 0;
+    if ((r2 = dupload(c, &c->d_Uf, Uf.data(), Uf.size())) || (r2 = dupload(c, &c->d_Vf, Vf.data(), Vf.size())))
+      return r2;
+  }
   return PNAMP_OK;
 }
+
+extern "C" int pnamp_set_form(pnamp_ctx *c, int form) {
+  if (!c) return PNAMP_E_ARG;
+  if (form != PNAMP_FORM_PLAIN && form != PNAMP_FORM_FACTORED) return fail(c, PNAMP_E_ARG, "unknown form");
+  c->form = form;
+  return PNAMP_OK;
+}
+
+extern "C" int pnamp_states_separable(pnamp_ctx *c) { return c ? (c->separable ? 1 : 0) : PNAMP_E_ARG; }
 
 extern "C" int pnamp_state_overlaps(pnamp_ctx *c, double *det) {
   if (!c || !det) return PNAMP_E_ARG;
@@ -1043,10 +1452,56 @@ extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0
   pp.blk_off = c->d_blk_off; pp.blk_sz = c->d_blk_sz; pp.blk_base = c->d_blk_base; pp.twom = c->d_twom;
   pp.q0 = c->d_q0; pp.q1 = c->d_q1; pp.vlist = c->d_vlist; pp.t1b = c->d_t1b;
   pp.zres = c->d_zres; pp.status = c->d_status;
-  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-  hipLaunchKernelGGL(k_point, dim3(c->NE, npairs), dim3(NT), 0, c->stream, pp);
-  HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  if (c->form == PNAMP_FORM_FACTORED) {
+    if (!c->separable)
+      return fail(c, PNAMP_E_ARG, "factored form needs states without proton-neutron mixing");
+    const int h = c->h, hh = h * h;
+    if (c->nphi > NPHI_MAX) return fail(c, PNAMP_E_SIZE, "factored form supports nphi <= 16");
+    const size_t comb_lds = sizeof(double2) * 2 * (size_t)c->nphi * hh;
+    if (comb_lds > 150 * 1024) return fail(c, PNAMP_E_SIZE, "2 nphi h^2 complex exceeds the combine LDS");
+    const size_t per_pair = (size_t)c->NE * 2 * c->nphi * hh;
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)npairs, ((size_t)4 << 30) / (per_pair * 16)));
+    if ((size_t)chunk > c->fac_cap) {
+      if ((r = dalloc(c, &c->d_frho, per_pair * chunk)) || (r = dalloc(c, &c->d_fout, (size_t)c->NE * 2 * c->nphi * 4 * chunk)) ||
+          (r = dalloc(c, &c->d_fstatus, (size_t)c->NE * 2 * chunk)))
+        return r;
+      c->fac_cap = chunk;
+    }
+    (void)hipFuncSetAttribute((const void *)k_fac_combine, hipFuncAttributeMaxDynamicSharedMemorySize, (int)comb_lds);
+    const size_t spec_lds = sizeof(double2) * 10 * (size_t)hh;
+    (void)hipFuncSetAttribute((const void *)k_fac_species, hipFuncAttributeMaxDynamicSharedMemorySize, (int)spec_lds);
+    FacParams fp;
+    fp.n = c->n; fp.h = h; fp.nalp = c->nalp; fp.nbet = c->nbet; fp.ngam = c->ngam; fp.nphi = c->nphi;
+    fp.npack = c->npack; fp.NE = c->NE; fp.cscale = c->cscale;
+    fp.alp = c->d_alp; fp.bet = c->d_bet; fp.gam = c->d_gam; fp.Uf = c->d_Uf; fp.Vf = c->d_Vf; fp.Z = c->d_Z;
+    fp.rtab = c->d_rtab; fp.pk_a = c->d_pk_a; fp.pk_b = c->d_pk_b; fp.blk_off = c->d_blk_off;
+    fp.blk_sz = c->d_blk_sz; fp.blk_base = c->d_blk_base; fp.twom = c->d_twom;
+    fp.vl0 = c->d_vs[0]; fp.vl1 = c->d_vs[1]; fp.nme0 = c->nme_s[0]; fp.nme1 = c->nme_s[1];
+    fp.t1b = c->d_t1b; fp.out = c->d_fout; fp.rho = c->d_frho; fp.status = c->d_fstatus;
+    CombParams cp;
+    cp.h = h; cp.nphi = c->nphi; cp.NE = c->NE; cp.zval = zval; cp.nval = nval;
+    cp.vac_scale = c->vac_scale; cp.snpfaf = c->snpfaf; cp.sgn_h = (h & 1) ? -1.0 : 1.0; cp.e0 = c->e0;
+    cp.det = c->d_det; cp.vpn = c->d_vs[2]; cp.nme_pn = c->nme_s[2];
+    cp.out = c->d_fout; cp.rho = c->d_frho; cp.fstatus = c->d_fstatus;
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    for (int p0 = 0; p0 < npairs; p0 += chunk) {
+      const int np = std::min(chunk, npairs - p0);
+      fp.q0 = c->d_q0 + p0; fp.q1 = c->d_q1 + p0;
+      cp.q0 = c->d_q0 + p0; cp.q1 = c->d_q1 + p0;
+      cp.zres = c->d_zres + (size_t)p0 * c->NE * NOBS;
+      cp.status = c->d_status + (size_t)p0 * c->NE;
+      hipLaunchKernelGGL(k_fac_species, dim3(c->NE, np, 2), dim3(NTF), spec_lds, c->stream, fp);
+      HIPCHK(c, hipGetLastError());
+      hipLaunchKernelGGL(k_fac_combine, dim3(c->NE, np), dim3(NTC), comb_lds, c->stream, cp);
+      HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  } else {
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    hipLaunchKernelGGL(k_point, dim3(c->NE, npairs), dim3(NT), 0, c->stream, pp);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  }
   ProjParams pr{c->nalp, c->nbet, c->ngam, c->NE, c->jmax, c->fold, c->npj, c->d_alp, c->d_walp, c->d_wbet,
                 c->d_gam, c->d_wgam, c->d_ptab, c->d_zres, c->d_status, c->d_kern, c->d_pair_status};
   hipLaunchKernelGGL(k_project, dim3(npairs, NOBS), dim3(256), 0, c->stream, pr);
@@ -1170,7 +1625,8 @@ extern "C" void pnamp_destroy(pnamp_ctx *c) {
   void *bufs[] = {c->d_alp, c->d_walp, c->d_bet, c->d_wbet, c->d_gam, c->d_wgam, c->d_rtab, c->d_ptab,
                   c->d_pk_a, c->d_pk_b, c->d_blk_off, c->d_blk_sz, c->d_blk_base, c->d_twoj, c->d_twom,
                   c->d_vlist, c->d_t1b, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status, c->d_q0,
-                  c->d_q1, c->d_zres, c->d_kern, c->d_status, c->d_pair_status};
+                  c->d_q1, c->d_zres, c->d_kern, c->d_status, c->d_pair_status, c->d_vs[0], c->d_vs[1],
+                  c->d_vs[2], c->d_Uf, c->d_Vf, c->d_fout, c->d_frho, c->d_fstatus};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto e : c->ev)

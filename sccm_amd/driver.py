@@ -1,0 +1,108 @@
+"""Host-side pieces around the library: the reference's PNAMP output layout,
+the Hill-Wheeler (GCM) assembly, and the case file of the Fortran host.
+
+* ``chop_kernels`` applies the CHOP rule of ``Kernel4Triaxial``
+  (``Codes/PNAMP/PNAMP_Kernel3D.f90:729-737``) to the library's projected sums
+  and returns the ``Kernel%njkk/hjkk/zz/nn`` quantities.
+* ``hw_matrices`` assembles the (q, K) norm and norm-weighted Hamiltonian
+  matrices of one J from the pair kernels (q <= q' computed, q > q' by
+  hermiticity, as the reference's symmetric-pair mode ``ICons = 0`` implies,
+  ``Codes/PNAMP/PNAMP_Main.f90:61-66``); ``gcm_energies`` solves them with the
+  library's HNDIAG (``Codes/GCM/GCM_HNDiag.f90``).
+* ``write_case`` / ``read_elem`` serve the Fortran host ``fortran/pnamp_driver``.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+CHOP = 1e-8
+
+
+def chop_kernels(kern: np.ndarray, ncore_p: int = 0, ncore_n: int = 0) -> dict:
+    """kern [5, J+1, 2J+1, 2J+1] -> njkk, hjkk, zz, nn (zero where |N| < CHOP)."""
+    N = kern[0]
+    ok = np.abs(N) >= CHOP
+    safe = np.where(ok, N, 1.0)
+    return dict(njkk=np.where(ok, N, 0), hjkk=np.where(ok, kern[4], 0),
+                zz=np.where(ok, kern[1] / safe + ncore_p, 0), nn=np.where(ok, kern[2] / safe + ncore_n, 0))
+
+
+def hw_matrices(kern_pairs: np.ndarray, pairs: np.ndarray, nstates: int, J: int, jmax: int, kvals=None):
+    """Real symmetric (N, H) over the basis (q, K), K in ``kvals`` (default -J..J)."""
+    if kvals is None:
+        kvals = list(range(-J, J + 1))
+    nk = len(kvals)
+    dim = nstates * nk
+    N = np.zeros((dim, dim), dtype=complex)
+    H = np.zeros((dim, dim), dtype=complex)
+    for (q0, q1), k in zip(pairs, kern_pairs):
+        for a, K1 in enumerate(kvals):
+            for c, K2 in enumerate(kvals):
+                nv = k[0, J, K1 + jmax, K2 + jmax]
+                hv = k[4, J, K1 + jmax, K2 + jmax]
+                N[q0 * nk + a, q1 * nk + c] = nv
+                H[q0 * nk + a, q1 * nk + c] = hv
+                if q0 != q1:
+                    N[q1 * nk + c, q0 * nk + a] = np.conj(nv)
+                    H[q1 * nk + c, q0 * nk + a] = np.conj(hv)
+    N = 0.5 * (N + N.conj().T)
+    H = 0.5 * (H + H.conj().T)
+    return N.real.copy(), H.real.copy(), float(np.abs(N.imag).max()), float(np.abs(H.imag).max())
+
+
+def gcm_energies(ctx, kern_pairs, pairs, nstates: int, jmax: int, eps: float = 1e-6, jlist=None,
+                 nlevels: int = 5) -> dict:
+    """{J: lowest ``nlevels`` Hill-Wheeler energies} (empty if no norm state)."""
+    out = {}
+    for J in (range(jmax + 1) if jlist is None else jlist):
+        N, H, _, _ = hw_matrices(kern_pairs, pairs, nstates, J, jmax)
+        if np.abs(N).max() < 1e-12:
+            out[J] = np.zeros(0)
+            continue
+        r = ctx.hw_solve(N, H, eps, want_f=False)
+        out[J] = r["E"][:nlevels] if r["ifl"] == 0 else np.zeros(0)
+    return out
+
+
+def write_case(path, basis, inter, mesh, U, V, pairs, coords, zval, nval, ncore=(0, 0)):
+    """Unformatted stream input of fortran/pnamp_driver."""
+    n = basis.n
+    nst = U.shape[0]
+    hdr = np.array([n, inter.nme, mesh.alp.size, mesh.bet.size, mesh.gam.size, mesh.nphi, mesh.jmax,
+                    mesh.fold, mesh.iscale, nst, len(pairs), zval, nval, ncore[0], ncore[1]], dtype=np.int32)
+    with open(path, "wb") as f:
+        f.write(hdr.tobytes())
+        f.write(np.array([mesh.cscale, inter.e0], dtype=np.float64).tobytes())
+        for a in (basis.t, basis.nq, basis.l, basis.twoj, basis.twom, inter.ka, inter.kb, inter.kc, inter.kd,
+                  np.asarray(pairs)[:, 0], np.asarray(pairs)[:, 1]):
+            f.write(np.ascontiguousarray(a, dtype=np.int32).tobytes())
+        f.write(np.asarray(inter.me, dtype=np.float64).tobytes())
+        f.write(np.asfortranarray(inter.t1b, dtype=np.float64).tobytes(order="F"))
+        for a in (mesh.alp, mesh.walp, mesh.bet, mesh.wbet, mesh.gam, mesh.wgam):
+            f.write(np.asarray(a, dtype=np.float64).tobytes())
+        f.write(np.ascontiguousarray(np.transpose(U, (0, 2, 1)), dtype=np.float64).tobytes())
+        f.write(np.ascontiguousarray(np.transpose(V, (0, 2, 1)), dtype=np.float64).tobytes())
+        f.write(np.ascontiguousarray(coords, dtype=np.float64).tobytes())
+
+
+def read_elem(path: str) -> dict:
+    """post_general file -> {J: {(K1, K2): (N, H, nn, zz)}}."""
+    lines = open(path).read().split("\n")
+    out, i = {}, 0
+
+    def cpx(s, k):
+        return complex(float(s[30 * k:30 * k + 15]), float(s[30 * k + 15:30 * k + 30]))
+
+    while i < len(lines) and lines[i].strip():
+        J = int(lines[i])
+        i += 2
+        k2max = J if J % 2 == 0 else abs(J - 1)
+        blk = {}
+        for _ in range((2 * k2max + 1) ** 2):
+            k1, k2 = (int(x) for x in lines[i].split())
+            nh = lines[i + 1]
+            nz = lines[i + 2]
+            blk[(k1, k2)] = (cpx(nh, 0), cpx(nh, 1), cpx(nz, 0), cpx(nz, 1))
+            i += 6
+        out[J] = blk
+    return out

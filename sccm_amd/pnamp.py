@@ -18,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpnamp_gpu.so")
 NOBS = 5
 OBS = ("norm", "Z", "N", "Ekin", "E")
+FORM_PLAIN, FORM_FACTORED = 0, 1
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
@@ -59,6 +60,10 @@ def load_library(path: str = LIB_PATH):
     L.pnamp_create.restype = C.c_int
     L.pnamp_set_states.argtypes = [vp, C.c_int, _f64p, _f64p]
     L.pnamp_set_states.restype = C.c_int
+    L.pnamp_set_form.argtypes = [vp, C.c_int]
+    L.pnamp_set_form.restype = C.c_int
+    L.pnamp_states_separable.argtypes = [vp]
+    L.pnamp_states_separable.restype = C.c_int
     L.pnamp_state_overlaps.argtypes = [vp, _f64p]
     L.pnamp_state_overlaps.restype = C.c_int
     L.pnamp_kernel_pair.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, _f64p, _f64p]
@@ -150,6 +155,14 @@ class PNAMP:
         Vf = np.ascontiguousarray(np.transpose(V, (0, 2, 1)))
         self._check(self._L.pnamp_set_states(self._h, U.shape[0], _p(Uf), _p(Vf)))
         self.nstates = U.shape[0]
+
+    def set_form(self, form: int):
+        """FORM_PLAIN (general) or FORM_FACTORED (states without p-n mixing)."""
+        self._check(self._L.pnamp_set_form(self._h, int(form)))
+
+    @property
+    def separable(self) -> bool:
+        return self._L.pnamp_states_separable(self._h) == 1
 
     def state_overlaps(self) -> np.ndarray:
         out = np.zeros(self.nstates)

@@ -159,3 +159,45 @@ def test_singular_state_is_reported():
         g.set_states(U, V)
     assert e.value.code > 0
     g.close()
+
+
+@pytest.mark.parametrize("space,zn,mesh_args", [
+    ("sd", (2, 2), (3, 4, 3, 5, 6)),
+    ("pf", (2, 4), (2, 3, 3, 9, 8)),
+    ("f5pg9", (4, 2), (2, 2, 2, 3, 4)),
+])
+def test_factored_form_matches_plain_and_oracle(oracle_mod, space, zn, mesh_args):
+    from sccm_amd.pnamp import FORM_FACTORED, FORM_PLAIN
+    b = make_basis(space)
+    inter = synthetic_interaction(b, 1.2, 201, 202)
+    mesh = make_mesh(*mesh_args)
+    Us, Vs = zip(*[synthetic_state(b, *zn, seed=s) for s in (21, 22, 23)])
+    U, V = np.array(Us), np.array(Vs)
+    g = _gpu(b, inter, mesh)
+    g.set_states(U, V)
+    assert g.separable
+    pairs = [(0, 1), (1, 2), (2, 2), (0, 0)]
+    g.set_form(FORM_PLAIN)
+    zp, kp = g.kernels_batched(pairs, *zn, want_zres=True)
+    g.set_form(FORM_FACTORED)
+    zf, kf = g.kernels_batched(pairs, *zn, want_zres=True)
+    for i in range(len(pairs)):
+        for io in range(5):
+            assert _rel(zf[i][:, io], zp[i][:, io]) < REL, (space, pairs[i], io)
+        assert _blockwise(kf[i], kp[i]) < REL
+    o = oracle_mod.Oracle(b, inter, mesh, *zn)
+    _, ko = o.kernel_pair(U[0], V[0], U[1], V[1])
+    assert _blockwise(kf[0], ko) < REL
+    g.close()
+
+
+def test_factored_form_rejects_mixed_states():
+    from sccm_amd.pnamp import FORM_FACTORED, PnampError
+    b, inter, U, V, betas, kern = ge76.load()
+    g = _gpu(b, inter, ge76.golden_mesh())
+    g.set_states(U[:2], V[:2])
+    assert not g.separable
+    g.set_form(FORM_FACTORED)
+    with pytest.raises(PnampError):
+        g.kernel_pair(0, 1, ge76.ZVAL, ge76.NVAL)
+    g.close()
