@@ -2,18 +2,22 @@
 
 Workload: the pf-space GCM (n = 40 m-states, Z,N valence 2,4, 20 HFB states,
 210 pairs q <= q', Euler mesh alpha = gamma 10, beta 20, 9 x 9 gauge points,
-J <= 8): 162,000 mesh points per pair, 34.02 M for the whole GCM.  A step is
-one batch of ``--pairs-per-step`` pairs (cycling through the 210) evaluated by
-the plain FP64 HIP path: rotation, Pfaffian norm, densities, energy, gauge sum
-and J-projection, inputs resident in HBM.
+J <= 8): 162,000 mesh points per pair, 34.02 M for the whole GCM.
 
-Prints one JSON line.  ``value`` is mesh points per second of the whole job
-(host wall clock around the timed steps, which includes the projection
-kernel and the copy of the projected kernels back to the host).
-``roofline`` is the mesh-point kernel's algorithmic FP64 rate from HIP events
-on the library's stream.  ``cpu_baseline`` times the CPU oracle (this
-project's C restatement of the reference call sequence) on a bounded sample
-of the same workload on the host cores.
+A step is one pass of the chosen form over ``--pairs-per-step`` pairs
+(default: all 210, i.e. the whole GCM) with inputs resident in HBM:
+rotation, per-species Pfaffians and densities, the proton-neutron energy
+contraction, gauge sums and the J-projection.  ``value`` is mesh points per
+second (host wall clock around the timed steps; it includes the projection
+and the copy of the projected kernels to the host).
+
+Beside it, untimed by the steps: one full pass of the plain FP64 path (the
+general drop-in) over all pairs, timed on its own, and the Hill-Wheeler
+energies of both, whose largest difference over the lowest levels of every
+J <= 8 is reported (the metric's |dE_GCM| condition).  ``roofline`` is the
+dominant kernel of the chosen form (HIP events on the library stream;
+algorithmic FP64 flops of the factored algorithm, see ``factored_flops``).
+``cpu_baseline`` times the CPU oracle on a bounded sample of the workload.
 """
 from __future__ import annotations
 
@@ -29,12 +33,23 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix peaks are equal)
 
 
-def flops_per_point(n: int, nme: int) -> float:
-    """Algorithmic real FP64 flops of one mesh point, the reference call
+def plain_flops_per_point(n: int, nme: int) -> float:
+    """Algorithmic real FP64 flops of one mesh point along the reference call
     sequence (complex multiply-add = 8 flops): 16 n^3 rotation (2 ZGEMM),
     16 n^3 Pfaf_Over inverse + ZGEMM, 19 n^3 Pfaffian of the 2n skew matrix,
     96 n^3 ROKAPPAphiomega (11 ZGEMM + inverse), 190 per interaction entry."""
     return 147.0 * n ** 3 + 190.0 * nme
+
+
+def factored_flops(h: int, sum_b2: int, nphi: int, l_species: int) -> float:
+    """Algorithmic real flops of k_fac_species per (pair, Euler point, species):
+    R V1, R* U1, R Z1 (block-sparse, 12 sum_b2 h), X, Y (8 h^3), M0 (8 sum_b2 h);
+    per gauge angle: Pfaffian of 2h x 2h (32 h^3 / 3), Gauss-Jordan on
+    [T | 1] (12 h^3), W1, W2 (16 h^3), rho, kappa10, kappa01 (12 h^3),
+    the species list (190 per entry)."""
+    setup = 12.0 * sum_b2 * h + 8.0 * h ** 3 + 8.0 * sum_b2 * h
+    per_phi = (32.0 / 3.0 + 12.0 + 16.0 + 12.0) * h ** 3 + 190.0 * l_species
+    return setup + nphi * per_phi
 
 
 def traffic_per_launch(form: str, pairs: int):
@@ -54,8 +69,6 @@ def log(*a):
 
 def cpu_baseline(w, threads: int):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
-
     import oracle
     from sccm_amd.mesh import Mesh
 
@@ -73,14 +86,33 @@ def cpu_baseline(w, threads: int):
                        f"x {sub.n_gauge} gauge = {pts} mesh points, {dt:.2f} s, OpenMP over Euler points")
 
 
+def gcm_compare(ctx, w, kern_a, kern_b, nlevels: int):
+    import numpy as np
+
+    from sccm_amd.driver import gcm_energies
+    ea = gcm_energies(ctx, kern_a, w.pairs, len(w.U), w.mesh.jmax, w.hw_eps, nlevels=nlevels)
+    eb = gcm_energies(ctx, kern_b, w.pairs, len(w.U), w.mesh.jmax, w.hw_eps, nlevels=nlevels)
+    worst, levels = 0.0, {}
+    for J in ea:
+        k = min(len(ea[J]), len(eb[J]))
+        if k:
+            d = float(np.abs(ea[J][:k] - eb[J][:k]).max())
+            worst = max(worst, d)
+            levels[str(J)] = [round(float(x), 6) for x in eb[J][:k]]
+    return worst, levels
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--pairs-per-step", type=int, default=16)
+    ap.add_argument("--pairs-per-step", type=int, default=0, help="0: all pairs")
     ap.add_argument("--nstates", type=int, default=20)
+    ap.add_argument("--form", choices=("factored", "plain"), default="factored")
+    ap.add_argument("--levels", type=int, default=3, help="lowest HW levels per J compared")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain reference pass and the GCM check")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -88,15 +120,19 @@ def main():
               "Hill-Wheeler solve); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
+
     from sccm_amd import configs
-    from sccm_amd.pnamp import PNAMP
+    from sccm_amd.pnamp import FORM_FACTORED, FORM_PLAIN, PNAMP
 
     log("bench: building the pf GCM workload")
     w = configs.pf_gcm(nstates=args.nstates)
     ctx = PNAMP(w.basis, w.inter, w.mesh)
     ctx.set_states(w.U, w.V)
-    P = args.pairs_per_step
+    form = FORM_FACTORED if args.form == "factored" else FORM_PLAIN
+    ctx.set_form(form)
     npairs = len(w.pairs)
+    P = args.pairs_per_step or npairs
 
     def batch(s):
         idx = [(s * P + i) % npairs for i in range(P)]
@@ -106,18 +142,30 @@ def main():
         ctx.kernels_batched(batch(s), w.zval, w.nval)
         log(f"bench: warmup step {s} done")
     k_ms = 0.0
+    kern_chosen = None
     t0 = time.perf_counter()
     for s in range(args.steps):
-        ctx.kernels_batched(batch(args.warmup + s), w.zval, w.nval)
+        k = ctx.kernels_batched(batch(args.warmup + s), w.zval, w.nval)
         ms, _ = ctx.timing()
         k_ms += ms
-        log(f"bench: step {s} k_point {ms:.1f} ms")
+        if P == npairs:
+            kern_chosen = k
+        log(f"bench: step {s} mesh-point kernels {ms:.1f} ms")
     elapsed = time.perf_counter() - t0
     pts_step = P * w.mesh.n_points
     value = args.steps * pts_step / elapsed
-    fpp = flops_per_point(w.n, w.inter.nme)
     k_avg = k_ms / args.steps
-    achieved = fpp * pts_step / (k_avg * 1e-3) / 1e12
+    h = w.n // 2
+    if form == FORM_FACTORED:
+        sum_b2 = sum(s * s for (_, s, _) in w.basis.blocks()) // 2
+        idx = np.stack([w.inter.ka, w.inter.kb, w.inter.kc, w.inter.kd], 1)
+        l_sp = ((idx <= h).all(1).sum() + (idx > h).all(1).sum()) / 2.0
+        fl = factored_flops(h, sum_b2, w.mesh.nphi, int(l_sp)) * 2 * w.mesh.n_euler * P
+        kname, tform = "k_fac_species + k_fac_combine", "factored_fp64"
+    else:
+        fl = plain_flops_per_point(w.n, w.inter.nme) * pts_step
+        kname, tform = "k_point", "plain_fp64"
+    achieved = fl / (k_avg * 1e-3) / 1e12
     out = {
         "metric": "mesh-pts/s: mixed+batched+surrogate vs plain FP64 GPU at |dE_GCM|<1 keV",
         "value": value,
@@ -134,7 +182,7 @@ def main():
         "config": {
             "workload": "pf_gcm: pf space n=40, Z,N=2,4, 20 HFB states / 210 pairs, "
                         "alpha=gamma 10 x beta 20 Euler x 9x9 gauge = 162000 mesh points per pair, J<=8",
-            "form": "plain_fp64",
+            "form": tform,
             "pairs_per_step": P,
             "mesh_points_per_step": pts_step,
         },
@@ -144,12 +192,30 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": traffic_per_launch("plain_fp64", P),
-            "kernel": "k_point",
-            "flops_per_point": fpp,
+            "traffic": traffic_per_launch(tform, P),
+            "kernel": kname,
+            "algorithmic_flops_per_launch": fl,
             "kernel_ms_per_launch": k_avg,
         },
     }
+    if form == FORM_FACTORED and not args.no_plain:
+        log("bench: plain FP64 reference pass over all pairs")
+        ctx.set_form(FORM_PLAIN)
+        t = time.perf_counter()
+        kern_plain = ctx.kernels_batched(w.pairs, w.zval, w.nval)
+        tp = time.perf_counter() - t
+        plain_value = npairs * w.mesh.n_points / tp
+        out["plain_fp64"] = {"value": plain_value, "unit": "mesh-pts/s", "pairs": npairs,
+                             "kernel_ms": ctx.timing()[0]}
+        out["speedup_vs_plain"] = value / plain_value
+        if kern_chosen is None:
+            ctx.set_form(form)
+            kern_chosen = ctx.kernels_batched(w.pairs, w.zval, w.nval)
+        rel = float(np.abs(kern_chosen - kern_plain).max() / np.abs(kern_plain).max())
+        dE, levels = gcm_compare(ctx, w, kern_chosen, kern_plain, args.levels)
+        out["gcm"] = {"max_abs_dE_keV": 1e3 * dE, "levels_compared_per_J": args.levels,
+                      "hw_norm_cutoff": w.hw_eps, "max_rel_kernel_diff": rel,
+                      "energies_plain_MeV": levels}
     if not args.no_cpu:
         threads = min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 16)
         log(f"bench: CPU oracle sample on {threads} threads")

@@ -838,20 +838,23 @@ struct CombParams {
   double vac_scale, snpfaf, sgn_h, e0;
   const double *det;
   const int *q0, *q1;
-  const VEntry *vpn;                // pn list: a,c proton-local; b,d neutron-local; pad = partner flag
-  int nme_pn;
+  const int *w_ptr, *w_col;         // W_pn in CSR over x = c + a h (rows), y = d + b h (columns)
+  const double *w_val;
   const double2 *out, *rho;
   const int *fstatus;
   double2 *zres;                    // [npairs][NE][NOBS]
   int *status;                      // [npairs][NE]
 };
 
-// One workgroup per (pair, Euler point): E_pn for all gauge pairs from the
-// pn entries of the list (for a in p, b in n the eight-term sum of
-// HFB_Energy_COMPLEX reduces to V [rho_p(c,a) rho_n(d,b) + d rho_p(a,c) rho_n(b,d)]),
-// then the nphi^2 gauge points are summed into ZRES.
+// One workgroup per (pair, Euler point).  The proton-neutron energy is the
+// bilinear form E_pn(phi_p, phi_n) = rho_p(phi_p)^T W rho_n(phi_n) with
+// W[(c,a),(d,b)] = V_abcd and W[(a,c),(b,d)] += V_abcd for list entries with
+// a + b != c + d (for a, c protons and b, d neutrons the eight-term sum of
+// HFB_Energy_COMPLEX reduces to V [rho_p(c,a) rho_n(d,b) + rho_p(a,c) rho_n(b,d)]).
+// G = W rho_n(phi_n) is formed once per phi_n from the CSR rows of W, then
+// the nphi^2 dot products with rho_p(phi_p); the gauge points are summed into ZRES.
 __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
-  extern __shared__ double2 dsm[];  // rho_p [nphi][h^2], rho_n [nphi][h^2]
+  extern __shared__ double2 dsm[];  // rho_p [nphi][h^2], rho_n -> G [nphi][h^2]
   __shared__ double2 epn[NPHI_MAX * NPHI_MAX];
   __shared__ double red[NTC / 64 * 10];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -864,20 +867,42 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
     rn[e] = p.rho[(base + 1) * nphi * hh + e];
   }
   __syncthreads();
+  // G rows x = tid, tid + NTC (h^2 <= 2 NTC), all phi_n, in registers
+  constexpr int RPT = 2;
+  double2 g[RPT][NPHI_MAX];
+  for (int r = 0; r < RPT; ++r) {
+    for (int f = 0; f < NPHI_MAX; ++f) g[r][f] = cz();
+    const int x = tid + r * NTC;
+    if (x >= hh) continue;
+    for (int q = p.w_ptr[x]; q < p.w_ptr[x + 1]; ++q) {
+      const int y = p.w_col[q];
+      const double v = p.w_val[q];
+      for (int f = 0; f < NPHI_MAX; ++f) {
+        if (f >= nphi) break;
+        rfma(g[r][f], v, rn[(size_t)f * hh + y]);
+      }
+    }
+  }
+  __syncthreads();
+  for (int r = 0; r < RPT; ++r) {
+    const int x = tid + r * NTC;
+    if (x >= hh) continue;
+    for (int f = 0; f < NPHI_MAX; ++f) {
+      if (f >= nphi) break;
+      rn[(size_t)f * hh + x] = g[r][f];
+    }
+  }
+  __syncthreads();
   const int grp = tid >> 5, gl = tid & 31;
   for (int fp = grp; fp < nphi; fp += NTC / 32) {
     double2 acc[NPHI_MAX];
     for (int f = 0; f < NPHI_MAX; ++f) acc[f] = cz();
     const double2 *rpf = rp + (size_t)fp * hh;
-    for (int q = gl; q < p.nme_pn; q += 32) {
-      const VEntry ve = p.vpn[q];
-      const int a = ve.idx & 255, b = (ve.idx >> 8) & 255, c = (ve.idx >> 16) & 255, d = ve.idx >> 24;
-      const double2 x = cscale(ve.me, rpf[c + a * h]);
-      const double2 xp = cscale(ve.pad ? ve.me : 0.0, rpf[a + c * h]);
+    for (int x = gl; x < hh; x += 32) {
+      const double2 a = rpf[x];
       for (int f = 0; f < NPHI_MAX; ++f) {
         if (f >= nphi) break;
-        cfma(acc[f], x, rn[(size_t)f * hh + d + b * h]);
-        cfma(acc[f], xp, rn[(size_t)f * hh + b + d * h]);
+        cfma(acc[f], a, rn[(size_t)f * hh + x]);
       }
     }
     for (int f = 0; f < nphi; ++f) {
@@ -1170,7 +1195,13 @@ struct pnamp_ctx {
   double *d_Uf = nullptr, *d_Vf = nullptr;
   double2 *d_fout = nullptr, *d_frho = nullptr;
   int *d_fstatus = nullptr;
+  int *d_wptr = nullptr, *d_wcol = nullptr;
+  double *d_wval = nullptr;
   size_t fac_cap = 0;  // pairs the factored scratch holds
+  // captured launch sequence of the factored form, valid for (npairs, zval,
+  // nval) and the current buffers; dropped whenever a buffer is reallocated
+  hipGraphExec_t fac_graph = nullptr;
+  int g_np = -1, g_z = 0, g_n = 0;
 };
 
 #define HIPCHK(ctx, x)                                                       \
@@ -1187,8 +1218,17 @@ static int fail(pnamp_ctx *c, int code, const std::string &msg) {
   return code;
 }
 
+static void drop_graph(pnamp_ctx *c) {
+  if (c && c->fac_graph) {
+    (void)hipGraphExecDestroy(c->fac_graph);
+    c->fac_graph = nullptr;
+    c->g_np = -1;
+  }
+}
+
 template <class T>
 static int dalloc(pnamp_ctx *c, T **p, size_t count) {
+  drop_graph(c);
   if (*p) {
     (void)hipFree(*p);
     *p = nullptr;
@@ -1330,6 +1370,29 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
       c->nme_s[k] = (int)ls[k].size();
       if ((r = dupload(c, &c->d_vs[k], ls[k].data(), ls[k].size()))) return bail(r, c->err);
     }
+    // W_pn as CSR: rows x = c + a h, columns y = d + b h (species-local indices)
+    const int hh = h * h;
+    std::vector<std::vector<std::pair<int, double>>> rows(hh);
+    auto add = [&](int x, int y, double v) {
+      for (auto &e : rows[x])
+        if (e.first == y) { e.second += v; return; }
+      rows[x].push_back({y, v});
+    };
+    for (const VEntry &e : ls[2]) {
+      const int a = e.idx & 255, b = (e.idx >> 8) & 255, cc = (e.idx >> 16) & 255, d = e.idx >> 24;
+      add(cc + a * h, d + b * h, e.me);
+      if (e.pad) add(a + cc * h, b + d * h, e.me);
+    }
+    std::vector<int> wptr(hh + 1, 0), wcol;
+    std::vector<double> wval;
+    for (int x = 0; x < hh; ++x) {
+      std::sort(rows[x].begin(), rows[x].end());
+      for (auto &e : rows[x]) { wcol.push_back(e.first); wval.push_back(e.second); }
+      wptr[x + 1] = (int)wcol.size();
+    }
+    if ((r = dupload(c, &c->d_wptr, wptr.data(), wptr.size())) || (r = dupload(c, &c->d_wcol, wcol.data(), wcol.size())) ||
+        (r = dupload(c, &c->d_wval, wval.data(), wval.size())))
+      return bail(r, c->err);
   }
   if ((r = dalloc(c, &c->d_rtab, (size_t)c->nbet * c->npack)) || (r = dalloc(c, &c->d_ptab, (size_t)c->nbet * 2 * c->npj)))
     return bail(r, c->err);
@@ -1459,6 +1522,7 @@ extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0
     if (c->nphi > NPHI_MAX) return fail(c, PNAMP_E_SIZE, "factored form supports nphi <= 16");
     const size_t comb_lds = sizeof(double2) * 2 * (size_t)c->nphi * hh;
     if (comb_lds > 150 * 1024) return fail(c, PNAMP_E_SIZE, "2 nphi h^2 complex exceeds the combine LDS");
+    if (hh > 2 * NTC) return fail(c, PNAMP_E_SIZE, "h^2 exceeds the combine rows per workgroup");
     const size_t per_pair = (size_t)c->NE * 2 * c->nphi * hh;
     const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)npairs, ((size_t)4 << 30) / (per_pair * 16)));
     if ((size_t)chunk > c->fac_cap) {
@@ -1481,20 +1545,38 @@ extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0
     CombParams cp;
     cp.h = h; cp.nphi = c->nphi; cp.NE = c->NE; cp.zval = zval; cp.nval = nval;
     cp.vac_scale = c->vac_scale; cp.snpfaf = c->snpfaf; cp.sgn_h = (h & 1) ? -1.0 : 1.0; cp.e0 = c->e0;
-    cp.det = c->d_det; cp.vpn = c->d_vs[2]; cp.nme_pn = c->nme_s[2];
+    cp.det = c->d_det; cp.w_ptr = c->d_wptr; cp.w_col = c->d_wcol; cp.w_val = c->d_wval;
     cp.out = c->d_fout; cp.rho = c->d_frho; cp.fstatus = c->d_fstatus;
+    if (c->fac_graph && (c->g_np != npairs || c->g_z != zval || c->g_n != nval)) drop_graph(c);
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    for (int p0 = 0; p0 < npairs; p0 += chunk) {
-      const int np = std::min(chunk, npairs - p0);
-      fp.q0 = c->d_q0 + p0; fp.q1 = c->d_q1 + p0;
-      cp.q0 = c->d_q0 + p0; cp.q1 = c->d_q1 + p0;
-      cp.zres = c->d_zres + (size_t)p0 * c->NE * NOBS;
-      cp.status = c->d_status + (size_t)p0 * c->NE;
-      hipLaunchKernelGGL(k_fac_species, dim3(c->NE, np, 2), dim3(NTF), spec_lds, c->stream, fp);
-      HIPCHK(c, hipGetLastError());
-      hipLaunchKernelGGL(k_fac_combine, dim3(c->NE, np), dim3(NTC), comb_lds, c->stream, cp);
-      HIPCHK(c, hipGetLastError());
+    if (!c->fac_graph) {
+      // capture the species / combine launches of all chunks once; later calls
+      // with the same shape replay the graph (the pair lists are copied into
+      // the same device buffers before each replay)
+      hipGraph_t graph = nullptr;
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      for (int p0 = 0; p0 < npairs; p0 += chunk) {
+        const int np = std::min(chunk, npairs - p0);
+        fp.q0 = c->d_q0 + p0; fp.q1 = c->d_q1 + p0;
+        cp.q0 = c->d_q0 + p0; cp.q1 = c->d_q1 + p0;
+        cp.zres = c->d_zres + (size_t)p0 * c->NE * NOBS;
+        cp.status = c->d_status + (size_t)p0 * c->NE;
+        hipLaunchKernelGGL(k_fac_species, dim3(c->NE, np, 2), dim3(NTF), spec_lds, c->stream, fp);
+        hipLaunchKernelGGL(k_fac_combine, dim3(c->NE, np), dim3(NTC), comb_lds, c->stream, cp);
+      }
+      const hipError_t cap = hipStreamEndCapture(c->stream, &graph);
+      if (cap != hipSuccess) return fail(c, PNAMP_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(cap));
+      const hipError_t ins = hipGraphInstantiate(&c->fac_graph, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ins != hipSuccess) {
+        c->fac_graph = nullptr;
+        return fail(c, PNAMP_E_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ins));
+      }
+      c->g_np = npairs;
+      c->g_z = zval;
+      c->g_n = nval;
     }
+    HIPCHK(c, hipGraphLaunch(c->fac_graph, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   } else {
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
@@ -1622,11 +1704,13 @@ extern "C" const char *pnamp_last_error(pnamp_ctx *c) { return c ? c->err.c_str(
 
 extern "C" void pnamp_destroy(pnamp_ctx *c) {
   if (!c) return;
+  drop_graph(c);
   void *bufs[] = {c->d_alp, c->d_walp, c->d_bet, c->d_wbet, c->d_gam, c->d_wgam, c->d_rtab, c->d_ptab,
                   c->d_pk_a, c->d_pk_b, c->d_blk_off, c->d_blk_sz, c->d_blk_base, c->d_twoj, c->d_twom,
                   c->d_vlist, c->d_t1b, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status, c->d_q0,
                   c->d_q1, c->d_zres, c->d_kern, c->d_status, c->d_pair_status, c->d_vs[0], c->d_vs[1],
-                  c->d_vs[2], c->d_Uf, c->d_Vf, c->d_fout, c->d_frho, c->d_fstatus};
+                  c->d_vs[2], c->d_Uf, c->d_Vf, c->d_fout, c->d_frho, c->d_fstatus, c->d_wptr,
+                  c->d_wcol, c->d_wval};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto e : c->ev)

@@ -189,6 +189,41 @@ def synthetic_interaction(b: Basis, tbme_sigma: float, seed_tbme: int,
                        idx[:, 3].copy(), me, t1b, 0.0)
 
 
+def parse_jscheme_tbme(text: str, b: Basis) -> dict:
+    """Shell-model J-scheme file ``t1..t4, n1..n4, lj1..lj4, J, V`` (comma or
+    blank separated; t = 1 proton, 0 neutron; lj = (2l + 2j - 1)/2), as read by
+    ``Codes/PNAMP/IMSRG_GenerateH_JTMT.f90:108``, into the channel table of
+    ``uncouple``.  The files store identical-orbit elements unnormalised:
+    V_norm = V / sqrt((1 + d_ab)(1 + d_cd)).  Pairs are put in ascending
+    orbit order (protons first) with the exchange phase -(-1)^(ja + jb - J)."""
+    norb = len(b.orbits)
+    lj_of = [(2 * l + tj - 1) // 2 for (_, l, tj) in b.orbits]
+    tj_of = [tj for (_, _, tj) in b.orbits]
+
+    def go(t, lj):
+        return lj_of.index(lj) + (0 if t == 1 else norb)
+
+    table = {}
+    for line in text.splitlines():
+        f = line.replace(",", " ").split()
+        if len(f) < 14:
+            continue
+        t1, t2, t3, t4, _, _, _, _, l1, l2, l3, l4, J = (int(x) for x in f[:13])
+        V = float(f[13])
+        a, bb, c, d = go(t1, l1), go(t2, l2), go(t3, l3), go(t4, l4)
+        ja, jb, jc, jd = (tj_of[x % norb] for x in (a, bb, c, d))
+        v = V / sqrt((1 + (a == bb)) * (1 + (c == d)))
+        if a > bb:
+            a, bb = bb, a
+            v *= -(-1) ** ((ja + jb) // 2 - J)
+        if c > d:
+            c, d = d, c
+            v *= -(-1) ** ((jc + jd) // 2 - J)
+        table[(2 * J, a, bb, c, d)] = v
+        table[(2 * J, c, d, a, bb)] = v
+    return table
+
+
 def read_mscheme_binary(path: str) -> np.ndarray:
     """Fortran sequential unformatted list, one real*8 per record
     (``HFBA_Hamiltonian.f90:373-382``)."""

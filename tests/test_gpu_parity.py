@@ -201,3 +201,22 @@ def test_factored_form_rejects_mixed_states():
     with pytest.raises(PnampError):
         g.kernel_pair(0, 1, ge76.ZVAL, ge76.NVAL)
     g.close()
+
+
+def test_factored_graph_replays_match():
+    """The factored form replays a captured hipGraph on calls of the same
+    shape; every replay, with eager work of the plain form in between, must
+    equal the first (capturing) call bit for bit."""
+    from sccm_amd.pnamp import FORM_FACTORED, FORM_PLAIN
+    w = configs.pf_gcm(nstates=4, nalp=2, nbet=3, nphi=5, jmax=4)
+    g = _gpu(w.basis, w.inter, w.mesh)
+    g.set_states(w.U, w.V)
+    g.set_form(FORM_FACTORED)
+    first = g.kernels_batched(w.pairs, w.zval, w.nval)
+    for rep in range(3):
+        g.set_form(FORM_PLAIN)
+        g.kernels_batched(w.pairs[:3], w.zval, w.nval)
+        g.set_form(FORM_FACTORED)
+        again = g.kernels_batched(w.pairs, w.zval, w.nval)
+        assert np.array_equal(again, first), rep
+    g.close()

@@ -50,3 +50,20 @@ def test_workload_shapes():
     w1 = configs.pf_gcm(nstates=20)
     assert len(w1.pairs) == 210 and w1.mesh.n_points == 162000
     assert w1.mesh_points == 34020000
+
+
+def test_converter_reproduces_reference_mscheme_list():
+    """Known answer: uncoupling the reference's J-scheme GCN2850 file
+    (Codes/int/GCN2850/SM_GCN2850_ME2B.dat) reproduces every record of its
+    binary m-scheme list IMSRG_s000_GCN2850pf5g9_hwHO10_m1m2m3m4.dat."""
+    import os
+
+    import ge76
+    from sccm_amd.interaction import parse_jscheme_tbme, uncouple
+    b = make_basis("f5pg9")
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gcn2850")
+    table = parse_jscheme_tbme(open(os.path.join(here, "SM_GCN2850_ME2B.dat")).read(), b)
+    me = uncouple(b, table)
+    _, inter, _, _, _, _ = ge76.load()
+    assert me.size == inter.me.size == 10330
+    assert np.abs(me - inter.me).max() < 1e-9
