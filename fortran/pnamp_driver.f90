@@ -73,12 +73,12 @@ contains
     end if
   end subroutine
 
-  ! post_general (PNAMP_IO.f90:39-94): norm, norm-weighted H, <N>, <Z>, and
-  ! the not-computed <J^2> / Q slots as zeros; |N| < CHOP = 1e-8 -> zeros.
+  ! post_general (PNAMP_IO.f90:39-94): norm, norm-weighted H, <N>, <Z>, <J^2>,
+  ! and the not-computed Q slots as zeros; |N| < CHOP = 1e-8 -> zeros.
   subroutine post_general(ip)
     integer, intent(in) :: ip
     integer :: jproj, k1, k2, k2max, lou
-    complex(c_double_complex) :: zn, zh, znn, zzz
+    complex(c_double_complex) :: zn, zh, znn, zzz, zj2
     character(len=600) :: fname
     real(c_double), parameter :: chop = 1.d-8
     write(fname, '(a,a,i4.4,a,i4.4,a)') trim(outdir), '/kern.', q0(ip), '.', q1(ip), '.elem'
@@ -95,17 +95,18 @@ contains
         do k1 = -k2max, k2max
           write(lou, '(10i5)') k1, k2
           zn = kern(k2, k1, jproj, 1, ip)
-          zh = (0.d0, 0.d0); znn = (0.d0, 0.d0); zzz = (0.d0, 0.d0)
+          zh = (0.d0, 0.d0); znn = (0.d0, 0.d0); zzz = (0.d0, 0.d0); zj2 = (0.d0, 0.d0)
           if (abs(zn) >= chop) then
             zh = kern(k2, k1, jproj, 5, ip)
             zzz = kern(k2, k1, jproj, 2, ip) / zn + ncp
             znn = kern(k2, k1, jproj, 3, ip) / zn + ncn
+            zj2 = kern(k2, k1, jproj, 6, ip) / zn
           else
             zn = (0.d0, 0.d0)
           end if
           write(lou, '(8e15.8)') zn, zh
           write(lou, '(8e15.8)') znn, zzz
-          write(lou, '(8e15.8)') (0.d0, 0.d0)
+          write(lou, '(8e15.8)') zj2
           write(lou, '(8e15.8)') (0.d0, 0.d0), (0.d0, 0.d0)
           write(lou, '(8e15.8)') (0.d0, 0.d0), (0.d0, 0.d0)
         end do

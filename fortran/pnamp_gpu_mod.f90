@@ -14,7 +14,7 @@ module pnamp_gpu
   public :: pnamp_hw_solve, pnamp_last_timing, pnamp_last_error_f, pnamp_destroy
   public :: PNAMP_NOBS
 
-  integer(c_int), parameter :: PNAMP_NOBS = 5
+  integer(c_int), parameter :: PNAMP_NOBS = 6
 
   type, bind(C) :: pnamp_basis_t
     integer(c_int) :: n

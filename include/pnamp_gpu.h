@@ -18,8 +18,8 @@
  *  - complex outputs are interleaved (re, im) doubles.
  *  - kern layout: [PNAMP_NOBS][jmax+1][2 jmax+1][2 jmax+1] complex with
  *    index (io, J, K1 + jmax, K2 + jmax); io = 0 norm, 1 Z, 2 N, 3 E_kin,
- *    4 E + E0 (the ZRES slots 1, 2, 3, 4, 7 of Kernel4Triaxial), each the
- *    projected *norm-weighted* sum, like ZsumJKK in the reference.
+ *    4 E + E0, 5 <J^2> (the ZRES slots 1, 2, 3, 4, 7, 8 of Kernel4Triaxial),
+ *    each the projected *norm-weighted* sum, like ZsumJKK in the reference.
  *  - zres layout: [n_euler][PNAMP_NOBS] complex, Euler points enumerated
  *    beta outer, alpha, gamma inner (ieuler_ang of the reference).
  *
@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define PNAMP_NOBS 5
+#define PNAMP_NOBS 6
 #define PNAMP_NMAX 44
 
 enum {

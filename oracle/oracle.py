@@ -14,7 +14,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libpnamp_oracle.so")
-NOBS = 5
+NOBS = 6
 
 _i32p = C.POINTER(C.c_int)
 _f64p = C.POINTER(C.c_double)

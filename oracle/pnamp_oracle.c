@@ -521,8 +521,58 @@ void or_energy_field(const or_cfg *c, const zc *ro, const zc *k10, const zc *k01
   free(G); free(D10);
 }
 
-/* Per-gauge-point observables: norm, Z, N, Ekin, E+E0 (the ZRES slots 1,2,3,4,7). */
-#define NOBS 5
+/* One-body angular momentum matrices <a|J_k|b> (k = x, y, z) and their
+ * squares, the cME1B%ZJ*ME arrays the reference builds in SingleParticleME
+ * (Codes/PNAMP/PNAMP_ME1B.f90:138-216): non-zero inside one orbit. */
+static void jmats(const or_cfg *c, zc *J, zc *J2) {
+  int n = c->n;
+  size_t nn = (size_t)n * n;
+  zc *Jp = calloc(nn, sizeof(zc));
+  for (size_t i = 0; i < 3 * nn; ++i) J[i] = 0;
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < n; ++b) {
+      if (c->t[a] != c->t[b] || c->nq[a] != c->nq[b] || c->l[a] != c->l[b] || c->twoj[a] != c->twoj[b]) continue;
+      double j = c->twoj[a] / 2.0, ma = c->twom[a] / 2.0, mb = c->twom[b] / 2.0;
+      if (fabs(ma - mb - 1.0) < 1e-9) Jp[IX(a, b, n)] = sqrt(j * (j + 1) - mb * (mb + 1));
+      if (a == b) J[2 * nn + IX(a, a, n)] = ma;
+    }
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < n; ++b) {
+      zc p = Jp[IX(a, b, n)], m = Jp[IX(b, a, n)];
+      J[IX(a, b, n)] = 0.5 * (p + m);
+      J[nn + IX(a, b, n)] = (p - m) / (2.0 * I);
+    }
+  for (int k = 0; k < 3; ++k) zgemm('n', 'n', n, n, n, 1.0, J + k * nn, n, J + k * nn, n, 0.0, J2 + k * nn, n);
+  free(Jp);
+}
+
+/* ZJ2_0 (Codes/PNAMP/PNAMP_ME2B.f90:1-152): <J^2> = sum_k Tr(J_k^2 rho)
+ * + Tr(J_k rho)^2 - Tr(J_k rho J_k rho) + Tr(J_k kappa10 J_k^T kappa01^T). */
+zc or_j2(const or_cfg *c, const zc *J, const zc *J2, const zc *ro, const zc *k10, const zc *k01) {
+  int n = c->n;
+  size_t nn = (size_t)n * n;
+  zc *a1 = malloc(sizeof(zc) * nn), *a2 = malloc(sizeof(zc) * nn), *a3 = malloc(sizeof(zc) * nn),
+     *a4 = malloc(sizeof(zc) * nn), *a5 = malloc(sizeof(zc) * nn), *a6 = malloc(sizeof(zc) * nn);
+  zc tot = 0;
+  for (int k = 0; k < 3; ++k) {
+    zgemm('n', 'n', n, n, n, 1.0, J2 + k * nn, n, ro, n, 0.0, a1, n);
+    zgemm('n', 'n', n, n, n, 1.0, J + k * nn, n, ro, n, 0.0, a2, n);
+    zgemm('n', 'n', n, n, n, 1.0, J + k * nn, n, k10, n, 0.0, a3, n);
+    zgemm('t', 't', n, n, n, 1.0, J + k * nn, n, k01, n, 0.0, a4, n);
+    zgemm('n', 'n', n, n, n, 1.0, a2, n, a2, n, 0.0, a5, n);
+    zgemm('n', 'n', n, n, n, 1.0, a3, n, a4, n, 0.0, a6, n);
+    zc t1 = 0, t2 = 0, t3 = 0, t4 = 0;
+    for (int i = 0; i < n; ++i) {
+      t1 += a1[IX(i, i, n)]; t2 += a2[IX(i, i, n)]; t3 += a5[IX(i, i, n)]; t4 += a6[IX(i, i, n)];
+    }
+    tot += t1 + t2 * t2 - t3 + t4;
+  }
+  free(a1); free(a2); free(a3); free(a4); free(a5); free(a6);
+  return tot;
+}
+
+/* Per-gauge-point observables: norm, Z, N, Ekin, E+E0, J^2 (the ZRES slots 1,2,3,4,7,8). */
+#define NOBS 6
 
 /* One Euler point: the Fomenko gauge loop of Kernel4Triaxial
  * (PNAMP_Kernel3D.f90:245-387), returns ZRES(iang, :) for the five observables. */
@@ -533,7 +583,8 @@ static int euler_point(const or_cfg *c, double alp, double bet, double gam, cons
   size_t nn = (size_t)n * n;
   zc *R = malloc(sizeof(zc) * nn), *Ub = malloc(sizeof(zc) * nn), *Vb = malloc(sizeof(zc) * nn),
      *A = malloc(sizeof(zc) * nn), *ro = malloc(sizeof(zc) * nn), *k10 = malloc(sizeof(zc) * nn),
-     *k01 = malloc(sizeof(zc) * nn);
+     *k01 = malloc(sizeof(zc) * nn), *J = malloc(sizeof(zc) * 3 * nn), *J2 = malloc(sizeof(zc) * 3 * nn);
+  jmats(c, J, J2);
   or_rotation_mat(c, alp, bet, gam, R);
   double aaa = (double)n * (n + 1) / 2.0;
   double snpfaf = ((long)aaa % 2) ? -1.0 : 1.0;
@@ -567,10 +618,11 @@ static int euler_point(const or_cfg *c, double alp, double bet, double gam, cons
       sum[2] += znorm * zn;
       sum[3] += znorm * ek;
       sum[4] += znorm * (eh + c->e0);
+      sum[5] += znorm * or_j2(c, J, J2, ro, k10, k01);
     }
   }
   for (int io = 0; io < NOBS; ++io) zres[io] = sum[io] / (double)(nf * nf);
-  free(R); free(Ub); free(Vb); free(A); free(ro); free(k10); free(k01);
+  free(R); free(Ub); free(Vb); free(A); free(ro); free(k10); free(k01); free(J); free(J2);
   return info;
 }
 

@@ -239,30 +239,38 @@ __device__ __forceinline__ void wave_best(double &v, int &k) {
   }
 }
 
-// Pfaffian of the N2 x N2 skew matrix whose upper triangle is in S
-// (column-major, leading dimension N2), by ZPfaffianF's algorithm
-// (Codes/PNAMP/PNAMP_Proj.f90:218-350): at step IB the largest |S(i,j)| of
-// the remaining upper triangle (first in row-major order on ties) is moved to
-// (NR, NC) with the two ZExch exchanges, the rank-2 Schur update follows, and
-// a pivot of modulus <= 1e-13 gives Pf = 0.  S is destroyed.  All NTH threads
-// of the block must call it; redv/redk need NTH/64 entries.
+// Pfaffian of the N2 x N2 skew matrix S (both triangles stored, column-major,
+// leading dimension N2), by ZPfaffianF's algorithm
+// (Codes/PNAMP/PNAMP_Proj.f90:218-350): at step IB the largest |S(i,j)| of the
+// remaining upper triangle (first in row-major order on ties) is brought to
+// (NR, NC) by the two row/column exchanges of ZExch, the rank-2 Schur update
+// follows, and a pivot of modulus <= 1e-13 gives Pf = 0.  The exchanges are
+// kept as a logical -> physical index map instead of moving data (two maps of
+// N2 ints in mpbuf, read from one and written to the other each step), so a
+// step costs two barriers.  S is destroyed.  All NTH threads of the block must
+// call it; redv/redk need NTH/64 entries.
 template <int NTH>
-__device__ double2 block_pfaffian(double2 *S, const int N2, double *redv, int *redk) {
+__device__ double2 block_pfaffian(double2 *S, const int N2, int *mpbuf, double *redv, int *redk) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   constexpr int NW = NTH / 64;
+  for (int i = tid; i < N2; i += NTH) mpbuf[i] = i;
+  __syncthreads();
   double2 pf = make_double2(1.0, 0.0);
   for (int IB = 0; IB < N2 / 2 - 1; ++IB) {
-    const int NR = 2 * IB, NC = NR + 1, m = N2 - NR;
+    const int NR = 2 * IB, NC = NR + 1;
+    const int *mp = mpbuf + (IB & 1) * N2;
+    int *mpn = mpbuf + ((IB + 1) & 1) * N2;
     double bv = -1.0;
     int bk = 0x7fffffff;
-    for (int j = NR + 1 + tid / 64; j < N2; j += NW)
+    for (int j = NR + 1 + tid / 64; j < N2; j += NW) {
+      const int pj = mp[j] * N2;
       for (int i = NR + lane; i < j; i += 64) {
-        const double2 v = S[i + j * N2];
+        const double2 v = S[mp[i] + pj];
         const double a = v.x * v.x + v.y * v.y;
         const int key = i * N2 + j;
         if (a > bv || (a == bv && key < bk)) { bv = a; bk = key; }
       }
-    (void)m;
+    }
     wave_best(bv, bk);
     if (lane == 0) { redv[wave] = bv; redk[wave] = bk; }
     __syncthreads();
@@ -274,26 +282,31 @@ __device__ double2 block_pfaffian(double2 *S, const int N2, double *redv, int *r
     double phas = 1.0;
     if (I1 == NR) phas = -phas;
     if (I2 == NC) phas = -phas;
-    for (int pass = 0; pass < 2; ++pass) {
-      const int i1 = pass ? NC : NR, i2 = pass ? I2 : I1;
-      if (i1 != i2) {
-        for (int i = tid; i < N2; i += NTH) {
-          if (i >= NR && i < i1) {
-            const double2 t = S[i + i1 * N2]; S[i + i1 * N2] = S[i + i2 * N2]; S[i + i2 * N2] = t;
-          } else if (i > i2) {
-            const double2 t = S[i1 + i * N2]; S[i1 + i * N2] = S[i2 + i * N2]; S[i2 + i * N2] = t;
-          } else if (i > i1 && i < i2) {
-            const double2 t = S[i1 + i * N2];
-            S[i1 + i * N2] = cneg(S[i + i2 * N2]);
-            S[i + i2 * N2] = cneg(t);
-          } else if (i == i1) {
-            S[i1 + i2 * N2] = cneg(S[i1 + i2 * N2]);
-          }
-        }
+    // The two exchanges of ZExch, (NR I1) then (NC I2), applied to the index
+    // map: every thread evaluates them on the four entries involved; the map
+    // of the next step is written to the other buffer.
+    int ix[4] = {NR, I1, NC, I2};
+    int vx[4] = {mp[NR], mp[I1], mp[NC], mp[I2]};
+    for (int sw = 0; sw < 2; ++sw) {
+      const int x = sw ? NC : NR, y = sw ? I2 : I1;
+      if (x == y) continue;
+      int vxx = 0, vyy = 0;
+      for (int k = 0; k < 4; ++k) {
+        if (ix[k] == x) vxx = vx[k];
+        if (ix[k] == y) vyy = vx[k];
       }
-      __syncthreads();
+      for (int k = 0; k < 4; ++k) {
+        if (ix[k] == x) vx[k] = vyy;
+        else if (ix[k] == y) vx[k] = vxx;
+      }
     }
-    const double2 ss = S[NR + NC * N2];
+    auto newmp = [&](int x) -> int {
+      for (int k = 0; k < 4; ++k)
+        if (ix[k] == x) return vx[k];
+      return mp[x];
+    };
+    const int pR = vx[0], pC = vx[2];
+    const double2 ss = S[pR + pC * N2];
     if (sqrt(ss.x * ss.x + ss.y * ss.y) <= 1.0e-13) {
       __syncthreads();
       return cz();
@@ -301,71 +314,147 @@ __device__ double2 block_pfaffian(double2 *S, const int N2, double *redv, int *r
     const double2 inv = crecip(ss);
     const int r0 = NR + 2;
     for (int j = r0 + 1 + tid / 64; j < N2; j += NW) {
-      const double2 snr_j = S[NR + j * N2], snc_j = S[NC + j * N2];
+      const int pj = newmp(j);
+      const double2 snr_j = S[pR + pj * N2], snc_j = S[pC + pj * N2];
       for (int i = r0 + lane; i < j; i += 64) {
-        double2 t = cmul(S[NC + i * N2], snr_j);
-        t = csub(t, cmul(S[NR + i * N2], snc_j));
-        S[i + j * N2] = cadd(S[i + j * N2], cmul(t, inv));
+        const int pi = newmp(i);
+        double2 t = cmul(S[pC + pi * N2], snr_j);
+        t = csub(t, cmul(S[pR + pi * N2], snc_j));
+        const double2 v = cadd(S[pi + pj * N2], cmul(t, inv));
+        S[pi + pj * N2] = v;
+        S[pj + pi * N2] = cneg(v);
       }
     }
+    for (int x = tid; x < N2; x += NTH) mpn[x] = newmp(x);
     pf = cscale(phas, cmul(pf, ss));
     __syncthreads();
   }
-  const double2 last = S[(N2 - 2) + (N2 - 1) * N2];
+  const int *mpl = mpbuf + ((N2 / 2 - 1) & 1) * N2;
+  const double2 last = S[mpl[N2 - 2] + mpl[N2 - 1] * N2];
   __syncthreads();
   return cmul(pf, last);
 }
 
 // Gauss-Jordan with partial pivoting (pivot = max |re|+|im|, first maximum,
-// as ZGETRF): A (n x n) -> 1, B -> A^{-1} B.  Returns the LAPACK-style INFO
-// of the first exactly-zero pivot (0 if none).  fcol needs n entries.
+// as ZGETRF) on [A | B] (n x n each): A -> 1, B -> A^{-1} B.  Row exchanges
+// are kept in the map mp (n ints): logical row k of the result is physical
+// row mp[k] of B.  Two barriers per step.  Returns the LAPACK-style INFO of
+// the first exactly-zero pivot (0 if none).  fcol needs n, prow 2n entries.
 template <int NTH>
-__device__ int block_gauss_jordan(double2 *A, double2 *B, const int n, double2 *fcol, int *redk) {
+__device__ int block_gauss_jordan(double2 *A, double2 *B, const int n, int *mp, double2 *fcol, double2 *prow,
+                                  int *redk) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int info = 0;
+  for (int i = tid; i < n; i += NTH) mp[i] = i;
+  __syncthreads();
   for (int k = 0; k < n; ++k) {
     if (wave == 0) {
       double bv = -1.0;
       int bk = 0x7fffffff;
       for (int i = k + lane; i < n; i += 64) {
-        const double a = cabs1(A[i + k * n]);
+        const double a = cabs1(A[mp[i] + k * n]);
         if (a > bv || (a == bv && i < bk)) { bv = a; bk = i; }
       }
       wave_best(bv, bk);
-      if (lane == 0) redk[0] = bk;
+      const int rk = mp[bk], rold = mp[k];
+      for (int j = lane; j < 2 * n; j += 64) prow[j] = j < n ? A[rk + j * n] : B[rk + (j - n) * n];
+      for (int r = lane; r < n; r += 64) fcol[r] = A[r + k * n];
+      if (lane == 0) {
+        mp[bk] = rold;
+        mp[k] = rk;
+        redk[0] = rk;
+      }
     }
     __syncthreads();
-    const int pr = redk[0];
-    if (pr != k)
-      for (int j = tid; j < 2 * n; j += NTH) {
-        double2 *M = j < n ? A : B;
-        const int jj = j < n ? j : j - n;
-        const double2 t = M[k + jj * n];
-        M[k + jj * n] = M[pr + jj * n];
-        M[pr + jj * n] = t;
-      }
-    __syncthreads();
-    const double2 pv = A[k + k * n];
+    const int rk = redk[0];
+    const double2 pv = prow[k];
     if (pv.x == 0.0 && pv.y == 0.0 && info == 0) info = k + 1;
     const double2 ipv = crecip(pv);
-    for (int i = tid; i < n; i += NTH) fcol[i] = (i == k) ? cz() : A[i + k * n];
-    for (int j = tid; j < 2 * n; j += NTH) {
+    // columns k+1..n-1 of A and all of B; row rk is scaled, the others updated
+    const int ncol = (n - k - 1) + n;
+    for (int c = tid / 64; c < ncol; c += NTH / 64) {
+      const int j = c < n - k - 1 ? k + 1 + c : n + (c - (n - k - 1));
       double2 *M = j < n ? A : B;
       const int jj = j < n ? j : j - n;
-      if (j >= k) M[k + jj * n] = cmul(M[k + jj * n], ipv);
-    }
-    __syncthreads();
-    for (int jj = tid / 64; jj < 2 * n; jj += NTH / 64) {
-      if (jj < n && jj <= k) continue;
-      double2 *M = jj < n ? A : B;
-      const int c = jj < n ? jj : jj - n;
-      const double2 rk = M[k + c * n];
-      for (int i = lane; i < n; i += 64)
-        if (i != k) M[i + c * n] = csub(M[i + c * n], cmul(fcol[i], rk));
+      const double2 pr = cmul(prow[j], ipv);
+      for (int r = lane; r < n; r += 64)
+        M[r + jj * n] = (r == rk) ? pr : csub(M[r + jj * n], cmul(fcol[r], pr));
     }
     __syncthreads();
   }
   return info;
+}
+
+// Pieces of <J^2> for one set of transition densities (ZJ2_0,
+// Codes/PNAMP/PNAMP_ME2B.f90:1-152): for k = x, y, z with the one-body J_k
+// (block-diagonal over orbits, packed like R: J[k][e], J^2 at [3 + k][e]),
+//   A   = sum_k Tr(J_k^2 rho) - Tr(J_k rho J_k rho) + Tr(J_k kappa10 J_k^T kappa01^T)
+//   t_k = Tr(J_k rho)
+// so that <J^2> = A + sum_k t_k^2 for one density set, and for the species
+// factorisation <J^2> = A_p + A_n + sum_k (t_kp + t_kn)^2.  rho, kappa are
+// m x m (ld m) with orbit blocks repeating every h rows; scrA, scrB are
+// m x m scratch.  Returns in out[0..3] = (A, t_x, t_y, t_z) on every thread.
+template <int NTH>
+__device__ __noinline__ void block_j2(const double2 *ro, const double2 *k10, const double2 *k01, const int m, const int h,
+                         const double2 *Jpk, const int npack, const short *m_off, const short *m_sz,
+                         const short *m_base, double2 *scrA, double2 *scrB, double *red, double2 out[4]) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int NW = NTH / 64;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // A (re, im), t_x, t_y, t_z
+  for (int k = 0; k < 3; ++k) {
+    const double2 *Jk = Jpk + (size_t)k * npack, *J2k = Jpk + (size_t)(3 + k) * npack;
+    double2 a = cz(), t2 = cz();
+    // Tr(J^2 rho), Tr(J rho); scrA = J rho; scrB = J kappa10
+    for (int e = tid; e < m * m; e += NTH) {
+      const int i = e % m, c = e / m;
+      const int sp = i / h, il = i - sp * h;
+      const int off = m_off[il], sz = m_sz[il], base = m_base[il];
+      double2 jr = cz(), jk = cz();
+      for (int q = 0; q < sz; ++q) {
+        const int b = sp * h + off + q;
+        const double2 jv = Jk[base + (il - off) + q * sz];
+        cfma(jr, jv, ro[b + c * m]);
+        cfma(jk, jv, k10[b + c * m]);
+        if (c == i) {
+          cfma(a, J2k[base + (il - off) + q * sz], ro[b + i * m]);
+          cfma(t2, jv, ro[b + i * m]);
+        }
+      }
+      scrA[e] = jr;
+      scrB[e] = jk;
+    }
+    __syncthreads();
+    // - Tr(J rho J rho)
+    for (int e = tid; e < m * m; e += NTH) {
+      const int i = e % m, c = e / m;
+      a = csub(a, cmul(scrA[i + c * m], scrA[c + i * m]));
+    }
+    __syncthreads();
+    // + sum (J kappa10)(i,c) (kappa01 J)(i,c): scrA = kappa01 J
+    for (int e = tid; e < m * m; e += NTH) {
+      const int i = e % m, c = e / m;
+      const int sp = c / h, cl = c - sp * h;
+      const int off = m_off[cl], sz = m_sz[cl], base = m_base[cl];
+      double2 kj = cz();
+      for (int q = 0; q < sz; ++q) cfma(kj, k01[i + (sp * h + off + q) * m], Jk[base + q + (cl - off) * sz]);
+      a = cadd(a, cmul(scrB[e], kj));
+    }
+    acc[0] += a.x;
+    acc[1] += a.y;
+    acc[2 + 2 * k] += t2.x;
+    acc[3 + 2 * k] += t2.y;
+    __syncthreads();
+  }
+  for (int v = 0; v < 8; ++v)
+    for (int off = 32; off > 0; off >>= 1) acc[v] += __shfl_xor(acc[v], off);
+  if (lane == 0)
+    for (int v = 0; v < 8; ++v) red[wave * 8 + v] = acc[v];
+  __syncthreads();
+  double tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int w = 0; w < NW; ++w)
+    for (int v = 0; v < 8; ++v) tot[v] += red[w * 8 + v];
+  __syncthreads();
+  for (int v = 0; v < 4; ++v) out[v] = make_double2(tot[2 * v], tot[2 * v + 1]);
 }
 
 // ------------------------------------------------------------------ mesh points
@@ -388,6 +477,7 @@ struct PointParams {
   const int *q0, *q1;               // [npairs]
   const VEntry *vlist;
   const double *t1b;
+  const double2 *jpack;             // [6][npack]: J_x, J_y, J_z, J_x^2, J_y^2, J_z^2 blocks
   double2 *zres;                    // [npairs][NE][NOBS]
   int *status;                      // [npairs][NE]
 };
@@ -397,6 +487,8 @@ __global__ __launch_bounds__(NT) void k_point(PointParams p) {
   __shared__ double2 sm[5 * NMAX * NMAX];
   __shared__ double2 sR[RPACK_MAX];
   __shared__ double2 fcol[NMAX];
+  __shared__ double2 prow[2 * NMAX];
+  __shared__ int pfmap[2 * 2 * NMAX], gjmap[NMAX];
   __shared__ double redv[NWAVE * 8];
   __shared__ int redk[NWAVE];
   __shared__ short m_off[HMAX], m_sz[HMAX], m_base[HMAX], m_twom[HMAX];
@@ -456,7 +548,10 @@ __global__ __launch_bounds__(NT) void k_point(PointParams p) {
     // conj(Vbar) conj(Ubar)^{-1} = A (V1 U1^{-1}) A^T since A is unitary)
     for (int e = tid; e < N2 * N2; e += NT) {
       const int i = e % N2, j = e / N2;
-      if (i >= j) continue;
+      if (i >= j) {
+        if (i == j) S[e] = cz();
+        continue;
+      }
       double2 v;
       if (j < n) {
         const int sp = j >= h, jl = j - sp * h;
@@ -470,9 +565,10 @@ __global__ __launch_bounds__(NT) void k_point(PointParams p) {
         v = make_double2(-Z0[(i - n) + (j - n) * n], 0.0);
       }
       S[i + j * N2] = cscale(p.cscale, v);
+      S[j + i * N2] = cscale(-p.cscale, v);
     }
     __syncthreads();
-    const double2 pf = block_pfaffian<NT>(S, N2, redv, redk);
+    const double2 pf = block_pfaffian<NT>(S, N2, pfmap, redv, redk);
     const double2 gphase = cexpi(-phip * p.zval - phin * p.nval);
     const double2 znorm = cmul(cscale(vac, pf), gphase);
     __syncthreads();
@@ -515,7 +611,7 @@ __global__ __launch_bounds__(NT) void k_point(PointParams p) {
     __syncthreads();
     // Gauss-Jordan: s2 -> 1, s3 -> T^{-1}
     {
-      const int info = block_gauss_jordan<NT>(s2, s3, n, fcol, redk);
+      const int info = block_gauss_jordan<NT>(s2, s3, n, gjmap, fcol, prow, redk);
       if (info && !status) status = info;
     }
     // s2 = W1 = (A V1) T^{-1}, s4 = W2 = (A* U1) T^{-1}
@@ -526,7 +622,7 @@ __global__ __launch_bounds__(NT) void k_point(PointParams p) {
       for (int k = 0; k < n; ++k) {
         const double2 x0 = s0[i + k * n], x1 = s0[i + 1 + k * n];
         const double2 y0 = s1[i + k * n], y1 = s1[i + 1 + k * n];
-        const double2 t0 = s3[k + j * n], t1 = s3[k + (j + 1) * n];
+        const double2 t0 = s3[gjmap[k] + j * n], t1 = s3[gjmap[k] + (j + 1) * n];
         cfma(a00, x0, t0); cfma(a10, x1, t0); cfma(a01, x0, t1); cfma(a11, x1, t1);
         cfma(b00, y0, t0); cfma(b10, y1, t0); cfma(b01, y0, t1); cfma(b11, y1, t1);
       }
@@ -557,6 +653,9 @@ __global__ __launch_bounds__(NT) void k_point(PointParams p) {
     __syncthreads();
 
     // ---------------------------------------------------------------- observables
+    double2 j2p[4];
+    block_j2<NT>(s0, s1, s3, n, h, p.jpack, p.npack, m_off, m_sz, m_base, s2, s4, redv, j2p);
+    const double2 J2 = cadd(j2p[0], cadd(cmul(j2p[1], j2p[1]), cadd(cmul(j2p[2], j2p[2]), cmul(j2p[3], j2p[3]))));
     const double2 *ro = s0, *k10 = s1, *k01 = s3;
     double2 e2 = cz(), ek = cz(), zp = cz(), zn = cz();
     for (int q = tid; q < p.nme; q += NT) {
@@ -610,6 +709,7 @@ __global__ __launch_bounds__(NT) void k_point(PointParams p) {
       acc[2] = cadd(acc[2], cmul(znorm, ZN));
       acc[3] = cadd(acc[3], cmul(znorm, EK));
       acc[4] = cadd(acc[4], cmul(znorm, cadd(cadd(E2, EK), make_double2(p.e0, 0.0))));
+      acc[5] = cadd(acc[5], cmul(znorm, J2));
     }
     __syncthreads();
   }
@@ -655,7 +755,8 @@ struct FacParams {
   const VEntry *vl0, *vl1;          // species lists with local indices (p, n)
   int nme0, nme1;
   const double *t1b;                // n x n
-  double2 *out;                     // [npairs][NE][2][nphi][4]: Pf, E2, Ekin, Tr rho
+  const double2 *jpack;             // [6][npack]
+  double2 *out;                     // [npairs][NE][2][nphi][8]: Pf, E2, Ekin, Tr rho, A, t_x, t_y, t_z
   double2 *rho;                     // [npairs][NE][2][nphi][h*h]
   int *status;                      // [npairs][NE][2]
 };
@@ -664,6 +765,8 @@ __global__ __launch_bounds__(NTF) void k_fac_species(FacParams p) {
   extern __shared__ double2 sm[];  // 10 h^2 complex
   __shared__ double2 sR[RPACK_MAX];
   __shared__ double2 fcol[HMAX];
+  __shared__ double2 prow[2 * HMAX];
+  __shared__ int pfmap[2 * 2 * HMAX], gjmap[HMAX];
   __shared__ double redv[NTF / 64 * 8];
   __shared__ int redk[NTF / 64];
   __shared__ short m_off[HMAX], m_sz[HMAX], m_base[HMAX];
@@ -733,15 +836,17 @@ __global__ __launch_bounds__(NTF) void k_fac_species(FacParams p) {
     const double2 e1 = cexpi(phi), e2 = cexpi(2.0 * phi);
     // S = cscale [[e^{2 i phi} M0, -1], [1, -Z0s]] (upper triangle, ld 2h)
     for (int j = tid / 64; j < N2; j += NTF / 64)
-      for (int i = lane; i < j; i += 64) {
+      for (int i = lane; i <= j; i += 64) {
         double2 v;
-        if (j < h) v = cmul(e2, M0[i + j * h]);
+        if (i == j) v = cz();
+        else if (j < h) v = cmul(e2, M0[i + j * h]);
         else if (i < h) v = make_double2((j - h == i) ? -1.0 : 0.0, 0.0);
         else v = make_double2(-Z0[(i - h) + (size_t)(j - h) * n], 0.0);
         SC[i + j * N2] = cscale(p.cscale, v);
+        SC[j + i * N2] = cscale(-p.cscale, v);
       }
     __syncthreads();
-    const double2 pf = block_pfaffian<NTF>(SC, N2, redv, redk);
+    const double2 pf = block_pfaffian<NTF>(SC, N2, pfmap, redv, redk);
     // T = e^{i phi} X + e^{-i phi} Y -> SC[0], 1 -> SC[1]
     double2 *T = SC, *Ti = SC + hh, *W1 = SC + 2 * hh, *W2 = SC + 3 * hh;
     for (int e = tid; e < hh; e += NTF) {
@@ -750,7 +855,7 @@ __global__ __launch_bounds__(NTF) void k_fac_species(FacParams p) {
     }
     __syncthreads();
     {
-      const int info = block_gauss_jordan<NTF>(T, Ti, h, fcol, redk);
+      const int info = block_gauss_jordan<NTF>(T, Ti, h, gjmap, fcol, prow, redk);
       if (info && !status) status = info;
     }
     // W1 = e^{i phi} RV1 T^{-1}, W2 = e^{-i phi} RcU1 T^{-1}
@@ -758,7 +863,7 @@ __global__ __launch_bounds__(NTF) void k_fac_species(FacParams p) {
       const int i = e % h, j = e / h;
       double2 a = cz(), b = cz();
       for (int k = 0; k < h; ++k) {
-        const double2 t = Ti[k + j * h];
+        const double2 t = Ti[gjmap[k] + j * h];
         cfma(a, RV1[i + k * h], t);
         cfma(b, RcU1[i + k * h], t);
       }
@@ -783,6 +888,8 @@ __global__ __launch_bounds__(NTF) void k_fac_species(FacParams p) {
       k01[e] = cneg(b);
     }
     __syncthreads();
+    double2 j2s[4];
+    block_j2<NTF>(ro, k10, k01, h, h, p.jpack, p.npack, m_off, m_sz, m_base, W1, W2, redv, j2s);
     double2 e2b = cz(), ek = cz(), tr = cz();
     for (int q = tid; q < nme; q += NTF) {
       const VEntry ve = vl[q];
@@ -822,11 +929,12 @@ __global__ __launch_bounds__(NTF) void k_fac_species(FacParams p) {
       double t[6] = {0, 0, 0, 0, 0, 0};
       for (int w = 0; w < NTF / 64; ++w)
         for (int v = 0; v < 6; ++v) t[v] += redv[w * 8 + v];
-      double2 *o = p.out + obase * 4;
+      double2 *o = p.out + obase * 8;
       o[0] = pf;
       o[1] = make_double2(t[0], t[1]);
       o[2] = make_double2(t[2], t[3]);
       o[3] = make_double2(t[4], t[5]);
+      for (int v = 0; v < 4; ++v) o[4 + v] = j2s[v];
     }
     __syncthreads();
   }
@@ -856,7 +964,7 @@ struct CombParams {
 __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
   extern __shared__ double2 dsm[];  // rho_p [nphi][h^2], rho_n -> G [nphi][h^2]
   __shared__ double2 epn[NPHI_MAX * NPHI_MAX];
-  __shared__ double red[NTC / 64 * 10];
+  __shared__ double red[NTC / 64 * 2 * NOBS];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int iang = blockIdx.x, ip = blockIdx.y;
   const int h = p.h, hh = h * h, nphi = p.nphi;
@@ -915,36 +1023,42 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
     }
   }
   __syncthreads();
-  const double2 *op = p.out + base * nphi * 4, *on = p.out + (base + 1) * nphi * 4;
+  const double2 *op = p.out + base * nphi * 8, *on = p.out + (base + 1) * nphi * 8;
   const double vac = p.snpfaf * p.sgn_h * (p.det[p.q0[ip]] * p.vac_scale) * (p.det[p.q1[ip]] * p.vac_scale);
-  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double v[2 * NOBS] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int t = tid; t < nphi * nphi; t += NTC) {
     const int fp = t / nphi, fn = t % nphi;
     const double phip = (nphi == 1) ? 0.0 : (2.0 * M_PI / nphi) * fp;
     const double phin = (nphi == 1) ? 0.0 : (2.0 * M_PI / nphi) * fn;
-    const double2 *a = op + fp * 4, *b = on + fn * 4;
+    const double2 *a = op + fp * 8, *b = on + fn * 8;
     const double2 zn = cmul(cscale(vac, cmul(a[0], b[0])), cexpi(-phip * p.zval - phin * p.nval));
     const double2 ekin = cadd(a[2], b[2]);
     const double2 E = cadd(cadd(cadd(a[1], b[1]), epn[fp * nphi + fn]), cadd(ekin, make_double2(p.e0, 0.0)));
-    const double2 o[5] = {zn, cmul(zn, a[3]), cmul(zn, b[3]), cmul(zn, ekin), cmul(zn, E)};
-    for (int k = 0; k < 5; ++k) {
+    // <J^2> = A_p + A_n + sum_k (t_kp + t_kn)^2
+    double2 J2 = cadd(a[4], b[4]);
+    for (int k = 0; k < 3; ++k) {
+      const double2 tk = cadd(a[5 + k], b[5 + k]);
+      J2 = cadd(J2, cmul(tk, tk));
+    }
+    const double2 o[NOBS] = {zn, cmul(zn, a[3]), cmul(zn, b[3]), cmul(zn, ekin), cmul(zn, E), cmul(zn, J2)};
+    for (int k = 0; k < NOBS; ++k) {
       v[2 * k] += o[k].x;
       v[2 * k + 1] += o[k].y;
     }
   }
-  for (int k = 0; k < 10; ++k)
+  for (int k = 0; k < 2 * NOBS; ++k)
     for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
   if (lane == 0)
-    for (int k = 0; k < 10; ++k) red[wave * 10 + k] = v[k];
+    for (int k = 0; k < 2 * NOBS; ++k) red[wave * 2 * NOBS + k] = v[k];
   __syncthreads();
   if (tid == 0) {
     const double inv = 1.0 / (double)(nphi * nphi);
     double2 *z = p.zres + ((size_t)ip * p.NE + iang) * NOBS;
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < NOBS; ++k) {
       double re = 0, im = 0;
       for (int w = 0; w < NTC / 64; ++w) {
-        re += red[w * 10 + 2 * k];
-        im += red[w * 10 + 2 * k + 1];
+        re += red[w * 2 * NOBS + 2 * k];
+        im += red[w * 2 * NOBS + 2 * k + 1];
       }
       z[k] = make_double2(inv * re, inv * im);
     }
@@ -1197,6 +1311,7 @@ struct pnamp_ctx {
   int *d_fstatus = nullptr;
   int *d_wptr = nullptr, *d_wcol = nullptr;
   double *d_wval = nullptr;
+  double2 *d_jpack = nullptr;
   size_t fac_cap = 0;  // pairs the factored scratch holds
   // captured launch sequence of the factored form, valid for (npairs, zval,
   // nval) and the current buffers; dropped whenever a buffer is reallocated
@@ -1289,6 +1404,38 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
     i += s;
   }
   if ((int)pa.size() > RPACK_MAX) return bail(PNAMP_E_SIZE, "rotation blocks exceed the packed LDS table");
+  // one-body J_x, J_y, J_z and their squares in the packed block layout
+  std::vector<double2> jpk(6 * pa.size());
+  {
+    const size_t np = pa.size();
+    std::vector<double> jpl(np, 0.0);  // <a|J_+|b>
+    for (size_t e = 0; e < np; ++e) {
+      const int a = pa[e], b = pb[e];
+      const double j = basis->twoj[a] / 2.0, ma = basis->twom[a] / 2.0, mb = basis->twom[b] / 2.0;
+      if (std::fabs(ma - mb - 1.0) < 1e-9) jpl[e] = std::sqrt(j * (j + 1) - mb * (mb + 1));
+    }
+    auto at = [&](int a, int b) -> size_t {  // packed index of (a, b) in a's block
+      return (size_t)base[a] + (a - off[a]) + (size_t)(b - off[a]) * sz[a];
+    };
+    for (size_t e = 0; e < np; ++e) {
+      const int a = pa[e], b = pb[e];
+      const double p = jpl[e], m = jpl[at(b, a)];
+      jpk[e] = make_double2(0.5 * (p + m), 0.0);                    // J_x
+      jpk[np + e] = make_double2(0.0, -0.5 * (p - m));              // J_y = (J+ - J-)/(2i)
+      jpk[2 * np + e] = make_double2(a == b ? basis->twom[a] / 2.0 : 0.0, 0.0);
+    }
+    for (int k = 0; k < 3; ++k)
+      for (size_t e = 0; e < np; ++e) {
+        const int a = pa[e], b = pb[e];
+        double2 acc = make_double2(0.0, 0.0);
+        for (int q = off[a]; q < off[a] + sz[a]; ++q) {
+          const double2 x = jpk[k * np + at(a, q)], y = jpk[k * np + at(q, b)];
+          acc.x += x.x * y.x - x.y * y.y;
+          acc.y += x.x * y.y + x.y * y.x;
+        }
+        jpk[(3 + k) * np + e] = acc;
+      }
+  }
   if (mesh->nalp < 1 || mesh->nbet < 1 || mesh->ngam < 1 || mesh->nphi < 1 || mesh->jmax < 0)
     return bail(PNAMP_E_ARG, "bad mesh sizes");
   if (mesh->jmax > JMAX_MAX) return bail(PNAMP_E_SIZE, "jmax too large");
@@ -1327,6 +1474,7 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
       (r = dupload(c, &c->d_bet, bet.data(), c->nbet)) || (r = dupload(c, &c->d_wbet, wbet.data(), c->nbet)) ||
       (r = dupload(c, &c->d_gam, mesh->gam, c->ngam)) || (r = dupload(c, &c->d_wgam, mesh->wgam, c->ngam)))
     return bail(r, c->err);
+  if ((r = dupload(c, &c->d_jpack, jpk.data(), jpk.size()))) return bail(r, c->err);
   if ((r = dupload(c, &c->d_pk_a, pa.data(), pa.size())) || (r = dupload(c, &c->d_pk_b, pb.data(), pb.size())) ||
       (r = dupload(c, &c->d_blk_off, off.data(), h)) || (r = dupload(c, &c->d_blk_sz, sz.data(), h)) ||
       (r = dupload(c, &c->d_blk_base, base.data(), h)) ||
@@ -1513,7 +1661,7 @@ extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0
   pp.U = c->d_U; pp.V = c->d_V; pp.Z = c->d_Z; pp.det = c->d_det;
   pp.rtab = c->d_rtab; pp.pk_a = c->d_pk_a; pp.pk_b = c->d_pk_b;
   pp.blk_off = c->d_blk_off; pp.blk_sz = c->d_blk_sz; pp.blk_base = c->d_blk_base; pp.twom = c->d_twom;
-  pp.q0 = c->d_q0; pp.q1 = c->d_q1; pp.vlist = c->d_vlist; pp.t1b = c->d_t1b;
+  pp.q0 = c->d_q0; pp.q1 = c->d_q1; pp.vlist = c->d_vlist; pp.t1b = c->d_t1b; pp.jpack = c->d_jpack;
   pp.zres = c->d_zres; pp.status = c->d_status;
   if (c->form == PNAMP_FORM_FACTORED) {
     if (!c->separable)
@@ -1526,7 +1674,7 @@ extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0
     const size_t per_pair = (size_t)c->NE * 2 * c->nphi * hh;
     const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)npairs, ((size_t)4 << 30) / (per_pair * 16)));
     if ((size_t)chunk > c->fac_cap) {
-      if ((r = dalloc(c, &c->d_frho, per_pair * chunk)) || (r = dalloc(c, &c->d_fout, (size_t)c->NE * 2 * c->nphi * 4 * chunk)) ||
+      if ((r = dalloc(c, &c->d_frho, per_pair * chunk)) || (r = dalloc(c, &c->d_fout, (size_t)c->NE * 2 * c->nphi * 8 * chunk)) ||
           (r = dalloc(c, &c->d_fstatus, (size_t)c->NE * 2 * chunk)))
         return r;
       c->fac_cap = chunk;
@@ -1541,7 +1689,7 @@ extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0
     fp.rtab = c->d_rtab; fp.pk_a = c->d_pk_a; fp.pk_b = c->d_pk_b; fp.blk_off = c->d_blk_off;
     fp.blk_sz = c->d_blk_sz; fp.blk_base = c->d_blk_base; fp.twom = c->d_twom;
     fp.vl0 = c->d_vs[0]; fp.vl1 = c->d_vs[1]; fp.nme0 = c->nme_s[0]; fp.nme1 = c->nme_s[1];
-    fp.t1b = c->d_t1b; fp.out = c->d_fout; fp.rho = c->d_frho; fp.status = c->d_fstatus;
+    fp.t1b = c->d_t1b; fp.jpack = c->d_jpack; fp.out = c->d_fout; fp.rho = c->d_frho; fp.status = c->d_fstatus;
     CombParams cp;
     cp.h = h; cp.nphi = c->nphi; cp.NE = c->NE; cp.zval = zval; cp.nval = nval;
     cp.vac_scale = c->vac_scale; cp.snpfaf = c->snpfaf; cp.sgn_h = (h & 1) ? -1.0 : 1.0; cp.e0 = c->e0;
@@ -1710,7 +1858,7 @@ extern "C" void pnamp_destroy(pnamp_ctx *c) {
                   c->d_vlist, c->d_t1b, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status, c->d_q0,
                   c->d_q1, c->d_zres, c->d_kern, c->d_status, c->d_pair_status, c->d_vs[0], c->d_vs[1],
                   c->d_vs[2], c->d_Uf, c->d_Vf, c->d_fout, c->d_frho, c->d_fstatus, c->d_wptr,
-                  c->d_wcol, c->d_wval};
+                  c->d_wcol, c->d_wval, c->d_jpack};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto e : c->ev)

@@ -16,8 +16,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpnamp_gpu.so")
-NOBS = 5
-OBS = ("norm", "Z", "N", "Ekin", "E")
+NOBS = 6
+OBS = ("norm", "Z", "N", "Ekin", "E", "J2")
 FORM_PLAIN, FORM_FACTORED = 0, 1
 
 _i32p = C.POINTER(C.c_int32)

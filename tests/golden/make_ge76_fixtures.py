@@ -8,7 +8,7 @@ Inputs (read from the reference checkout, path given as argv[1], default
   data/Ge/kern.1D.05.01.16.01.GCN2850_pf5g9_hwHO09_s000_<q>_<q'>.elem  (expected kernels)
 Outputs (this directory):
   ge76_inputs.npz  U[5,44,44], V[5,44,44] (as read, U[i,k]), beta2[5], me[10330], spe file lines
-  ge76_kernels.json  {"<beta_q>,<beta_q'>": {"J": [N_re,N_im,H_re,H_im,nn_re,nn_im,zz_re,zz_im]}}
+  ge76_kernels.json  {"<beta_q>,<beta_q'>": {"J": [N_re,N_im,H_re,H_im,nn_re,nn_im,zz_re,zz_im,J2_re,J2_im]}}
 The numbers are copied, not recomputed: they are the reference run's inputs
 and outputs (8 significant digits for the kernels).
 """
@@ -59,6 +59,7 @@ def main():
                 vals = []
                 for ln in (L[3], L[4]):
                     vals += [float(ln[i:i + 15]) for i in range(0, 60, 15)]
+                vals += [float(L[5][i:i + 15]) for i in range(0, 30, 15)]
                 rec[str(J)] = vals
             kern[f"{b0:+.1f},{b1:+.1f}"] = rec
     json.dump(kern, open(f"{OUT}/ge76_kernels.json", "w"), indent=1)

@@ -61,7 +61,7 @@ def test_pair_matches_oracle(oracle_mod, space, zn, mesh_args, fold):
     for (p, q) in ((0, 1), (1, 1)):
         zr_g, k_g = g.kernel_pair(p, q, *zn, want_zres=True)
         zr_o, k_o = o.kernel_pair(Us[p], Vs[p], Us[q], Vs[q])
-        for io in range(5):
+        for io in range(zr_o.shape[1]):
             assert _rel(zr_g[:, io], zr_o[:, io]) < REL, (space, p, q, io)
         assert _blockwise(k_g, k_o) < REL
     g.close()
@@ -98,6 +98,8 @@ def test_ge76_goldens_on_gpu(oracle_mod):
             N, H = k[0, J, jm, jm].real, k[4, J, jm, jm].real
             assert abs(N - gold[str(J)][0]) <= 2e-7 * max(abs(gold[str(J)][0]), 1e-6 * abs(n0))
             assert abs(H - gold[str(J)][2]) <= 2e-7 * max(abs(gold[str(J)][2]), 1e-6 * abs(h0))
+            j2 = (k[5, J, jm, jm] / k[0, J, jm, jm]).real
+            assert abs(j2 - gold[str(J)][8]) <= 2e-7 * max(abs(gold[str(J)][8]), 10.0 * abs(n0 / N))
     o = oracle_mod.Oracle(b, inter, mesh, ge76.ZVAL, ge76.NVAL)
     _, ko = o.kernel_pair(U[1], V[1], U[3], V[3])
     assert _blockwise(kb[pairs.index((1, 3))], ko) < REL
@@ -125,6 +127,8 @@ def test_exact_projection_on_gpu():
     for (J, K1, K2), Pm in P.items():
         assert abs(np.vdot(v0[sec], Pm @ v1[sec]) - kern[0, J, K1 + jmax, K2 + jmax]) < 1e-12
         assert abs(np.vdot(v0[sec], H @ (Pm @ v1[sec])) - kern[4, J, K1 + jmax, K2 + jmax]) < 1e-11
+    for J in range(jmax + 1):  # <J^2> = J(J+1) on the projected kernels
+        assert np.abs(kern[5, J] - J * (J + 1) * kern[0, J]).max() < 1e-11
     g.close()
 
 
@@ -182,7 +186,7 @@ def test_factored_form_matches_plain_and_oracle(oracle_mod, space, zn, mesh_args
     g.set_form(FORM_FACTORED)
     zf, kf = g.kernels_batched(pairs, *zn, want_zres=True)
     for i in range(len(pairs)):
-        for io in range(5):
+        for io in range(zp.shape[2]):
             assert _rel(zf[i][:, io], zp[i][:, io]) < REL, (space, pairs[i], io)
         assert _blockwise(kf[i], kp[i]) < REL
     o = oracle_mod.Oracle(b, inter, mesh, *zn)

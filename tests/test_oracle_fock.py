@@ -80,3 +80,17 @@ def test_plain_projection_exact(oracle_mod, setup):
             for K2 in range(-J, J + 1):
                 if (J, K1, K2) not in seen:
                     assert abs(kern[0, J, K1 + jmax, K2 + jmax]) < 1e-13
+
+
+def test_j2_kernel_is_j_j_plus_1(oracle_mod, setup):
+    """<J^2> projected onto J is J(J+1) times the norm kernel (the reference's
+    own check, Codes/GCM/GCM_IO.f90:233-237), exactly on an exact mesh."""
+    b, inter, uv, F, v0, v1 = setup
+    na, jmax = 12, 4
+    alp = np.arange(na) * 2 * np.pi / na
+    w = np.full(na, 2 * np.pi / na)
+    x, wb = np.polynomial.legendre.leggauss(12)
+    m = Mesh(alp, w, np.arccos(x), wb, alp.copy(), w.copy(), 7, jmax, 0, 0)
+    _, kern = oracle_mod.Oracle(b, inter, m, 2, 2).kernel_pair(*uv)
+    for J in range(jmax + 1):
+        assert np.abs(kern[5, J] - J * (J + 1) * kern[0, J]).max() < 1e-12

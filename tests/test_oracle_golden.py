@@ -45,8 +45,10 @@ def test_ge76_kernels(oracle_mod, ge):
                 H = k[4, J, jm, jm]
                 nn = (k[2, J, jm, jm] / N).real + ge76.NCORE
                 zz = (k[1, J, jm, jm] / N).real + ge76.NCORE
+                j2 = (k[5, J, jm, jm] / N).real
                 checks = [("N", N.real, g[0], abs(n0)), ("H", H.real, g[2], abs(gold["0"][2])),
-                          ("<N>", nn, g[4], 44.0), ("<Z>", zz, g[6], 32.0)]
+                          ("<N>", nn, g[4], 44.0), ("<Z>", zz, g[6], 32.0),
+                          ("<J^2>", j2, g[8], 10.0 * abs(n0 / N.real))]
                 for name, got, want, scale in checks:
                     if not _close(got, want, scale):
                         bad.append((b0, b1, J, name, got, want))
