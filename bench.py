@@ -41,15 +41,21 @@ def plain_flops_per_point(n: int, nme: int) -> float:
     return 147.0 * n ** 3 + 190.0 * nme
 
 
-def factored_flops(h: int, sum_b2: int, nphi: int, l_species: int) -> float:
-    """Algorithmic real flops of k_fac_species per (pair, Euler point, species):
-    R V1, R* U1, R Z1 (block-sparse, 12 sum_b2 h), X, Y (8 h^3), M0 (8 sum_b2 h);
-    per gauge angle: Pfaffian of 2h x 2h (32 h^3 / 3), Gauss-Jordan on
-    [T | 1] (12 h^3), W1, W2 (16 h^3), rho, kappa10, kappa01 (12 h^3),
-    the species list (190 per entry)."""
+def factored_flops(h: int, sum_b2: int, nphi: int, l_species: int, l_cross: int, nnz_pn: int) -> float:
+    """Algorithmic real flops of the factored form per (pair, Euler point),
+    complex multiply-add = 8 flops.  Per species: R V1, R* U1, R Z1 (block-sparse,
+    12 sum_b2 h), X, Y (8 h^3), M0 (8 sum_b2 h); then per gauge angle the
+    Pfaffian of the 2h x 2h skew matrix (32 h^3 / 3), Gauss-Jordan on
+    [T^T | W-right-hand sides] (20 h^3), rho, kappa10, kappa01 (12 h^3), the
+    <J^2> ladder sums and traces (40 h^2) and the regrouped species list
+    (28 flops per entry, 56 when its (cd, ab) partner term is present:
+    ``l_species`` entries, ``l_cross`` of them with the partner).  Per Euler
+    point once: G = W_pn rho_n (4 flops per CSR entry and phi_n), the nphi^2
+    dot products (8 h^2 each) and the gauge sums (100 per gauge pair)."""
     setup = 12.0 * sum_b2 * h + 8.0 * h ** 3 + 8.0 * sum_b2 * h
-    per_phi = (32.0 / 3.0 + 12.0 + 16.0 + 12.0) * h ** 3 + 190.0 * l_species
-    return setup + nphi * per_phi
+    per_phi = (32.0 / 3.0 + 20.0 + 12.0) * h ** 3 + 40.0 * h * h + 28.0 * (l_species + l_cross)
+    combine = 4.0 * nnz_pn * nphi + 8.0 * nphi * nphi * h * h + 100.0 * nphi * nphi
+    return 2.0 * (setup + nphi * per_phi) + combine
 
 
 def traffic_per_launch(form: str, pairs: int):
@@ -158,10 +164,20 @@ def main():
     h = w.n // 2
     if form == FORM_FACTORED:
         sum_b2 = sum(s * s for (_, s, _) in w.basis.blocks()) // 2
-        idx = np.stack([w.inter.ka, w.inter.kb, w.inter.kc, w.inter.kd], 1)
-        l_sp = ((idx <= h).all(1).sum() + (idx > h).all(1).sum()) / 2.0
-        fl = factored_flops(h, sum_b2, w.mesh.nphi, int(l_sp)) * 2 * w.mesh.n_euler * P
-        kname, tform = "k_fac_species + k_fac_combine", "factored_fp64"
+        idx = np.stack([w.inter.ka, w.inter.kb, w.inter.kc, w.inter.kd], 1) - 1
+        same = ((idx < h).all(1)) | ((idx >= h).all(1))
+        cross = idx[:, 0] + idx[:, 1] != idx[:, 2] + idx[:, 3]
+        l_sp = int(same.sum()) // 2
+        l_cr = int((same & cross).sum()) // 2
+        pn = (idx[:, 0] < h) & (idx[:, 1] >= h) & (idx[:, 2] < h) & (idx[:, 3] >= h)
+        nz = set()
+        for a, b, c, d in idx[pn]:
+            bl, dl = b - h, d - h
+            nz.add((c + a * h, dl + bl * h))
+            if a + b != c + d:
+                nz.add((a + c * h, bl + dl * h))
+        fl = factored_flops(h, sum_b2, w.mesh.nphi, l_sp, l_cr, len(nz)) * w.mesh.n_euler * P
+        kname, tform = "k_fac_setup + k_fac_phi + k_fac_combine", "factored_fp64"
     else:
         fl = plain_flops_per_point(w.n, w.inter.nme) * pts_step
         kname, tform = "k_point", "plain_fp64"

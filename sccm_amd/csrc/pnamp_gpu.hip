@@ -739,66 +739,58 @@ __global__ __launch_bounds__(NT) void k_point(PointParams p) {
 // instead of nphi^2 full ones.  The energy splits into E_pp(phi_p) + E_nn(phi_n)
 // (species lists) + E_pn(phi_p, phi_n), a bilinear form in rho_p and rho_n
 // evaluated for all nphi^2 gauge pairs by k_fac_combine.
-constexpr int NTF = 256;     // threads of the species workgroup
+//
+// Three launches per chunk of pairs:
+//   k_fac_setup    one workgroup per (pair, Euler point, species): the
+//                  gauge-independent X, Y, R V1s, R* U1s, M0 = R Z1s R^T -> HBM;
+//   k_fac_phi<h>   one wavefront per (pair, Euler point, species, phi): the
+//                  Pfaffian and Gauss-Jordan matrices live in registers (no
+//                  workgroup barriers), rho / kappa in LDS for the list;
+//   k_fac_combine  one workgroup per (pair, Euler point): the p-n bilinear form
+//                  and the nphi^2 gauge sum.
+constexpr int NTS = 256;     // threads of the setup workgroup
 constexpr int NTC = 288;     // threads of the combine workgroup (9 groups of 32)
 constexpr int NPHI_MAX = 16;
+constexpr int FAC_OUT = 8;   // per (pair, Euler, species, phi): Pf, E2, Ekin, Tr rho, A, t_+, t_-, t_z
 
-struct FacParams {
-  int n, h, nalp, nbet, ngam, nphi, npack, NE;
-  double cscale;
-  const double *alp, *bet, *gam;
+struct SetupParams {
+  int n, h, nalp, ngam, npack, NE;
+  const double *alp, *gam;
   const double *Uf, *Vf;            // [nstates][2][h*h], species blocks, column-major
   const double *Z;                  // [nstates][n*n]
-  const double *rtab;
+  const double *rtab;               // [nbet][npack]
   const int *pk_a, *pk_b, *blk_off, *blk_sz, *blk_base, *twom;
   const int *q0, *q1;
-  const VEntry *vl0, *vl1;          // species lists with local indices (p, n)
-  int nme0, nme1;
-  const double *t1b;                // n x n
-  const double2 *jpack;             // [6][npack]
-  double2 *out;                     // [npairs][NE][2][nphi][8]: Pf, E2, Ekin, Tr rho, A, t_x, t_y, t_z
-  double2 *rho;                     // [npairs][NE][2][nphi][h*h]
-  int *status;                      // [npairs][NE][2]
+  double2 *sbuf;                    // [npairs][NE][2][5][h*h]: X, Y, R V1s, R* U1s, M0 (column-major)
 };
 
-__global__ __launch_bounds__(NTF) void k_fac_species(FacParams p) {
-  extern __shared__ double2 sm[];  // 10 h^2 complex
+__global__ __launch_bounds__(NTS) void k_fac_setup(SetupParams p) {
+  extern __shared__ double2 sm[];  // 3 h^2: R V1s, R* U1s, R Z1s
   __shared__ double2 sR[RPACK_MAX];
-  __shared__ double2 fcol[HMAX];
-  __shared__ double2 prow[2 * HMAX];
-  __shared__ int pfmap[2 * 2 * HMAX], gjmap[HMAX];
-  __shared__ double redv[NTF / 64 * 8];
-  __shared__ int redk[NTF / 64];
   __shared__ short m_off[HMAX], m_sz[HMAX], m_base[HMAX];
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x;
   const int iang = blockIdx.x, ip = blockIdx.y, sp = blockIdx.z;
-  const int n = p.n, h = p.h, hh = h * h, nn = n * n, N2 = 2 * h;
+  const int n = p.n, h = p.h, hh = h * h;
   const int ig = iang % p.ngam, ia = (iang / p.ngam) % p.nalp, ib = iang / (p.ngam * p.nalp);
   const double alp = p.alp[ia], gam = p.gam[ig];
   const int q0 = p.q0[ip], q1 = p.q1[ip];
   const double *U0 = p.Uf + ((size_t)q0 * 2 + sp) * hh, *V0 = p.Vf + ((size_t)q0 * 2 + sp) * hh;
   const double *U1 = p.Uf + ((size_t)q1 * 2 + sp) * hh, *V1 = p.Vf + ((size_t)q1 * 2 + sp) * hh;
-  const double *Z0 = p.Z + (size_t)q0 * nn + (size_t)sp * h * (n + 1);  // species block, ld n
-  const double *Z1 = p.Z + (size_t)q1 * nn + (size_t)sp * h * (n + 1);
-  const VEntry *vl = sp ? p.vl1 : p.vl0;
-  const int nme = sp ? p.nme1 : p.nme0;
-
-  for (int i = tid; i < h; i += NTF) {
+  const double *Z1 = p.Z + (size_t)q1 * n * n + (size_t)sp * h * (n + 1);  // species block, ld n
+  for (int i = tid; i < h; i += NTS) {
     m_off[i] = (short)p.blk_off[i];
     m_sz[i] = (short)p.blk_sz[i];
     m_base[i] = (short)p.blk_base[i];
   }
-  for (int e = tid; e < p.npack; e += NTF) {
+  for (int e = tid; e < p.npack; e += NTS) {
     const int a = p.pk_a[e], b = p.pk_b[e];
     const double d = p.rtab[(size_t)ib * p.npack + e];
     sR[e] = cscale(d, cexpi(-alp * 0.5 * p.twom[a] - gam * 0.5 * p.twom[b]));
   }
   __syncthreads();
-  double2 *RV1 = sm, *RcU1 = sm + hh, *X = sm + 2 * hh, *Y = sm + 3 * hh, *M0 = sm + 4 * hh;
-  double2 *SC = sm + 5 * hh;  // 5 h^2 scratch; the Pfaffian uses its first 4 h^2
-  // RV1 = R V1s, RcU1 = R* U1s, SC = R Z1s
-  for (int e = tid; e < hh; e += NTF) {
+  double2 *RV1 = sm, *RcU1 = sm + hh, *RZ1 = sm + 2 * hh;
+  double2 *out = p.sbuf + (((size_t)ip * p.NE + iang) * 2 + sp) * 5 * hh;
+  for (int e = tid; e < hh; e += NTS) {
     const int i = e % h, j = e / h;
     const int off = m_off[i], sz = m_sz[i], base = m_base[i];
     double2 a = cz(), b = cz(), c = cz();
@@ -810,146 +802,400 @@ __global__ __launch_bounds__(NTF) void k_fac_species(FacParams p) {
     }
     RV1[e] = a;
     RcU1[e] = b;
-    SC[e] = c;
+    RZ1[e] = c;
+    out[2 * hh + e] = a;
+    out[3 * hh + e] = b;
   }
   __syncthreads();
   // X = V0s^T RV1, Y = U0s^T RcU1, M0 = (R Z1s) R^T
-  for (int e = tid; e < hh; e += NTF) {
+  for (int e = tid; e < hh; e += NTS) {
     const int i = e % h, j = e / h;
     double2 x = cz(), y = cz();
     for (int k = 0; k < h; ++k) {
       rfma(x, V0[k + i * h], RV1[k + j * h]);
       rfma(y, U0[k + i * h], RcU1[k + j * h]);
     }
-    X[e] = x;
-    Y[e] = y;
     const int off = m_off[j], sz = m_sz[j], base = m_base[j];
     double2 m = cz();
-    for (int k = 0; k < sz; ++k) cfma(m, SC[i + (off + k) * h], sR[base + (j - off) + k * sz]);
-    M0[e] = m;
+    for (int k = 0; k < sz; ++k) cfma(m, RZ1[i + (off + k) * h], sR[base + (j - off) + k * sz]);
+    out[e] = x;
+    out[hh + e] = y;
+    out[4 * hh + e] = m;
+  }
+}
+
+struct PhiParams {
+  int n, nphi, NE, nwg;
+  double cscale;
+  const double2 *sbuf;              // k_fac_setup output
+  const double *UfT, *VfT;          // [nstates][2][h*h], species blocks, row-major (U_s(j, k) at j h + k)
+  const double *Z;                  // [nstates][n*n]
+  const double *t1b;                // n x n
+  const double *mtab;               // [3][h]: m, <m+1|J_+|m> (0 at m = j), j(j+1) per species-local state
+  const VEntry *vl0, *vl1;          // species lists with local indices (p, n)
+  int nme0, nme1;
+  const int *q0, *q1;
+  double2 *out;                     // [npairs][NE][2][nphi][FAC_OUT]
+  double2 *rho;                     // [npairs][NE][2][nphi][h*h], column-major
+  int *status;                      // [npairs][NE][2][nphi]
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Row p (wave-uniform) of a column held in registers, by masks: an indexed
+// read of a register array would be placed in scratch memory.
+template <int H>
+__device__ __forceinline__ double2 pick_row(const double2 (&col)[H], int p) {
+  unsigned long long x = 0ull, y = 0ull;
+#pragma unroll
+  for (int r = 0; r < H; ++r) {
+    const unsigned long long m = (p == r) ? ~0ull : 0ull;
+    x |= m & (unsigned long long)__double_as_longlong(col[r].x);
+    y |= m & (unsigned long long)__double_as_longlong(col[r].y);
+  }
+  return make_double2(__longlong_as_double((long long)x), __longlong_as_double((long long)y));
+}
+
+// Gauss-Jordan with partial pivoting (pivot = max |re| + |im| over the rows not
+// yet used, first maximum, as ZGETRF's IZAMAX; LAPACK-style INFO for an
+// exactly-zero pivot) on an H-row system held one column per lane of the
+// wavefront: col[r] is row r of this lane's column, lanes 0..H-1 hold the
+// coefficient matrix, the other lanes right-hand sides.  Rows are not
+// exchanged: logical row k of the solution is physical row lmap[k].  The
+// workgroup is this one wavefront, so __syncthreads() only orders LDS.
+template <int H>
+__device__ int wave_gauss_jordan(double2 (&col)[H], double2 *fc, int *lmap) {
+  const int lane = threadIdx.x;
+  unsigned used = 0u;
+  int info = 0;
+  for (int k = 0; k < H; ++k) {
+    if (lane == k) {
+#pragma unroll
+      for (int r = 0; r < H; ++r) fc[r] = col[r];
+    }
+    __syncthreads();
+    double a = -1.0;
+    if (lane < H && !((used >> lane) & 1u)) {
+      a = cabs1(fc[lane]);
+      if (!(a >= 0.0)) a = 0.0;  // NaN: keep the choice wave-uniform and among the unused rows
+    }
+    int pr = lane;
+    wave_best(a, pr);
+    if (lane == 0) lmap[k] = pr;
+    used |= 1u << pr;
+    const double2 pv = fc[pr];
+    if (pv.x == 0.0 && pv.y == 0.0 && info == 0) info = k + 1;
+    const double2 prow = cmul(pick_row<H>(col, pr), crecip(pv));
+#pragma unroll
+    for (int r = 0; r < H; ++r) {
+      const double2 u = csub(col[r], cmul(fc[r], prow));
+      col[r] = (r == pr) ? prow : u;
+    }
+    __syncthreads();
+  }
+  return info;
+}
+
+// One wavefront per (pair, Euler point, species, gauge angle phi).
+//
+// Pfaffian of S(phi) (2H x 2H skew) by Parlett-Reid elimination with partial
+// pivoting: at step k the pivot of row r = 2k is the largest |S(r, c)|, c > r,
+// index c is exchanged with r + 1 (one transposition, sign -1), and the rank-2
+// Schur update removes r, r + 1.  ZPfaffianF (Codes/PNAMP/PNAMP_Proj.f90:218-350)
+// pivots on the largest entry of the whole remaining matrix; the Pfaffian is
+// the same polynomial, the rounding differs.  Its zero rule is kept: a pivot
+// of modulus <= 1e-13 gives Pf = 0.  The strict upper triangle is held in
+// registers, entry e <-> lane e % 64, slot e / 64, with entries numbered from
+// the bottom-right corner (rows descending), so the trailing matrix of step k
+// is entries e < C(2H - 2k, 2): the slot loop shrinks with k.  The columns
+// r, r + 1, c of the step go through LDS.
+//
+// T = e^{i phi} X + e^{-i phi} Y, W1 = e^{i phi} R V1s T^-1 and
+// W2 = e^{-i phi} R* U1s T^-1 solve T^T [W1 W2]^T = [e^{i phi} RV1, e^{-i phi} RcU1]^T
+// (Gauss-Jordan on lanes: T^T in lanes 0..H-1, right-hand sides in lanes
+// H..3H-1, or in two passes when 3H > 64), then rho = W1 V0s^T,
+// kappa10 = W1 U0s^T, kappa01 = -W2 V0s^T (ROKAPPAphiomega,
+// Codes/PNAMP/PNAMP_Proj.f90:459-551) with V0s, U0s read as wave-uniform
+// operands.
+//
+// <J^2> pieces (ZJ2_0, Codes/PNAMP/PNAMP_ME2B.f90:1-152) from the ladder form
+// sum_k J_k (x) J_k = J_z (x) J_z + (J_+ (x) J_- + J_- (x) J_+) / 2 with J_z
+// diagonal and J_+ one off-diagonal inside each orbit:
+//   A = sum_a j_a(j_a+1) rho_aa - sum_k Tr(J_k rho J_k rho) + sum_k Tr(J_k kappa10 J_k^T kappa01^T),
+//   t_+ = Tr(J_+ rho), t_- = Tr(J_- rho), t_z = Tr(J_z rho),
+// so <J^2> = A_p + A_n + (t_zp + t_zn)^2 + (t_+p + t_+n)(t_-p + t_-n).
+//
+// The eight-term list sum of HFB_Energy_COMPLEX
+// (Codes/PNAMP/PNAMP_Energy_OpenMP.f90:5-100) is regrouped exactly: with
+// A01 = kappa01 - kappa01^T and A10 = kappa10 - kappa10^T,
+//   (1/4) V [r_ijkl - r_jikl - r_ijlk + r_jilk + d (r_klij - r_klji - r_lkij + r_lkji)]
+//   = V [rho_ki rho_lj - rho_li rho_kj + A01_ij A10_kl / 4
+//        + d (rho_ik rho_jl - rho_jk rho_il + A01_kl A10_ij / 4)].
+template <int H>
+__global__ __launch_bounds__(64) void k_fac_phi(PhiParams p) {
+  constexpr int N2 = 2 * H, NU = N2 * (N2 - 1) / 2, SL = (NU + 63) / 64, HH = H * H;
+  constexpr int PASSES = (3 * H <= 64) ? 1 : 2;
+  __shared__ double2 sA[3 * HH];  // rho, kappa10, kappa01; Gauss-Jordan un-permute buffer before
+  __shared__ double2 colR[N2], colA[N2], colC[N2];
+  __shared__ double2 fc[H];
+  __shared__ int lmap[H];
+  const int lane = threadIdx.x;
+  // XCD-aware order: consecutive logical indices (the nphi waves of one setup
+  // block) stay on one XCD and its L2; gridDim.x is a multiple of 8.
+  const unsigned L = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  if (L >= (unsigned)p.nwg) return;
+  const int nphi = p.nphi, n = p.n;
+  const int f = (int)(L % (unsigned)nphi);
+  unsigned t = L / (unsigned)nphi;
+  const int sp = (int)(t & 1u);
+  t >>= 1;
+  const int iang = (int)(t % (unsigned)p.NE), ip = (int)(t / (unsigned)p.NE);
+  const size_t g = ((size_t)ip * p.NE + iang) * 2 + sp;
+  const size_t obase = g * nphi + f;
+  const double2 *X = p.sbuf + g * 5 * HH, *Y = X + HH, *RV1 = X + 2 * HH, *RcU1 = X + 3 * HH, *M0 = X + 4 * HH;
+  const int q0 = p.q0[ip];
+  const double *V0T = p.VfT + ((size_t)q0 * 2 + sp) * HH, *U0T = p.UfT + ((size_t)q0 * 2 + sp) * HH;
+  const double *Z0 = p.Z + (size_t)q0 * n * n + (size_t)sp * H * (n + 1);  // species block, ld n
+  const double phi = (nphi == 1) ? 0.0 : (2.0 * M_PI / nphi) * f;
+  const double2 e1 = cexpi(phi), e2 = cexpi(2.0 * phi);
+
+  // ---- Pfaffian
+  double2 pf = make_double2(1.0, 0.0);
+  {
+    double2 sv[SL];
+    int sij[SL];
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int e = s * 64 + lane;
+      int i = 0, j = 0;
+      double2 v = cz();
+      if (e < NU) {
+        int tt = (int)((1.0 + sqrt(1.0 + 8.0 * e)) * 0.5);
+        while (tt * (tt - 1) / 2 > e) --tt;
+        while ((tt + 1) * tt / 2 <= e) ++tt;
+        i = N2 - 1 - tt;
+        j = i + 1 + (e - tt * (tt - 1) / 2);
+        if (j < H) v = cmul(e2, M0[i + j * H]);
+        else if (i < H) v = make_double2((j - H == i) ? -1.0 : 0.0, 0.0);
+        else v = make_double2(-Z0[(i - H) + (size_t)(j - H) * n], 0.0);
+        v = cscale(p.cscale, v);
+      }
+      sv[s] = v;
+      sij[s] = i | (j << 8);
+    }
+    bool zero = false;
+    for (int k = 0; k < H - 1; ++k) {
+      const int r = 2 * k, m = N2 - r;
+      const int cnt = m * (m - 1) / 2, lo = (m - 1) * (m - 2) / 2, cnt2 = (m - 2) * (m - 3) / 2;
+      double bv = -1.0;
+      int bj = 1 << 20;
+#pragma unroll
+      for (int s = 0; s < SL; ++s) {
+        if (s * 64 < cnt && (s + 1) * 64 > lo) {
+          const int e = s * 64 + lane;
+          if (e >= lo && e < cnt) {
+            const double a = sv[s].x * sv[s].x + sv[s].y * sv[s].y;
+            const int j = sij[s] >> 8;
+            if (a > bv || (a == bv && j < bj)) { bv = a; bj = j; }
+          }
+        }
+      }
+      wave_best(bv, bj);
+      if (!(bv > 1.0e-26)) {  // |pivot| <= 1e-13
+        zero = true;
+        break;
+      }
+      const int c = bj;
+#pragma unroll
+      for (int s = 0; s < SL; ++s) {
+        if (s * 64 < cnt) {
+          const int e = s * 64 + lane;
+          if (e < cnt) {
+            const int i = sij[s] & 255, j = sij[s] >> 8;
+            const double2 v = sv[s];
+            if (i == r) colR[j] = cneg(v);
+            if (i == r + 1) colA[j] = cneg(v);
+            if (j == r + 1) colA[i] = v;
+            if (i == c) colC[j] = cneg(v);
+            if (j == c) colC[i] = v;
+          }
+        }
+      }
+      __syncthreads();
+      const double2 spiv = cneg(colR[c]);  // S(r, c)
+      const double2 inv = crecip(spiv);
+      pf = cmul(pf, (c != r + 1) ? cneg(spiv) : spiv);
+      // trailing entries i >= r + 2 after exchanging c <-> r + 1:
+      // S'(x, r) = colR[s(x)], S'(x, r + 1) = colC[s(x)], s = the exchange
+#pragma unroll
+      for (int s = 0; s < SL; ++s) {
+        if (s * 64 < cnt2) {
+          const int e = s * 64 + lane;
+          if (e < cnt2) {
+            const int i = sij[s] & 255, j = sij[s] >> 8;
+            double2 v = sv[s];
+            if (j == c) v = colA[i];
+            else if (i == c) v = cneg(colA[j]);
+            const int si = (i == c) ? r + 1 : i, sj = (j == c) ? r + 1 : j;
+            const double2 tu = csub(cmul(colC[si], colR[sj]), cmul(colR[si], colC[sj]));
+            sv[s] = cadd(v, cmul(tu, inv));
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (zero) {
+      pf = cz();
+    } else {
+      const double2 last = make_double2(__shfl(sv[0].x, 0), __shfl(sv[0].y, 0));  // S(2H-2, 2H-1)
+      pf = cmul(pf, last);
+    }
+  }
+
+  // ---- Gauss-Jordan and the density products
+  int info = 0;
+  const bool w1lane = lane >= H && lane < 2 * H;
+  const bool w2lane1 = lane >= 2 * H && lane < 3 * H;  // PASSES == 1 only
+  const int irow = w2lane1 ? lane - 2 * H : lane - H;
+  double2 *rho_out = p.rho + obase * HH;
+#pragma unroll 1
+  for (int pass = 0; pass < PASSES; ++pass) {
+    double2 col[H];
+    if (lane < H) {
+#pragma unroll
+      for (int r = 0; r < H; ++r)
+        col[r] = cadd(cmul(e1, X[lane + r * H]), cmul(cconj(e1), Y[lane + r * H]));
+    } else if (w1lane || (PASSES == 1 && w2lane1)) {
+      const bool second = (PASSES == 1) ? w2lane1 : (pass == 1);
+      const double2 *src = second ? RcU1 : RV1;
+      const double2 ph = second ? cconj(e1) : e1;
+#pragma unroll
+      for (int r = 0; r < H; ++r) col[r] = cmul(ph, src[irow + r * H]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < H; ++r) col[r] = cz();
+    }
+    const int inf = wave_gauss_jordan<H>(col, fc, lmap);
+    if (inf && !info) info = inf;
+    // solution rows back to logical order through LDS
+    constexpr int NB = (PASSES == 1) ? 2 * H : H;
+    double2 *ub = (PASSES == 1) ? sA + HH : sA + 2 * HH;
+    const bool blane = lane >= H && lane < H + NB;
+    if (blane) {
+#pragma unroll
+      for (int r = 0; r < H; ++r) ub[(lane - H) * H + r] = col[r];
+    }
+    __syncthreads();
+    double2 w[H];
+#pragma unroll
+    for (int k = 0; k < H; ++k) w[k] = blane ? ub[(lane - H) * H + lmap[k]] : cz();
+    __syncthreads();
+    const bool isW1 = w1lane && (PASSES == 1 || pass == 0);
+    const bool isW2 = (PASSES == 1) ? w2lane1 : (pass == 1 && w1lane);
+    // V0s sweep: rho (W1 lanes), kappa01 (W2 lanes)
+    for (int j = 0; j < H; ++j) {
+      double2 acc = cz();
+#pragma unroll
+      for (int k = 0; k < H; ++k) rfma(acc, V0T[j * H + k], w[k]);
+      if (isW1) {
+        sA[irow + j * H] = acc;
+        rho_out[irow + j * H] = acc;
+      }
+      if (isW2) sA[2 * HH + irow + j * H] = cneg(acc);
+    }
+    if (PASSES == 1 || pass == 0) {
+      // U0s sweep: kappa10 (W1 lanes)
+      for (int j = 0; j < H; ++j) {
+        double2 acc = cz();
+#pragma unroll
+        for (int k = 0; k < H; ++k) rfma(acc, U0T[j * H + k], w[k]);
+        if (isW1) sA[HH + irow + j * H] = acc;
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- traces, <J^2> pieces, kinetic energy
+  const double2 *ro = sA;
+  double2 *k10 = sA + HH, *k01 = sA + 2 * HH;
+  const double *mz = p.mtab, *cp = p.mtab + H, *jj1 = p.mtab + 2 * H;
+  const double *t1 = p.t1b + (size_t)sp * H * (n + 1);  // species block, ld n
+  double2 tr = cz(), ek = cz(), aj = cz(), tp = cz(), tm = cz(), tz = cz();
+  for (int e = lane; e < HH; e += 64) {
+    const int a = e % H, b = e / H;
+    const double ma = mz[a], mb = mz[b], ca = cp[a], cb = cp[b];
+    const double2 rab = ro[a + b * H];
+    ek = cadd(ek, cscale(t1[b + (size_t)a * n], rab));
+    aj = csub(aj, cscale(ma * mb, cmul(rab, ro[b + a * H])));
+    aj = cadd(aj, cscale(ma * mb, cmul(k10[a + b * H], k01[a + b * H])));
+    if (a > 0 && b > 0 && ca != 0.0 && cb != 0.0) {
+      aj = csub(aj, cscale(ca * cb, cmul(rab, ro[(b - 1) + (a - 1) * H])));
+      const double2 kk = cadd(cmul(k10[a + (b - 1) * H], k01[(a - 1) + b * H]),
+                              cmul(k10[(a - 1) + b * H], k01[a + (b - 1) * H]));
+      aj = cadd(aj, cscale(0.5 * ca * cb, kk));
+    }
+    if (a == b) {
+      tr = cadd(tr, rab);
+      tz = cadd(tz, cscale(ma, rab));
+      aj = cadd(aj, cscale(jj1[a], rab));
+    }
+    if (a == b + 1) tp = cadd(tp, cscale(ca, rab));
+    if (b == a + 1) tm = cadd(tm, cscale(cb, rab));
   }
   __syncthreads();
-  int status = 0;
-  const int nphi = p.nphi;
-  for (int f = 0; f < nphi; ++f) {
-    const double phi = (nphi == 1) ? 0.0 : (2.0 * M_PI / nphi) * f;
-    const double2 e1 = cexpi(phi), e2 = cexpi(2.0 * phi);
-    // S = cscale [[e^{2 i phi} M0, -1], [1, -Z0s]] (upper triangle, ld 2h)
-    for (int j = tid / 64; j < N2; j += NTF / 64)
-      for (int i = lane; i <= j; i += 64) {
-        double2 v;
-        if (i == j) v = cz();
-        else if (j < h) v = cmul(e2, M0[i + j * h]);
-        else if (i < h) v = make_double2((j - h == i) ? -1.0 : 0.0, 0.0);
-        else v = make_double2(-Z0[(i - h) + (size_t)(j - h) * n], 0.0);
-        SC[i + j * N2] = cscale(p.cscale, v);
-        SC[j + i * N2] = cscale(-p.cscale, v);
-      }
-    __syncthreads();
-    const double2 pf = block_pfaffian<NTF>(SC, N2, pfmap, redv, redk);
-    // T = e^{i phi} X + e^{-i phi} Y -> SC[0], 1 -> SC[1]
-    double2 *T = SC, *Ti = SC + hh, *W1 = SC + 2 * hh, *W2 = SC + 3 * hh;
-    for (int e = tid; e < hh; e += NTF) {
-      T[e] = cadd(cmul(e1, X[e]), cmul(cconj(e1), Y[e]));
-      Ti[e] = make_double2((e % h) == (e / h) ? 1.0 : 0.0, 0.0);
+  // kappa -> A = kappa - kappa^T in place
+  for (int e = lane; e < HH; e += 64) {
+    const int a = e % H, b = e / H;
+    if (a < b) {
+      const double2 x10 = csub(k10[a + b * H], k10[b + a * H]);
+      const double2 x01 = csub(k01[a + b * H], k01[b + a * H]);
+      k10[a + b * H] = x10;
+      k10[b + a * H] = cneg(x10);
+      k01[a + b * H] = x01;
+      k01[b + a * H] = cneg(x01);
     }
-    __syncthreads();
-    {
-      const int info = block_gauss_jordan<NTF>(T, Ti, h, gjmap, fcol, prow, redk);
-      if (info && !status) status = info;
-    }
-    // W1 = e^{i phi} RV1 T^{-1}, W2 = e^{-i phi} RcU1 T^{-1}
-    for (int e = tid; e < hh; e += NTF) {
-      const int i = e % h, j = e / h;
-      double2 a = cz(), b = cz();
-      for (int k = 0; k < h; ++k) {
-        const double2 t = Ti[gjmap[k] + j * h];
-        cfma(a, RV1[i + k * h], t);
-        cfma(b, RcU1[i + k * h], t);
-      }
-      W1[e] = cmul(e1, a);
-      W2[e] = cmul(cconj(e1), b);
-    }
-    __syncthreads();
-    // rho = W1 V0s^T -> SC[0], kappa10 = W1 U0s^T -> SC[1], kappa01 = -W2 V0s^T -> SC[4]
-    double2 *ro = SC, *k10 = SC + hh, *k01 = SC + 4 * hh;
-    for (int e = tid; e < hh; e += NTF) {
-      const int i = e % h, j = e / h;
-      double2 r = cz(), a = cz(), b = cz();
-      for (int k = 0; k < h; ++k) {
-        const double2 w1 = W1[i + k * h], w2 = W2[i + k * h];
-        const double v0 = V0[j + k * h], u0 = U0[j + k * h];
-        rfma(r, v0, w1);
-        rfma(a, u0, w1);
-        rfma(b, v0, w2);
-      }
-      ro[e] = r;
-      k10[e] = a;
-      k01[e] = cneg(b);
-    }
-    __syncthreads();
-    double2 j2s[4];
-    block_j2<NTF>(ro, k10, k01, h, h, p.jpack, p.npack, m_off, m_sz, m_base, W1, W2, redv, j2s);
-    double2 e2b = cz(), ek = cz(), tr = cz();
-    for (int q = tid; q < nme; q += NTF) {
-      const VEntry ve = vl[q];
-      const int i = ve.idx & 255, j = (ve.idx >> 8) & 255, k = (ve.idx >> 16) & 255, l = ve.idx >> 24;
-      const double2 rki = ro[k + i * h], rlj = ro[l + j * h], rli = ro[l + i * h], rkj = ro[k + j * h];
-      const double2 rik = ro[i + k * h], rjl = ro[j + l * h], rjk = ro[j + k * h], ril = ro[i + l * h];
-      const double2 pij = k01[i + j * h], pji = k01[j + i * h], pkl = k01[k + l * h], plk = k01[l + k * h];
-      const double2 qkl = k10[k + l * h], qlk = k10[l + k * h], qij = k10[i + j * h], qji = k10[j + i * h];
-      const double2 r_ijkl = cadd(csub(cmul(rki, rlj), cmul(rli, rkj)), cmul(pij, qkl));
-      const double2 r_jikl = cadd(csub(cmul(rkj, rli), cmul(rlj, rki)), cmul(pji, qkl));
-      const double2 r_jilk = cadd(csub(cmul(rlj, rki), cmul(rkj, rli)), cmul(pji, qlk));
-      const double2 r_ijlk = cadd(csub(cmul(rli, rkj), cmul(rki, rlj)), cmul(pij, qlk));
-      const double2 r_klij = cadd(csub(cmul(rik, rjl), cmul(rjk, ril)), cmul(pkl, qij));
-      const double2 r_klji = cadd(csub(cmul(rjk, ril), cmul(rik, rjl)), cmul(pkl, qji));
-      const double2 r_lkij = cadd(csub(cmul(ril, rjk), cmul(rjl, rik)), cmul(plk, qij));
-      const double2 r_lkji = cadd(csub(cmul(rjl, rik), cmul(ril, rjk)), cmul(plk, qji));
-      const double d = (k + l == i + j) ? 0.0 : 1.0;
-      double2 sa = cadd(csub(csub(r_ijkl, r_jikl), r_ijlk), r_jilk);
-      double2 sb = cadd(csub(csub(r_klij, r_klji), r_lkij), r_lkji);
-      e2b = cadd(e2b, cscale(0.25 * ve.me, cadd(sa, cscale(d, sb))));
-    }
-    for (int e = tid; e < hh; e += NTF) {
-      const int a = e % h, c = e / h;
-      ek = cadd(ek, cscale(p.t1b[(sp * h + a) + (size_t)(sp * h + c) * n], ro[c + a * h]));
-    }
-    for (int i = tid; i < h; i += NTF) tr = cadd(tr, ro[i + i * h]);
-    double vals[6] = {e2b.x, e2b.y, ek.x, ek.y, tr.x, tr.y};
-    for (int v = 0; v < 6; ++v)
-      for (int off = 32; off > 0; off >>= 1) vals[v] += __shfl_xor(vals[v], off);
-    if (lane == 0)
-      for (int v = 0; v < 6; ++v) redv[wave * 8 + v] = vals[v];
-    const size_t obase = ((((size_t)ip * p.NE + iang) * 2 + sp) * nphi + f);
-    double2 *rout = p.rho + obase * hh;
-    for (int e = tid; e < hh; e += NTF) rout[e] = ro[e];
-    __syncthreads();
-    if (tid == 0) {
-      double t[6] = {0, 0, 0, 0, 0, 0};
-      for (int w = 0; w < NTF / 64; ++w)
-        for (int v = 0; v < 6; ++v) t[v] += redv[w * 8 + v];
-      double2 *o = p.out + obase * 8;
-      o[0] = pf;
-      o[1] = make_double2(t[0], t[1]);
-      o[2] = make_double2(t[2], t[3]);
-      o[3] = make_double2(t[4], t[5]);
-      for (int v = 0; v < 4; ++v) o[4 + v] = j2s[v];
-    }
-    __syncthreads();
   }
-  if (tid == 0) p.status[((size_t)ip * p.NE + iang) * 2 + sp] = status;
+  __syncthreads();
+  // ---- species list
+  const VEntry *vl = sp ? p.vl1 : p.vl0;
+  const int nme = sp ? p.nme1 : p.nme0;
+  const double2 *A10 = k10, *A01 = k01;
+  double2 e2b = cz();
+  for (int q = lane; q < nme; q += 64) {
+    const VEntry ve = vl[q];
+    const int i = ve.idx & 255, j = (ve.idx >> 8) & 255, k = (ve.idx >> 16) & 255, l = ve.idx >> 24;
+    double2 s = csub(cmul(ro[k + i * H], ro[l + j * H]), cmul(ro[l + i * H], ro[k + j * H]));
+    s = cadd(s, cscale(0.25, cmul(A01[i + j * H], A10[k + l * H])));
+    if (k + l != i + j) {
+      s = cadd(s, csub(cmul(ro[i + k * H], ro[j + l * H]), cmul(ro[j + k * H], ro[i + l * H])));
+      s = cadd(s, cscale(0.25, cmul(A01[k + l * H], A10[i + j * H])));
+    }
+    e2b = cadd(e2b, cscale(ve.me, s));
+  }
+  double v[14] = {e2b.x, e2b.y, ek.x, ek.y, tr.x, tr.y, aj.x, aj.y, tp.x, tp.y, tm.x, tm.y, tz.x, tz.y};
+#pragma unroll
+  for (int q = 0; q < 14; ++q) v[q] = wave_sum(v[q]);
+  if (lane == 0) {
+    double2 *o = p.out + obase * FAC_OUT;
+    o[0] = pf;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) o[1 + q] = make_double2(v[2 * q], v[2 * q + 1]);
+    p.status[obase] = info;
+  }
 }
 
 struct CombParams {
-  int h, nphi, NE, zval, nval;
+  int h, NE, zval, nval;
   double vac_scale, snpfaf, sgn_h, e0;
   const double *det;
   const int *q0, *q1;
   const int *w_ptr, *w_col;         // W_pn in CSR over x = c + a h (rows), y = d + b h (columns)
   const double *w_val;
   const double2 *out, *rho;
-  const int *fstatus;
+  const int *fstatus;               // [npairs][NE][2][nphi]
   double2 *zres;                    // [npairs][NE][NOBS]
   int *status;                      // [npairs][NE]
 };
@@ -960,99 +1206,102 @@ struct CombParams {
 // a + b != c + d (for a, c protons and b, d neutrons the eight-term sum of
 // HFB_Energy_COMPLEX reduces to V [rho_p(c,a) rho_n(d,b) + rho_p(a,c) rho_n(b,d)]).
 // G = W rho_n(phi_n) is formed once per phi_n from the CSR rows of W, then
-// the nphi^2 dot products with rho_p(phi_p); the gauge points are summed into ZRES.
+// the nphi^2 dot products with rho_p(phi_p); the gauge points are summed into
+// ZRES.  NP = nphi is a template argument so the per-gauge register arrays
+// stay in registers.
+template <int NP>
 __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
-  extern __shared__ double2 dsm[];  // rho_p [nphi][h^2], rho_n -> G [nphi][h^2]
-  __shared__ double2 epn[NPHI_MAX * NPHI_MAX];
+  extern __shared__ double2 dsm[];  // rho_p [NP][h^2], rho_n -> G [NP][h^2]
+  __shared__ double2 epn[NP * NP];
   __shared__ double red[NTC / 64 * 2 * NOBS];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int iang = blockIdx.x, ip = blockIdx.y;
-  const int h = p.h, hh = h * h, nphi = p.nphi;
+  const int h = p.h, hh = h * h;
   const size_t base = ((size_t)ip * p.NE + iang) * 2;
-  double2 *rp = dsm, *rn = dsm + (size_t)nphi * hh;
-  for (int e = tid; e < nphi * hh; e += NTC) {
-    rp[e] = p.rho[base * nphi * hh + e];
-    rn[e] = p.rho[(base + 1) * nphi * hh + e];
+  double2 *rp = dsm, *rn = dsm + (size_t)NP * hh;
+  for (int e = tid; e < NP * hh; e += NTC) {
+    rp[e] = p.rho[base * NP * hh + e];
+    rn[e] = p.rho[(base + 1) * NP * hh + e];
   }
   __syncthreads();
   // G rows x = tid, tid + NTC (h^2 <= 2 NTC), all phi_n, in registers
   constexpr int RPT = 2;
-  double2 g[RPT][NPHI_MAX];
+  double2 g[RPT][NP];
+#pragma unroll
   for (int r = 0; r < RPT; ++r) {
-    for (int f = 0; f < NPHI_MAX; ++f) g[r][f] = cz();
+#pragma unroll
+    for (int f = 0; f < NP; ++f) g[r][f] = cz();
     const int x = tid + r * NTC;
-    if (x >= hh) continue;
-    for (int q = p.w_ptr[x]; q < p.w_ptr[x + 1]; ++q) {
-      const int y = p.w_col[q];
-      const double v = p.w_val[q];
-      for (int f = 0; f < NPHI_MAX; ++f) {
-        if (f >= nphi) break;
-        rfma(g[r][f], v, rn[(size_t)f * hh + y]);
+    if (x < hh) {
+      for (int q = p.w_ptr[x]; q < p.w_ptr[x + 1]; ++q) {
+        const int y = p.w_col[q];
+        const double v = p.w_val[q];
+#pragma unroll
+        for (int f = 0; f < NP; ++f) rfma(g[r][f], v, rn[(size_t)f * hh + y]);
       }
     }
   }
   __syncthreads();
+#pragma unroll
   for (int r = 0; r < RPT; ++r) {
     const int x = tid + r * NTC;
-    if (x >= hh) continue;
-    for (int f = 0; f < NPHI_MAX; ++f) {
-      if (f >= nphi) break;
-      rn[(size_t)f * hh + x] = g[r][f];
+    if (x < hh) {
+#pragma unroll
+      for (int f = 0; f < NP; ++f) rn[(size_t)f * hh + x] = g[r][f];
     }
   }
   __syncthreads();
   const int grp = tid >> 5, gl = tid & 31;
-  for (int fp = grp; fp < nphi; fp += NTC / 32) {
-    double2 acc[NPHI_MAX];
-    for (int f = 0; f < NPHI_MAX; ++f) acc[f] = cz();
+  for (int fp = grp; fp < NP; fp += NTC / 32) {
+    double2 acc[NP];
+#pragma unroll
+    for (int f = 0; f < NP; ++f) acc[f] = cz();
     const double2 *rpf = rp + (size_t)fp * hh;
     for (int x = gl; x < hh; x += 32) {
       const double2 a = rpf[x];
-      for (int f = 0; f < NPHI_MAX; ++f) {
-        if (f >= nphi) break;
-        cfma(acc[f], a, rn[(size_t)f * hh + x]);
-      }
+#pragma unroll
+      for (int f = 0; f < NP; ++f) cfma(acc[f], a, rn[(size_t)f * hh + x]);
     }
-    for (int f = 0; f < nphi; ++f) {
+#pragma unroll
+    for (int f = 0; f < NP; ++f) {
       double re = acc[f].x, im = acc[f].y;
       for (int off = 16; off > 0; off >>= 1) {
         re += __shfl_xor(re, off);
         im += __shfl_xor(im, off);
       }
-      if (gl == 0) epn[fp * nphi + f] = make_double2(re, im);
+      if (gl == 0) epn[fp * NP + f] = make_double2(re, im);
     }
   }
   __syncthreads();
-  const double2 *op = p.out + base * nphi * 8, *on = p.out + (base + 1) * nphi * 8;
+  const double2 *op = p.out + base * NP * FAC_OUT, *on = p.out + (base + 1) * NP * FAC_OUT;
   const double vac = p.snpfaf * p.sgn_h * (p.det[p.q0[ip]] * p.vac_scale) * (p.det[p.q1[ip]] * p.vac_scale);
   double v[2 * NOBS] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int t = tid; t < nphi * nphi; t += NTC) {
-    const int fp = t / nphi, fn = t % nphi;
-    const double phip = (nphi == 1) ? 0.0 : (2.0 * M_PI / nphi) * fp;
-    const double phin = (nphi == 1) ? 0.0 : (2.0 * M_PI / nphi) * fn;
-    const double2 *a = op + fp * 8, *b = on + fn * 8;
+  for (int t = tid; t < NP * NP; t += NTC) {
+    const int fp = t / NP, fn = t % NP;
+    const double phip = (NP == 1) ? 0.0 : (2.0 * M_PI / NP) * fp;
+    const double phin = (NP == 1) ? 0.0 : (2.0 * M_PI / NP) * fn;
+    const double2 *a = op + fp * FAC_OUT, *b = on + fn * FAC_OUT;
     const double2 zn = cmul(cscale(vac, cmul(a[0], b[0])), cexpi(-phip * p.zval - phin * p.nval));
     const double2 ekin = cadd(a[2], b[2]);
-    const double2 E = cadd(cadd(cadd(a[1], b[1]), epn[fp * nphi + fn]), cadd(ekin, make_double2(p.e0, 0.0)));
-    // <J^2> = A_p + A_n + sum_k (t_kp + t_kn)^2
-    double2 J2 = cadd(a[4], b[4]);
-    for (int k = 0; k < 3; ++k) {
-      const double2 tk = cadd(a[5 + k], b[5 + k]);
-      J2 = cadd(J2, cmul(tk, tk));
-    }
+    const double2 E = cadd(cadd(cadd(a[1], b[1]), epn[fp * NP + fn]), cadd(ekin, make_double2(p.e0, 0.0)));
+    // <J^2> = A_p + A_n + (t_zp + t_zn)^2 + (t_+p + t_+n)(t_-p + t_-n)
+    const double2 tzs = cadd(a[7], b[7]);
+    const double2 J2 = cadd(cadd(cadd(a[4], b[4]), cmul(tzs, tzs)), cmul(cadd(a[5], b[5]), cadd(a[6], b[6])));
     const double2 o[NOBS] = {zn, cmul(zn, a[3]), cmul(zn, b[3]), cmul(zn, ekin), cmul(zn, E), cmul(zn, J2)};
+#pragma unroll
     for (int k = 0; k < NOBS; ++k) {
       v[2 * k] += o[k].x;
       v[2 * k + 1] += o[k].y;
     }
   }
+#pragma unroll
   for (int k = 0; k < 2 * NOBS; ++k)
     for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
   if (lane == 0)
     for (int k = 0; k < 2 * NOBS; ++k) red[wave * 2 * NOBS + k] = v[k];
   __syncthreads();
   if (tid == 0) {
-    const double inv = 1.0 / (double)(nphi * nphi);
+    const double inv = 1.0 / (double)(NP * NP);
     double2 *z = p.zres + ((size_t)ip * p.NE + iang) * NOBS;
     for (int k = 0; k < NOBS; ++k) {
       double re = 0, im = 0;
@@ -1062,8 +1311,85 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
       }
       z[k] = make_double2(inv * re, inv * im);
     }
-    const int s0 = p.fstatus[base], s1 = p.fstatus[base + 1];
-    p.status[(size_t)ip * p.NE + iang] = s0 ? s0 : s1;
+    int st = 0;
+    for (int q = 0; q < 2 * NP && !st; ++q) st = p.fstatus[base * NP + q];
+    p.status[(size_t)ip * p.NE + iang] = st;
+  }
+}
+
+template <int H>
+static void launch_phi(unsigned grid, hipStream_t s, const PhiParams &pp) {
+  hipLaunchKernelGGL(k_fac_phi<H>, dim3(grid), dim3(64), 0, s, pp);
+}
+
+// Species sizes h = n / 2 with an instantiated k_fac_phi (every h is even:
+// an orbit holds 2j + 1 states).
+static bool launch_phi_h(int h, unsigned grid, hipStream_t s, const PhiParams &pp) {
+  switch (h) {
+    case 2: launch_phi<2>(grid, s, pp); return true;
+    case 4: launch_phi<4>(grid, s, pp); return true;
+    case 6: launch_phi<6>(grid, s, pp); return true;
+    case 8: launch_phi<8>(grid, s, pp); return true;
+    case 10: launch_phi<10>(grid, s, pp); return true;
+    case 12: launch_phi<12>(grid, s, pp); return true;
+    case 14: launch_phi<14>(grid, s, pp); return true;
+    case 16: launch_phi<16>(grid, s, pp); return true;
+    case 18: launch_phi<18>(grid, s, pp); return true;
+    case 20: launch_phi<20>(grid, s, pp); return true;
+    case 22: launch_phi<22>(grid, s, pp); return true;
+    default: return false;
+  }
+}
+
+template <int NP>
+static void launch_combine(dim3 grid, size_t lds, hipStream_t s, const CombParams &cp) {
+  hipLaunchKernelGGL(k_fac_combine<NP>, grid, dim3(NTC), lds, s, cp);
+}
+
+// Dynamic LDS limit of k_fac_combine<np>, set before any capture.
+static bool combine_attr(int np, size_t lds) {
+  const void *fn = nullptr;
+  switch (np) {
+    case 1: fn = (const void *)k_fac_combine<1>; break;
+    case 2: fn = (const void *)k_fac_combine<2>; break;
+    case 3: fn = (const void *)k_fac_combine<3>; break;
+    case 4: fn = (const void *)k_fac_combine<4>; break;
+    case 5: fn = (const void *)k_fac_combine<5>; break;
+    case 6: fn = (const void *)k_fac_combine<6>; break;
+    case 7: fn = (const void *)k_fac_combine<7>; break;
+    case 8: fn = (const void *)k_fac_combine<8>; break;
+    case 9: fn = (const void *)k_fac_combine<9>; break;
+    case 10: fn = (const void *)k_fac_combine<10>; break;
+    case 11: fn = (const void *)k_fac_combine<11>; break;
+    case 12: fn = (const void *)k_fac_combine<12>; break;
+    case 13: fn = (const void *)k_fac_combine<13>; break;
+    case 14: fn = (const void *)k_fac_combine<14>; break;
+    case 15: fn = (const void *)k_fac_combine<15>; break;
+    case 16: fn = (const void *)k_fac_combine<16>; break;
+    default: return false;
+  }
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess;
+}
+
+static bool launch_combine_np(int np, dim3 grid, size_t lds, hipStream_t s, const CombParams &cp) {
+  switch (np) {
+    case 1: launch_combine<1>(grid, lds, s, cp); return true;
+    case 2: launch_combine<2>(grid, lds, s, cp); return true;
+    case 3: launch_combine<3>(grid, lds, s, cp); return true;
+    case 4: launch_combine<4>(grid, lds, s, cp); return true;
+    case 5: launch_combine<5>(grid, lds, s, cp); return true;
+    case 6: launch_combine<6>(grid, lds, s, cp); return true;
+    case 7: launch_combine<7>(grid, lds, s, cp); return true;
+    case 8: launch_combine<8>(grid, lds, s, cp); return true;
+    case 9: launch_combine<9>(grid, lds, s, cp); return true;
+    case 10: launch_combine<10>(grid, lds, s, cp); return true;
+    case 11: launch_combine<11>(grid, lds, s, cp); return true;
+    case 12: launch_combine<12>(grid, lds, s, cp); return true;
+    case 13: launch_combine<13>(grid, lds, s, cp); return true;
+    case 14: launch_combine<14>(grid, lds, s, cp); return true;
+    case 15: launch_combine<15>(grid, lds, s, cp); return true;
+    case 16: launch_combine<16>(grid, lds, s, cp); return true;
+    default: return false;
   }
 }
 
@@ -1306,8 +1632,9 @@ struct pnamp_ctx {
   bool separable = false;
   int nme_s[3] = {0, 0, 0};
   VEntry *d_vs[3] = {nullptr, nullptr, nullptr};  // pp, nn (local indices), pn
-  double *d_Uf = nullptr, *d_Vf = nullptr;
-  double2 *d_fout = nullptr, *d_frho = nullptr;
+  double *d_Uf = nullptr, *d_Vf = nullptr, *d_UfT = nullptr, *d_VfT = nullptr;
+  double *d_mtab = nullptr;         // [3][h]: m, <m+1|J_+|m>, j(j+1) per species-local state
+  double2 *d_fout = nullptr, *d_frho = nullptr, *d_sbuf = nullptr;
   int *d_fstatus = nullptr;
   int *d_wptr = nullptr, *d_wcol = nullptr;
   double *d_wval = nullptr;
@@ -1475,6 +1802,18 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
       (r = dupload(c, &c->d_gam, mesh->gam, c->ngam)) || (r = dupload(c, &c->d_wgam, mesh->wgam, c->ngam)))
     return bail(r, c->err);
   if ((r = dupload(c, &c->d_jpack, jpk.data(), jpk.size()))) return bail(r, c->err);
+  {
+    // ladder-operator table of the factored form, per species-local state a
+    // (m = j .. -j inside an orbit, so a - 1 is the m + 1 partner of a)
+    std::vector<double> mt(3 * (size_t)h, 0.0);
+    for (int a = 0; a < h; ++a) {
+      const double j = basis->twoj[a] / 2.0, m = basis->twom[a] / 2.0;
+      mt[a] = m;
+      mt[h + a] = (a > off[a]) ? std::sqrt(j * (j + 1) - m * (m + 1)) : 0.0;
+      mt[2 * h + a] = j * (j + 1);
+    }
+    if ((r = dupload(c, &c->d_mtab, mt.data(), mt.size()))) return bail(r, c->err);
+  }
   if ((r = dupload(c, &c->d_pk_a, pa.data(), pa.size())) || (r = dupload(c, &c->d_pk_b, pb.data(), pb.size())) ||
       (r = dupload(c, &c->d_blk_off, off.data(), h)) || (r = dupload(c, &c->d_blk_sz, sz.data(), h)) ||
       (r = dupload(c, &c->d_blk_base, base.data(), h)) ||
@@ -1606,7 +1945,16 @@ extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, cons
   c->separable = sep;
   if (sep) {
     int r2 = 0;
-    if ((r2 = dupload(c, &c->d_Uf, Uf.data(), Uf.size())) || (r2 = dupload(c, &c->d_Vf, Vf.data(), Vf.size())))
+    // row-major copies (U_s(j, k) at j h + k) for the wave-uniform reads of k_fac_phi
+    std::vector<double> UfT(Uf.size()), VfT(Vf.size());
+    for (size_t b = 0; b < (size_t)nstates * 2; ++b)
+      for (int j = 0; j < h; ++j)
+        for (int k = 0; k < h; ++k) {
+          UfT[b * h * h + (size_t)j * h + k] = Uf[b * h * h + j + (size_t)k * h];
+          VfT[b * h * h + (size_t)j * h + k] = Vf[b * h * h + j + (size_t)k * h];
+        }
+    if ((r2 = dupload(c, &c->d_Uf, Uf.data(), Uf.size())) || (r2 = dupload(c, &c->d_Vf, Vf.data(), Vf.size())) ||
+        (r2 = dupload(c, &c->d_UfT, UfT.data(), UfT.size())) || (r2 = dupload(c, &c->d_VfT, VfT.data(), VfT.size())))
       return r2;
   }
   return PNAMP_OK;
@@ -1671,27 +2019,33 @@ extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0
     const size_t comb_lds = sizeof(double2) * 2 * (size_t)c->nphi * hh;
     if (comb_lds > 150 * 1024) return fail(c, PNAMP_E_SIZE, "2 nphi h^2 complex exceeds the combine LDS");
     if (hh > 2 * NTC) return fail(c, PNAMP_E_SIZE, "h^2 exceeds the combine rows per workgroup");
-    const size_t per_pair = (size_t)c->NE * 2 * c->nphi * hh;
-    const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)npairs, ((size_t)4 << 30) / (per_pair * 16)));
+    if (h > HMAX || (h & 1)) return fail(c, PNAMP_E_SIZE, "factored form: species size h must be even and <= 22");
+    // per pair: rho [NE][2][nphi][h^2], setup blocks [NE][2][5][h^2], per-phi scalars and status;
+    // chunks of pairs up to 16 GiB of scratch
+    const size_t rho_pp = (size_t)c->NE * 2 * c->nphi * hh, sb_pp = (size_t)c->NE * 2 * 5 * hh;
+    const size_t out_pp = (size_t)c->NE * 2 * c->nphi * FAC_OUT, st_pp = (size_t)c->NE * 2 * c->nphi;
+    const size_t bytes_pp = 16 * (rho_pp + sb_pp + out_pp) + 4 * st_pp;
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)npairs, ((size_t)16 << 30) / bytes_pp));
     if ((size_t)chunk > c->fac_cap) {
-      if ((r = dalloc(c, &c->d_frho, per_pair * chunk)) || (r = dalloc(c, &c->d_fout, (size_t)c->NE * 2 * c->nphi * 8 * chunk)) ||
-          (r = dalloc(c, &c->d_fstatus, (size_t)c->NE * 2 * chunk)))
+      if ((r = dalloc(c, &c->d_frho, rho_pp * chunk)) || (r = dalloc(c, &c->d_sbuf, sb_pp * chunk)) ||
+          (r = dalloc(c, &c->d_fout, out_pp * chunk)) || (r = dalloc(c, &c->d_fstatus, st_pp * chunk)))
         return r;
       c->fac_cap = chunk;
     }
-    (void)hipFuncSetAttribute((const void *)k_fac_combine, hipFuncAttributeMaxDynamicSharedMemorySize, (int)comb_lds);
-    const size_t spec_lds = sizeof(double2) * 10 * (size_t)hh;
-    (void)hipFuncSetAttribute((const void *)k_fac_species, hipFuncAttributeMaxDynamicSharedMemorySize, (int)spec_lds);
-    FacParams fp;
-    fp.n = c->n; fp.h = h; fp.nalp = c->nalp; fp.nbet = c->nbet; fp.ngam = c->ngam; fp.nphi = c->nphi;
-    fp.npack = c->npack; fp.NE = c->NE; fp.cscale = c->cscale;
-    fp.alp = c->d_alp; fp.bet = c->d_bet; fp.gam = c->d_gam; fp.Uf = c->d_Uf; fp.Vf = c->d_Vf; fp.Z = c->d_Z;
-    fp.rtab = c->d_rtab; fp.pk_a = c->d_pk_a; fp.pk_b = c->d_pk_b; fp.blk_off = c->d_blk_off;
-    fp.blk_sz = c->d_blk_sz; fp.blk_base = c->d_blk_base; fp.twom = c->d_twom;
+    if (!combine_attr(c->nphi, comb_lds)) return fail(c, PNAMP_E_SIZE, "factored form supports nphi <= 16");
+    const size_t setup_lds = sizeof(double2) * 3 * (size_t)hh;
+    SetupParams sp;
+    sp.n = c->n; sp.h = h; sp.nalp = c->nalp; sp.ngam = c->ngam; sp.npack = c->npack; sp.NE = c->NE;
+    sp.alp = c->d_alp; sp.gam = c->d_gam; sp.Uf = c->d_Uf; sp.Vf = c->d_Vf; sp.Z = c->d_Z; sp.rtab = c->d_rtab;
+    sp.pk_a = c->d_pk_a; sp.pk_b = c->d_pk_b; sp.blk_off = c->d_blk_off; sp.blk_sz = c->d_blk_sz;
+    sp.blk_base = c->d_blk_base; sp.twom = c->d_twom; sp.sbuf = c->d_sbuf;
+    PhiParams fp;
+    fp.n = c->n; fp.nphi = c->nphi; fp.NE = c->NE; fp.cscale = c->cscale; fp.sbuf = c->d_sbuf;
+    fp.UfT = c->d_UfT; fp.VfT = c->d_VfT; fp.Z = c->d_Z; fp.t1b = c->d_t1b; fp.mtab = c->d_mtab;
     fp.vl0 = c->d_vs[0]; fp.vl1 = c->d_vs[1]; fp.nme0 = c->nme_s[0]; fp.nme1 = c->nme_s[1];
-    fp.t1b = c->d_t1b; fp.jpack = c->d_jpack; fp.out = c->d_fout; fp.rho = c->d_frho; fp.status = c->d_fstatus;
+    fp.out = c->d_fout; fp.rho = c->d_frho; fp.status = c->d_fstatus;
     CombParams cp;
-    cp.h = h; cp.nphi = c->nphi; cp.NE = c->NE; cp.zval = zval; cp.nval = nval;
+    cp.h = h; cp.NE = c->NE; cp.zval = zval; cp.nval = nval;
     cp.vac_scale = c->vac_scale; cp.snpfaf = c->snpfaf; cp.sgn_h = (h & 1) ? -1.0 : 1.0; cp.e0 = c->e0;
     cp.det = c->d_det; cp.w_ptr = c->d_wptr; cp.w_col = c->d_wcol; cp.w_val = c->d_wval;
     cp.out = c->d_fout; cp.rho = c->d_frho; cp.fstatus = c->d_fstatus;
@@ -1705,12 +2059,16 @@ extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       for (int p0 = 0; p0 < npairs; p0 += chunk) {
         const int np = std::min(chunk, npairs - p0);
+        sp.q0 = c->d_q0 + p0; sp.q1 = c->d_q1 + p0;
         fp.q0 = c->d_q0 + p0; fp.q1 = c->d_q1 + p0;
         cp.q0 = c->d_q0 + p0; cp.q1 = c->d_q1 + p0;
         cp.zres = c->d_zres + (size_t)p0 * c->NE * NOBS;
         cp.status = c->d_status + (size_t)p0 * c->NE;
-        hipLaunchKernelGGL(k_fac_species, dim3(c->NE, np, 2), dim3(NTF), spec_lds, c->stream, fp);
-        hipLaunchKernelGGL(k_fac_combine, dim3(c->NE, np), dim3(NTC), comb_lds, c->stream, cp);
+        const size_t nwg = (size_t)np * c->NE * 2 * c->nphi;
+        fp.nwg = (int)nwg;
+        hipLaunchKernelGGL(k_fac_setup, dim3(c->NE, np, 2), dim3(NTS), setup_lds, c->stream, sp);
+        launch_phi_h(h, (unsigned)((nwg + 7) / 8 * 8), c->stream, fp);
+        launch_combine_np(c->nphi, dim3(c->NE, np), comb_lds, c->stream, cp);
       }
       const hipError_t cap = hipStreamEndCapture(c->stream, &graph);
       if (cap != hipSuccess) return fail(c, PNAMP_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(cap));
@@ -1857,8 +2215,8 @@ extern "C" void pnamp_destroy(pnamp_ctx *c) {
                   c->d_pk_a, c->d_pk_b, c->d_blk_off, c->d_blk_sz, c->d_blk_base, c->d_twoj, c->d_twom,
                   c->d_vlist, c->d_t1b, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status, c->d_q0,
                   c->d_q1, c->d_zres, c->d_kern, c->d_status, c->d_pair_status, c->d_vs[0], c->d_vs[1],
-                  c->d_vs[2], c->d_Uf, c->d_Vf, c->d_fout, c->d_frho, c->d_fstatus, c->d_wptr,
-                  c->d_wcol, c->d_wval, c->d_jpack};
+                  c->d_vs[2], c->d_Uf, c->d_Vf, c->d_UfT, c->d_VfT, c->d_mtab, c->d_fout, c->d_frho,
+                  c->d_sbuf, c->d_fstatus, c->d_wptr, c->d_wcol, c->d_wval, c->d_jpack};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto e : c->ev)
