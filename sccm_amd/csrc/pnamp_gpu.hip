@@ -845,55 +845,69 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Row p (wave-uniform) of a column held in registers, by masks: an indexed
-// read of a register array would be placed in scratch memory.
-template <int H>
-__device__ __forceinline__ double2 pick_row(const double2 (&col)[H], int p) {
-  unsigned long long x = 0ull, y = 0ull;
-#pragma unroll
-  for (int r = 0; r < H; ++r) {
-    const unsigned long long m = (p == r) ? ~0ull : 0ull;
-    x |= m & (unsigned long long)__double_as_longlong(col[r].x);
-    y |= m & (unsigned long long)__double_as_longlong(col[r].y);
-  }
-  return make_double2(__longlong_as_double((long long)x), __longlong_as_double((long long)y));
+template <int CTRL, int RM>
+__device__ __forceinline__ unsigned dpp_umax(unsigned v) {
+  return max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, RM, 0xF, false));
 }
 
-// Gauss-Jordan with partial pivoting (pivot = max |re| + |im| over the rows not
-// yet used, first maximum, as ZGETRF's IZAMAX; LAPACK-style INFO for an
-// exactly-zero pivot) on an H-row system held one column per lane of the
-// wavefront: col[r] is row r of this lane's column, lanes 0..H-1 hold the
-// coefficient matrix, the other lanes right-hand sides.  Rows are not
-// exchanged: logical row k of the solution is physical row lmap[k].  The
-// workgroup is this one wavefront, so __syncthreads() only orders LDS.
+// Maximum of v over the wavefront by DPP (quad exchanges, row mirrors, row
+// broadcasts: lane 63 ends with the maximum), returned wave-uniform.
+__device__ __forceinline__ unsigned wave_umax(unsigned v) {
+  v = dpp_umax<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+  v = dpp_umax<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+  v = dpp_umax<0x141, 0xF>(v);  // row_half_mirror
+  v = dpp_umax<0x140, 0xF>(v);  // row_mirror
+  v = dpp_umax<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v = dpp_umax<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Pivot key: a non-negative magnitude as FP32 bits with the low KB bits
+// replaced by (2^KB - 1 - idx), so one unsigned maximum over the wavefront
+// picks the largest magnitude (to 2^-(23-KB) relative) and, among equal keys,
+// the smallest index.  A NaN magnitude counts as zero.
+template <int KB>
+__device__ __forceinline__ unsigned pivot_key(double mag, int idx) {
+  const float f = (mag >= 0.0) ? (float)mag : 0.0f;
+  return (__float_as_uint(f) & ~((1u << KB) - 1u)) | ((1u << KB) - 1u - (unsigned)idx);
+}
+
+// Gauss-Jordan on an H-row system held one column per lane of the wavefront:
+// col[r] is row r of this lane's column, lanes 0..H-1 hold the coefficient
+// matrix M, the other lanes right-hand sides.  Step k pivots on row k and the
+// unused coefficient column q with the largest |M(k, q)| (|re| + |im|, ties to
+// the smallest q): for M = T^T this is the row pivoting of T by ZGETRF's
+// IZAMAX.  The row index is static, so nothing is picked out of registers by
+// a run-time index; column q is broadcast through LDS.  On return the unknown
+// q_k = qmap[k] is row k of each right-hand-side column; INFO as LAPACK for an
+// exactly-zero pivot.  The workgroup is this one wavefront, so
+// __syncthreads() only orders LDS.
 template <int H>
-__device__ int wave_gauss_jordan(double2 (&col)[H], double2 *fc, int *lmap) {
+__device__ int wave_gauss_jordan(double2 (&col)[H], double2 *fc, int *qmap) {
   const int lane = threadIdx.x;
   unsigned used = 0u;
   int info = 0;
+#pragma unroll
   for (int k = 0; k < H; ++k) {
-    if (lane == k) {
+    unsigned key = 0u;
+    if (lane < H && !((used >> lane) & 1u)) key = pivot_key<5>(cabs1(col[k]), lane);
+    key = wave_umax(key);
+    const int q = 31 - (int)(key & 31u);
+    used |= 1u << q;
+    if (lane == q) {
 #pragma unroll
       for (int r = 0; r < H; ++r) fc[r] = col[r];
+      if (k == 0) qmap[0] = q;
     }
+    if (lane == 0 && k > 0) qmap[k] = q;
     __syncthreads();
-    double a = -1.0;
-    if (lane < H && !((used >> lane) & 1u)) {
-      a = cabs1(fc[lane]);
-      if (!(a >= 0.0)) a = 0.0;  // NaN: keep the choice wave-uniform and among the unused rows
-    }
-    int pr = lane;
-    wave_best(a, pr);
-    if (lane == 0) lmap[k] = pr;
-    used |= 1u << pr;
-    const double2 pv = fc[pr];
+    const double2 pv = fc[k];
     if (pv.x == 0.0 && pv.y == 0.0 && info == 0) info = k + 1;
-    const double2 prow = cmul(pick_row<H>(col, pr), crecip(pv));
+    const double2 prow = cmul(col[k], crecip(pv));
 #pragma unroll
-    for (int r = 0; r < H; ++r) {
-      const double2 u = csub(col[r], cmul(fc[r], prow));
-      col[r] = (r == pr) ? prow : u;
-    }
+    for (int r = 0; r < H; ++r)
+      if (r != k) col[r] = csub(col[r], cmul(fc[r], prow));
+    col[k] = prow;
     __syncthreads();
   }
   return info;
@@ -935,13 +949,13 @@ __device__ int wave_gauss_jordan(double2 (&col)[H], double2 *fc, int *lmap) {
 //   = V [rho_ki rho_lj - rho_li rho_kj + A01_ij A10_kl / 4
 //        + d (rho_ik rho_jl - rho_jk rho_il + A01_kl A10_ij / 4)].
 template <int H>
-__global__ __launch_bounds__(64) void k_fac_phi(PhiParams p) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_fac_phi(PhiParams p) {
   constexpr int N2 = 2 * H, NU = N2 * (N2 - 1) / 2, SL = (NU + 63) / 64, HH = H * H;
   constexpr int PASSES = (3 * H <= 64) ? 1 : 2;
   __shared__ double2 sA[3 * HH];  // rho, kappa10, kappa01; Gauss-Jordan un-permute buffer before
   __shared__ double2 colR[N2], colA[N2], colC[N2];
   __shared__ double2 fc[H];
-  __shared__ int lmap[H];
+  __shared__ int qmap[H];
   const int lane = threadIdx.x;
   // XCD-aware order: consecutive logical indices (the nphi waves of one setup
   // block) stay on one XCD and its L2; gridDim.x is a multiple of 8.
@@ -990,25 +1004,16 @@ __global__ __launch_bounds__(64) void k_fac_phi(PhiParams p) {
     for (int k = 0; k < H - 1; ++k) {
       const int r = 2 * k, m = N2 - r;
       const int cnt = m * (m - 1) / 2, lo = (m - 1) * (m - 2) / 2, cnt2 = (m - 2) * (m - 3) / 2;
-      double bv = -1.0;
-      int bj = 1 << 20;
+      unsigned key = 0u;
 #pragma unroll
       for (int s = 0; s < SL; ++s) {
         if (s * 64 < cnt && (s + 1) * 64 > lo) {
           const int e = s * 64 + lane;
-          if (e >= lo && e < cnt) {
-            const double a = sv[s].x * sv[s].x + sv[s].y * sv[s].y;
-            const int j = sij[s] >> 8;
-            if (a > bv || (a == bv && j < bj)) { bv = a; bj = j; }
-          }
+          if (e >= lo && e < cnt) key = max(key, pivot_key<6>(sv[s].x * sv[s].x + sv[s].y * sv[s].y, sij[s] >> 8));
         }
       }
-      wave_best(bv, bj);
-      if (!(bv > 1.0e-26)) {  // |pivot| <= 1e-13
-        zero = true;
-        break;
-      }
-      const int c = bj;
+      key = wave_umax(key);
+      const int c = 63 - (int)(key & 63u);
 #pragma unroll
       for (int s = 0; s < SL; ++s) {
         if (s * 64 < cnt) {
@@ -1026,6 +1031,10 @@ __global__ __launch_bounds__(64) void k_fac_phi(PhiParams p) {
       }
       __syncthreads();
       const double2 spiv = cneg(colR[c]);  // S(r, c)
+      if (!(spiv.x * spiv.x + spiv.y * spiv.y > 1.0e-26)) {  // |pivot| <= 1e-13
+        zero = true;
+        break;
+      }
       const double2 inv = crecip(spiv);
       pf = cmul(pf, (c != r + 1) ? cneg(spiv) : spiv);
       // trailing entries i >= r + 2 after exchanging c <-> r + 1:
@@ -1078,20 +1087,20 @@ __global__ __launch_bounds__(64) void k_fac_phi(PhiParams p) {
 #pragma unroll
       for (int r = 0; r < H; ++r) col[r] = cz();
     }
-    const int inf = wave_gauss_jordan<H>(col, fc, lmap);
+    const int inf = wave_gauss_jordan<H>(col, fc, qmap);
     if (inf && !info) info = inf;
-    // solution rows back to logical order through LDS
+    // solution rows in unknown order through LDS: unknown qmap[k] is row k
     constexpr int NB = (PASSES == 1) ? 2 * H : H;
     double2 *ub = (PASSES == 1) ? sA + HH : sA + 2 * HH;
     const bool blane = lane >= H && lane < H + NB;
     if (blane) {
 #pragma unroll
-      for (int r = 0; r < H; ++r) ub[(lane - H) * H + r] = col[r];
+      for (int r = 0; r < H; ++r) ub[(lane - H) * H + qmap[r]] = col[r];
     }
     __syncthreads();
     double2 w[H];
 #pragma unroll
-    for (int k = 0; k < H; ++k) w[k] = blane ? ub[(lane - H) * H + lmap[k]] : cz();
+    for (int k = 0; k < H; ++k) w[k] = blane ? ub[(lane - H) * H + k] : cz();
     __syncthreads();
     const bool isW1 = w1lane && (PASSES == 1 || pass == 0);
     const bool isW2 = (PASSES == 1) ? w2lane1 : (pass == 1 && w1lane);
