@@ -93,19 +93,38 @@ def cpu_baseline(w, threads: int):
 
 
 def gcm_compare(ctx, w, kern_a, kern_b, nlevels: int):
+    """Hill-Wheeler energies of two kernel sets: the largest |dE| over the
+    lowest ``nlevels`` levels of every J, the levels of ``kern_b`` and the
+    per-level |dE| (MeV)."""
     import numpy as np
 
     from sccm_amd.driver import gcm_energies
     ea = gcm_energies(ctx, kern_a, w.pairs, len(w.U), w.mesh.jmax, w.hw_eps, nlevels=nlevels)
     eb = gcm_energies(ctx, kern_b, w.pairs, len(w.U), w.mesh.jmax, w.hw_eps, nlevels=nlevels)
-    worst, levels = 0.0, {}
+    worst, levels, dlev = 0.0, {}, {}
     for J in ea:
         k = min(len(ea[J]), len(eb[J]))
         if k:
-            d = float(np.abs(ea[J][:k] - eb[J][:k]).max())
-            worst = max(worst, d)
+            d = np.abs(ea[J][:k] - eb[J][:k])
+            worst = max(worst, float(d.max()))
             levels[str(J)] = [round(float(x), 6) for x in eb[J][:k]]
-    return worst, levels
+            dlev[str(J)] = [float(x) for x in d]
+    return worst, levels, dlev
+
+
+def kernel_errors(kern_a, kern_b, jmax: int):
+    """Per J: the largest |a - b| / max|b| of the norm (N^J) and energy (H^J)
+    kernels over all pairs."""
+    import numpy as np
+    out = {}
+    for J in range(jmax + 1):
+        e = {}
+        for name, io in (("N", 0), ("H", 4)):
+            b = kern_b[:, io, J]
+            nb = np.abs(b).max()
+            e[name] = float(np.abs(kern_a[:, io, J] - b).max() / nb) if nb > 0 else 0.0
+        out[str(J)] = e
+    return out
 
 
 def main():
@@ -115,7 +134,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--pairs-per-step", type=int, default=0, help="0: all pairs")
     ap.add_argument("--nstates", type=int, default=20)
-    ap.add_argument("--form", choices=("factored", "plain"), default="factored")
+    ap.add_argument("--form", choices=("factored", "mixed-fp32", "mixed-bf16", "plain"), default="factored")
     ap.add_argument("--levels", type=int, default=3, help="lowest HW levels per J compared")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain reference pass and the GCM check")
@@ -129,13 +148,14 @@ def main():
     import numpy as np
 
     from sccm_amd import configs
-    from sccm_amd.pnamp import FORM_FACTORED, FORM_PLAIN, PNAMP
+    from sccm_amd.pnamp import FORM_FACTORED, FORM_MIXED_BF16, FORM_MIXED_FP32, FORM_PLAIN, PNAMP
 
     log("bench: building the pf GCM workload")
     w = configs.pf_gcm(nstates=args.nstates)
     ctx = PNAMP(w.basis, w.inter, w.mesh)
     ctx.set_states(w.U, w.V)
-    form = FORM_FACTORED if args.form == "factored" else FORM_PLAIN
+    form = {"factored": FORM_FACTORED, "mixed-fp32": FORM_MIXED_FP32, "mixed-bf16": FORM_MIXED_BF16,
+            "plain": FORM_PLAIN}[args.form]
     ctx.set_form(form)
     npairs = len(w.pairs)
     P = args.pairs_per_step or npairs
@@ -162,7 +182,7 @@ def main():
     value = args.steps * pts_step / elapsed
     k_avg = k_ms / args.steps
     h = w.n // 2
-    if form == FORM_FACTORED:
+    if form != FORM_PLAIN:
         sum_b2 = sum(s * s for (_, s, _) in w.basis.blocks()) // 2
         idx = np.stack([w.inter.ka, w.inter.kb, w.inter.kc, w.inter.kd], 1) - 1
         same = ((idx < h).all(1)) | ((idx >= h).all(1))
@@ -177,7 +197,10 @@ def main():
             if a + b != c + d:
                 nz.add((a + c * h, bl + dl * h))
         fl = factored_flops(h, sum_b2, w.mesh.nphi, l_sp, l_cr, len(nz)) * w.mesh.n_euler * P
-        kname, tform = "k_fac_setup + k_fac_phi + k_fac_combine", "factored_fp64"
+        kname = "k_fac_setup + k_fac_phi + k_fac_combine"
+        tform = {FORM_FACTORED: "factored_fp64", FORM_MIXED_FP32: "mixed_fp32", FORM_MIXED_BF16: "mixed_bf16"}[form]
+        if form != FORM_FACTORED:
+            kname += " + k_vgemm"
     else:
         fl = plain_flops_per_point(w.n, w.inter.nme) * pts_step
         kname, tform = "k_point", "plain_fp64"
@@ -193,7 +216,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {FORM_MIXED_FP32: "f64+f32(mfma)", FORM_MIXED_BF16: "f64+bf16x3(mfma)"}.get(form, "f64"),
         "data": "synthetic (seeded pf-space interaction and HFB states)",
         "config": {
             "workload": "pf_gcm: pf space n=40, Z,N=2,4, 20 HFB states / 210 pairs, "
@@ -214,7 +237,7 @@ def main():
             "kernel_ms_per_launch": k_avg,
         },
     }
-    if form == FORM_FACTORED and not args.no_plain:
+    if form != FORM_PLAIN and not args.no_plain:
         log("bench: plain FP64 reference pass over all pairs")
         ctx.set_form(FORM_PLAIN)
         t = time.perf_counter()
@@ -228,10 +251,12 @@ def main():
             ctx.set_form(form)
             kern_chosen = ctx.kernels_batched(w.pairs, w.zval, w.nval)
         rel = float(np.abs(kern_chosen - kern_plain).max() / np.abs(kern_plain).max())
-        dE, levels = gcm_compare(ctx, w, kern_chosen, kern_plain, args.levels)
+        dE, levels, dlev = gcm_compare(ctx, w, kern_chosen, kern_plain, args.levels)
         out["gcm"] = {"max_abs_dE_keV": 1e3 * dE, "levels_compared_per_J": args.levels,
                       "hw_norm_cutoff": w.hw_eps, "max_rel_kernel_diff": rel,
-                      "energies_plain_MeV": levels}
+                      "energies_plain_MeV": levels,
+                      "abs_dE_keV_per_level": {J: [round(1e3 * x, 9) for x in v] for J, v in dlev.items()},
+                      "kernel_rel_err_per_J": kernel_errors(kern_chosen, kern_plain, w.mesh.jmax)}
     if not args.no_cpu:
         threads = min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 16)
         log(f"bench: CPU oracle sample on {threads} threads")

@@ -59,8 +59,15 @@ enum { PNAMP_FOLD_PLAIN = 0, PNAMP_FOLD_LITERAL = 1 };
  * FACTORED  FP64, for states without proton-neutron mixing: per Euler point
  *           nphi proton and nphi neutron Pfaffians / density sets (h = n/2),
  *           the proton-neutron energy for all nphi^2 gauge pairs from the pn
- *           part of the interaction list.  Same results as PLAIN to rounding. */
-enum { PNAMP_FORM_PLAIN = 0, PNAMP_FORM_FACTORED = 1 };
+ *           part of the interaction list.  Same results as PLAIN to rounding.
+ * MIXED_FP32, MIXED_BF16  the FACTORED form with the two-body contraction
+ *           (V times densities) as dense (h^2 x h^2) x (h^2 x columns) GEMMs
+ *           on MFMA over all gauge points of a batch: FP32, or split BF16
+ *           (hi + lo, three products, FP32 accumulation).  Pfaffian, the
+ *           inversion of T, traces and <J^2> stay FP64; the GEMM output is
+ *           dotted with FP64 densities.  Precision form of the field
+ *           contraction HFB_FIELD_COMPLEX (Codes/GCM/VAPHFB_HFB_Field.f90:4-123). */
+enum { PNAMP_FORM_PLAIN = 0, PNAMP_FORM_FACTORED = 1, PNAMP_FORM_MIXED_FP32 = 2, PNAMP_FORM_MIXED_BF16 = 3 };
 
 typedef struct pnamp_ctx pnamp_ctx;
 
@@ -104,8 +111,8 @@ int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *inter, const
 int pnamp_set_states(pnamp_ctx *ctx, int nstates, const double *U, const double *V);
 
 /* Selects the evaluation form (default PNAMP_FORM_PLAIN).  No counterpart in
- * the reference, which has one form; callers choosing FACTORED must set
- * states for which pnamp_states_separable returns 1. */
+ * the reference, which has one form; callers choosing FACTORED or a MIXED
+ * form must set states for which pnamp_states_separable returns 1. */
 int pnamp_set_form(pnamp_ctx *ctx, int form);
 
 /* 1 if every state set has no proton-neutron mixing (U, V block-diagonal by

@@ -751,7 +751,8 @@ __global__ __launch_bounds__(NT) void k_point(PointParams p) {
 constexpr int NTS = 256;     // threads of the setup workgroup
 constexpr int NTC = 288;     // threads of the combine workgroup (9 groups of 32)
 constexpr int NPHI_MAX = 16;
-constexpr int FAC_OUT = 8;   // per (pair, Euler, species, phi): Pf, E2, Ekin, Tr rho, A, t_+, t_-, t_z
+constexpr int FAC_OUT = 9;   // per (pair, Euler, species, phi): Pf, E2, Ekin, Tr rho, A, t_+, t_-, t_z, E2 (kappa part, mixed forms)
+constexpr int MIX_FP64 = 0, MIX_FP32 = 1, MIX_BF16 = 2;  // precision of the V contraction
 
 struct SetupParams {
   int n, h, nalp, ngam, npack, NE;
@@ -838,7 +839,52 @@ struct PhiParams {
   double2 *out;                     // [npairs][NE][2][nphi][FAC_OUT]
   double2 *rho;                     // [npairs][NE][2][nphi][h*h], column-major
   int *status;                      // [npairs][NE][2][nphi]
+  // mixed forms: GEMM operands (fragment-ordered, see put_b) per species and
+  // the FP64 A01 = kappa01 - kappa01^T dot vectors [npairs][NE][2][nphi][h*h]
+  void *brho[2], *brho_lo[2], *ba10[2], *ba10_lo[2];
+  double2 *a01;
 };
+
+// Fragment order of the V-contraction operands (k_vgemm): element (k, col)
+// of a K x ncols operand, 32 columns per tile.  FP32 (v_mfma_f32_32x32x2_f32):
+// per k-octet, lane l = (col % 32) + 32 (k % 2) holds k = 8 kq + 2 e + (k % 2),
+// e = 0..3, as one 16-byte load.  BF16 (v_mfma_f32_32x32x16_bf16): per
+// 16-k block, lane l = (col % 32) + 32 ((k / 8) % 2) holds 8 consecutive k.
+template <int KP>
+__device__ __forceinline__ size_t swz_f32(int k, size_t col) {
+  return ((((col >> 5) * (KP / 8) + (size_t)(k >> 3)) * 64 + (col & 31) + 32 * (k & 1)) << 2) + ((k >> 1) & 3);
+}
+template <int KP>
+__device__ __forceinline__ size_t swz_bf16(int k, size_t col) {
+  return ((((col >> 5) * (KP / 16) + (size_t)(k >> 4)) * 64 + (col & 31) + 32 * ((k >> 3) & 1)) << 3) + (k & 7);
+}
+
+// bfloat16 by round-to-nearest-even of the FP32 bits (finite inputs)
+__host__ __device__ __forceinline__ unsigned short bf16_rne(float f) {
+  unsigned u;
+  __builtin_memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__host__ __device__ __forceinline__ float bf16_float(unsigned short b) {
+  const unsigned u = (unsigned)b << 16;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+
+// One operand element: FP32, or split BF16 hi + lo with lo = bf16(v - hi).
+template <int MIX, int KP>
+__device__ __forceinline__ void put_b(void *hi, void *lo, int k, size_t col, double v) {
+  if (MIX == MIX_FP32) {
+    ((float *)hi)[swz_f32<KP>(k, col)] = (float)v;
+  } else {
+    const unsigned short bh = bf16_rne((float)v);
+    const size_t i = swz_bf16<KP>(k, col);
+    ((unsigned short *)hi)[i] = bh;
+    ((unsigned short *)lo)[i] = bf16_rne((float)(v - (double)bf16_float(bh)));
+  }
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -948,9 +994,10 @@ __device__ int wave_gauss_jordan(double2 (&col)[H], double2 *fc, int *qmap) {
 //   (1/4) V [r_ijkl - r_jikl - r_ijlk + r_jilk + d (r_klij - r_klji - r_lkij + r_lkji)]
 //   = V [rho_ki rho_lj - rho_li rho_kj + A01_ij A10_kl / 4
 //        + d (rho_ik rho_jl - rho_jk rho_il + A01_kl A10_ij / 4)].
-template <int H>
+template <int H, int MIX>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_fac_phi(PhiParams p) {
   constexpr int N2 = 2 * H, NU = N2 * (N2 - 1) / 2, SL = (NU + 63) / 64, HH = H * H;
+  constexpr int KP = (HH + 31) / 32 * 32;  // V-contraction K padded to the MFMA tile
   constexpr int PASSES = (3 * H <= 64) ? 1 : 2;
   __shared__ double2 sA[3 * HH];  // rho, kappa10, kappa01; Gauss-Jordan un-permute buffer before
   __shared__ double2 colR[N2], colA[N2], colC[N2];
@@ -1070,6 +1117,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   const bool w2lane1 = lane >= 2 * H && lane < 3 * H;  // PASSES == 1 only
   const int irow = w2lane1 ? lane - 2 * H : lane - H;
   double2 *rho_out = p.rho + obase * HH;
+  const size_t mcol = ((size_t)ip * p.NE + iang) * nphi + f;  // GEMM column pair of this phi in species sp
 #pragma unroll 1
   for (int pass = 0; pass < PASSES; ++pass) {
     double2 col[H];
@@ -1112,6 +1160,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
       if (isW1) {
         sA[irow + j * H] = acc;
         rho_out[irow + j * H] = acc;
+        if (MIX != MIX_FP64) {
+          put_b<MIX, KP>(p.brho[sp], p.brho_lo[sp], irow + j * H, 2 * mcol, acc.x);
+          put_b<MIX, KP>(p.brho[sp], p.brho_lo[sp], irow + j * H, 2 * mcol + 1, acc.y);
+        }
       }
       if (isW2) sA[2 * HH + irow + j * H] = cneg(acc);
     }
@@ -1168,11 +1220,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     }
   }
   __syncthreads();
+  double2 e2b = cz();
+  const double2 *A10 = k10, *A01 = k01;
+  if (MIX != MIX_FP64) {
+    // the species list moves to k_vgemm: A01 (dot vector) and A10 (operand)
+    double2 *a01 = p.a01 + obase * HH;
+    for (int e = lane; e < HH; e += 64) {
+      a01[e] = A01[e];
+      put_b<MIX, KP>(p.ba10[sp], p.ba10_lo[sp], e, 2 * mcol, A10[e].x);
+      put_b<MIX, KP>(p.ba10[sp], p.ba10_lo[sp], e, 2 * mcol + 1, A10[e].y);
+    }
+  }
   // ---- species list
   const VEntry *vl = sp ? p.vl1 : p.vl0;
-  const int nme = sp ? p.nme1 : p.nme0;
-  const double2 *A10 = k10, *A01 = k01;
-  double2 e2b = cz();
+  const int nme = (MIX == MIX_FP64) ? (sp ? p.nme1 : p.nme0) : 0;
   for (int q = lane; q < nme; q += 64) {
     const VEntry ve = vl[q];
     const int i = ve.idx & 255, j = (ve.idx >> 8) & 255, k = (ve.idx >> 16) & 255, l = ve.idx >> 24;
@@ -1192,6 +1253,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     o[0] = pf;
 #pragma unroll
     for (int q = 0; q < 7; ++q) o[1 + q] = make_double2(v[2 * q], v[2 * q + 1]);
+    o[8] = cz();
     p.status[obase] = info;
   }
 }
@@ -1205,6 +1267,7 @@ struct CombParams {
   const double *w_val;
   const double2 *out, *rho;
   const int *fstatus;               // [npairs][NE][2][nphi]
+  const float2 *gpn;                // mixed forms: G = W_pn rho_n from k_vgemm, [npairs][NE][nphi][h*h]
   double2 *zres;                    // [npairs][NE][NOBS]
   int *status;                      // [npairs][NE]
 };
@@ -1217,8 +1280,8 @@ struct CombParams {
 // G = W rho_n(phi_n) is formed once per phi_n from the CSR rows of W, then
 // the nphi^2 dot products with rho_p(phi_p); the gauge points are summed into
 // ZRES.  NP = nphi is a template argument so the per-gauge register arrays
-// stay in registers.
-template <int NP>
+// stay in registers.  With GMIX (mixed forms) G comes from k_vgemm instead.
+template <int NP, bool GMIX>
 __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
   extern __shared__ double2 dsm[];  // rho_p [NP][h^2], rho_n -> G [NP][h^2]
   __shared__ double2 epn[NP * NP];
@@ -1230,9 +1293,15 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
   double2 *rp = dsm, *rn = dsm + (size_t)NP * hh;
   for (int e = tid; e < NP * hh; e += NTC) {
     rp[e] = p.rho[base * NP * hh + e];
-    rn[e] = p.rho[(base + 1) * NP * hh + e];
+    if (GMIX) {
+      const float2 gv = p.gpn[((size_t)ip * p.NE + iang) * NP * hh + e];
+      rn[e] = make_double2(gv.x, gv.y);
+    } else {
+      rn[e] = p.rho[(base + 1) * NP * hh + e];
+    }
   }
   __syncthreads();
+  if (!GMIX) {
   // G rows x = tid, tid + NTC (h^2 <= 2 NTC), all phi_n, in registers
   constexpr int RPT = 2;
   double2 g[RPT][NP];
@@ -1260,6 +1329,7 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
     }
   }
   __syncthreads();
+  }
   const int grp = tid >> 5, gl = tid & 31;
   for (int fp = grp; fp < NP; fp += NTC / 32) {
     double2 acc[NP];
@@ -1292,7 +1362,8 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
     const double2 *a = op + fp * FAC_OUT, *b = on + fn * FAC_OUT;
     const double2 zn = cmul(cscale(vac, cmul(a[0], b[0])), cexpi(-phip * p.zval - phin * p.nval));
     const double2 ekin = cadd(a[2], b[2]);
-    const double2 E = cadd(cadd(cadd(a[1], b[1]), epn[fp * NP + fn]), cadd(ekin, make_double2(p.e0, 0.0)));
+    const double2 e2s = cadd(cadd(a[1], b[1]), cadd(a[8], b[8]));
+    const double2 E = cadd(cadd(e2s, epn[fp * NP + fn]), cadd(ekin, make_double2(p.e0, 0.0)));
     // <J^2> = A_p + A_n + (t_zp + t_zn)^2 + (t_+p + t_+n)(t_-p + t_-n)
     const double2 tzs = cadd(a[7], b[7]);
     const double2 J2 = cadd(cadd(cadd(a[4], b[4]), cmul(tzs, tzs)), cmul(cadd(a[5], b[5]), cadd(a[6], b[6])));
@@ -1326,78 +1397,231 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
   }
 }
 
+// ------------------------------------------------------------------ V contraction on MFMA (mixed forms)
+// The quadratic forms of the V list as dense h^2 x h^2 matrices applied to
+// all density columns of a chunk at once, as in HFB_FIELD_COMPLEX
+// (Codes/GCM/VAPHFB_HFB_Field.f90:4-123) with E = Tr(t rho) + Tr(Gamma rho) / 2
+// + Tr(Delta10 kappa01^T) / 2 (Codes/PNAMP/PNAMP_HFB_Energy.f90:1-35):
+//   E_rho(s)   = rho_s^T (W_ss rho_s),  E_kappa(s) = A01_s^T (Q_ss A10_s),
+//   G_pn       = W_pn rho_n, dotted with rho_p(phi_p) by k_fac_combine,
+// W and Q built from the species lists so that they reproduce the eight-term
+// sum of HFB_Energy_COMPLEX exactly (see pnamp_create).  C = A B with A = W or
+// Q (KP x KP, fragment order), B = density columns (Re and Im of each phi as
+// two columns).  A workgroup of four waves covers 128 columns, a wave 32
+// columns and all KP / 32 row blocks.  The epilogue dots C with the FP64
+// vector of its column pair (rho or A01) and writes one complex number per
+// phi, or stores G in FP32.  FP32: v_mfma_f32_32x32x2_f32.  BF16: split
+// operands x = hi + lo, three products hi*hi + hi*lo + lo*hi per
+// v_mfma_f32_32x32x16_bf16, FP32 accumulation.
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+struct VProblem {
+  const void *ahi, *alo, *bhi, *blo;
+  const double2 *dot;               // mode 0: [npairs][NE][2][nphi][h*h] (rho or A01)
+  float *g;                         // mode 1: [npairs][NE][nphi][h*h] complex FP32
+  int mode, species, slot, ncols;
+};
+constexpr int NVPROB = 5;           // W_pp rho_p, W_nn rho_n, Q_pp A10_p, Q_nn A10_n, W_pn rho_n
+struct VGemmParams {
+  VProblem prob[NVPROB];
+  int h, nphi;
+  double2 *out;                     // [npairs][NE][2][nphi][FAC_OUT]
+};
+
+template <int KP, int MIX>
+__global__ __launch_bounds__(256) void k_vgemm(VGemmParams p) {
+  constexpr int RB = KP / 32;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const VProblem &P = p.prob[blockIdx.y];
+  const int ct = blockIdx.x * 4 + wave;
+  if (ct * 32 >= P.ncols) return;
+  f32x16 acc[RB];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[rb][q] = 0.0f;
+  if (MIX == MIX_FP32) {
+    constexpr int KQ = KP / 8;
+    const float4 *A = (const float4 *)P.ahi;
+    const float4 *B = (const float4 *)P.bhi + (size_t)ct * KQ * 64;
+    for (int kq = 0; kq < KQ; ++kq) {
+      const float4 b = B[kq * 64 + lane];
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) {
+        const float4 a = A[((size_t)rb * KQ + kq) * 64 + lane];
+        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.x, acc[rb], 0, 0, 0);
+        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b.y, acc[rb], 0, 0, 0);
+        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, b.z, acc[rb], 0, 0, 0);
+        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, b.w, acc[rb], 0, 0, 0);
+      }
+    }
+  } else {
+    constexpr int KB = KP / 16;
+    const bf16x8 *Ah = (const bf16x8 *)P.ahi, *Al = (const bf16x8 *)P.alo;
+    const bf16x8 *Bh = (const bf16x8 *)P.bhi + (size_t)ct * KB * 64;
+    const bf16x8 *Bl = (const bf16x8 *)P.blo + (size_t)ct * KB * 64;
+    for (int kb = 0; kb < KB; ++kb) {
+      const bf16x8 bh = Bh[kb * 64 + lane], bl = Bl[kb * 64 + lane];
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) {
+        const size_t ia = ((size_t)rb * KB + kb) * 64 + lane;
+        const bf16x8 ah = Ah[ia], al = Al[ia];
+        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[rb], 0, 0, 0);
+        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[rb], 0, 0, 0);
+        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[rb], 0, 0, 0);
+      }
+    }
+  }
+  // C(row, col) = acc[rb][q], row = 32 rb + 8 (q / 4) + 4 (lane / 32) + q % 4, col = 32 ct + lane % 32
+  const int hh = p.h * p.h;
+  const size_t col = (size_t)ct * 32 + (lane & 31);
+  const bool cv = col < (size_t)P.ncols;
+  const size_t m = col >> 1;  // phi column pair: m = (pair * NE + Euler) * nphi + phi
+  const int part = (int)(col & 1);
+  if (P.mode == 0) {
+    const size_t ob = ((m / p.nphi) * 2 + P.species) * p.nphi + (m % p.nphi);
+    double sr = 0.0, si = 0.0;
+    if (cv) {
+      const double2 *d = P.dot + ob * hh;
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const int row = 32 * rb + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+          if (row < hh) {
+            const double g = (double)acc[rb][q];
+            const double2 dv = d[row];
+            if (part == 0) {
+              sr += dv.x * g;
+              si += dv.y * g;
+            } else {  // i G_im
+              sr -= dv.y * g;
+              si += dv.x * g;
+            }
+          }
+        }
+    }
+    sr += __shfl_xor(sr, 32);
+    si += __shfl_xor(si, 32);
+    sr += __shfl_xor(sr, 1);
+    si += __shfl_xor(si, 1);
+    if (cv && (lane & 33) == 0) p.out[ob * FAC_OUT + P.slot] = make_double2(sr, si);
+  } else if (cv) {
+    float *g = P.g + m * (size_t)hh * 2 + part;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int row = 32 * rb + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+        if (row < hh) g[(size_t)row * 2] = acc[rb][q];
+      }
+  }
+}
+
+template <int KP>
+static void launch_vgemm_kp(int mix, dim3 grid, hipStream_t s, const VGemmParams &vp) {
+  if (mix == MIX_FP32) hipLaunchKernelGGL((k_vgemm<KP, MIX_FP32>), grid, dim3(256), 0, s, vp);
+  else hipLaunchKernelGGL((k_vgemm<KP, MIX_BF16>), grid, dim3(256), 0, s, vp);
+}
+
+// KP = h^2 rounded up to 32, for the instantiated species sizes
+static bool launch_vgemm(int h, int mix, dim3 grid, hipStream_t s, const VGemmParams &vp) {
+  switch (h) {
+    case 2: case 4: launch_vgemm_kp<32>(mix, grid, s, vp); return true;
+    case 6: launch_vgemm_kp<64>(mix, grid, s, vp); return true;
+    case 8: launch_vgemm_kp<64>(mix, grid, s, vp); return true;
+    case 10: launch_vgemm_kp<128>(mix, grid, s, vp); return true;
+    case 12: launch_vgemm_kp<160>(mix, grid, s, vp); return true;
+    case 14: launch_vgemm_kp<224>(mix, grid, s, vp); return true;
+    case 16: launch_vgemm_kp<256>(mix, grid, s, vp); return true;
+    case 18: launch_vgemm_kp<352>(mix, grid, s, vp); return true;
+    case 20: launch_vgemm_kp<416>(mix, grid, s, vp); return true;
+    case 22: launch_vgemm_kp<512>(mix, grid, s, vp); return true;
+    default: return false;
+  }
+}
+
 template <int H>
-static void launch_phi(unsigned grid, hipStream_t s, const PhiParams &pp) {
-  hipLaunchKernelGGL(k_fac_phi<H>, dim3(grid), dim3(64), 0, s, pp);
+static void launch_phi(int mix, unsigned grid, hipStream_t s, const PhiParams &pp) {
+  if (mix == MIX_FP32) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP32>), dim3(grid), dim3(64), 0, s, pp);
+  else if (mix == MIX_BF16) hipLaunchKernelGGL((k_fac_phi<H, MIX_BF16>), dim3(grid), dim3(64), 0, s, pp);
+  else hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64>), dim3(grid), dim3(64), 0, s, pp);
 }
 
 // Species sizes h = n / 2 with an instantiated k_fac_phi (every h is even:
 // an orbit holds 2j + 1 states).
-static bool launch_phi_h(int h, unsigned grid, hipStream_t s, const PhiParams &pp) {
+static bool launch_phi_h(int h, int mix, unsigned grid, hipStream_t s, const PhiParams &pp) {
   switch (h) {
-    case 2: launch_phi<2>(grid, s, pp); return true;
-    case 4: launch_phi<4>(grid, s, pp); return true;
-    case 6: launch_phi<6>(grid, s, pp); return true;
-    case 8: launch_phi<8>(grid, s, pp); return true;
-    case 10: launch_phi<10>(grid, s, pp); return true;
-    case 12: launch_phi<12>(grid, s, pp); return true;
-    case 14: launch_phi<14>(grid, s, pp); return true;
-    case 16: launch_phi<16>(grid, s, pp); return true;
-    case 18: launch_phi<18>(grid, s, pp); return true;
-    case 20: launch_phi<20>(grid, s, pp); return true;
-    case 22: launch_phi<22>(grid, s, pp); return true;
+    case 2: launch_phi<2>(mix, grid, s, pp); return true;
+    case 4: launch_phi<4>(mix, grid, s, pp); return true;
+    case 6: launch_phi<6>(mix, grid, s, pp); return true;
+    case 8: launch_phi<8>(mix, grid, s, pp); return true;
+    case 10: launch_phi<10>(mix, grid, s, pp); return true;
+    case 12: launch_phi<12>(mix, grid, s, pp); return true;
+    case 14: launch_phi<14>(mix, grid, s, pp); return true;
+    case 16: launch_phi<16>(mix, grid, s, pp); return true;
+    case 18: launch_phi<18>(mix, grid, s, pp); return true;
+    case 20: launch_phi<20>(mix, grid, s, pp); return true;
+    case 22: launch_phi<22>(mix, grid, s, pp); return true;
     default: return false;
   }
 }
 
 template <int NP>
-static void launch_combine(dim3 grid, size_t lds, hipStream_t s, const CombParams &cp) {
-  hipLaunchKernelGGL(k_fac_combine<NP>, grid, dim3(NTC), lds, s, cp);
+static void launch_combine(bool gmix, dim3 grid, size_t lds, hipStream_t s, const CombParams &cp) {
+  if (gmix) hipLaunchKernelGGL((k_fac_combine<NP, true>), grid, dim3(NTC), lds, s, cp);
+  else hipLaunchKernelGGL((k_fac_combine<NP, false>), grid, dim3(NTC), lds, s, cp);
 }
 
-// Dynamic LDS limit of k_fac_combine<np>, set before any capture.
+// Dynamic LDS limit of k_fac_combine<np, *>, set before any capture.
+template <int NP>
+static bool combine_attr_np(size_t lds) {
+  return hipFuncSetAttribute((const void *)k_fac_combine<NP, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds) == hipSuccess &&
+         hipFuncSetAttribute((const void *)k_fac_combine<NP, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds) == hipSuccess;
+}
 static bool combine_attr(int np, size_t lds) {
-  const void *fn = nullptr;
   switch (np) {
-    case 1: fn = (const void *)k_fac_combine<1>; break;
-    case 2: fn = (const void *)k_fac_combine<2>; break;
-    case 3: fn = (const void *)k_fac_combine<3>; break;
-    case 4: fn = (const void *)k_fac_combine<4>; break;
-    case 5: fn = (const void *)k_fac_combine<5>; break;
-    case 6: fn = (const void *)k_fac_combine<6>; break;
-    case 7: fn = (const void *)k_fac_combine<7>; break;
-    case 8: fn = (const void *)k_fac_combine<8>; break;
-    case 9: fn = (const void *)k_fac_combine<9>; break;
-    case 10: fn = (const void *)k_fac_combine<10>; break;
-    case 11: fn = (const void *)k_fac_combine<11>; break;
-    case 12: fn = (const void *)k_fac_combine<12>; break;
-    case 13: fn = (const void *)k_fac_combine<13>; break;
-    case 14: fn = (const void *)k_fac_combine<14>; break;
-    case 15: fn = (const void *)k_fac_combine<15>; break;
-    case 16: fn = (const void *)k_fac_combine<16>; break;
+    case 1: return combine_attr_np<1>(lds);
+    case 2: return combine_attr_np<2>(lds);
+    case 3: return combine_attr_np<3>(lds);
+    case 4: return combine_attr_np<4>(lds);
+    case 5: return combine_attr_np<5>(lds);
+    case 6: return combine_attr_np<6>(lds);
+    case 7: return combine_attr_np<7>(lds);
+    case 8: return combine_attr_np<8>(lds);
+    case 9: return combine_attr_np<9>(lds);
+    case 10: return combine_attr_np<10>(lds);
+    case 11: return combine_attr_np<11>(lds);
+    case 12: return combine_attr_np<12>(lds);
+    case 13: return combine_attr_np<13>(lds);
+    case 14: return combine_attr_np<14>(lds);
+    case 15: return combine_attr_np<15>(lds);
+    case 16: return combine_attr_np<16>(lds);
     default: return false;
   }
-  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess;
 }
 
-static bool launch_combine_np(int np, dim3 grid, size_t lds, hipStream_t s, const CombParams &cp) {
+static bool launch_combine_np(int np, bool gmix, dim3 grid, size_t lds, hipStream_t s, const CombParams &cp) {
   switch (np) {
-    case 1: launch_combine<1>(grid, lds, s, cp); return true;
-    case 2: launch_combine<2>(grid, lds, s, cp); return true;
-    case 3: launch_combine<3>(grid, lds, s, cp); return true;
-    case 4: launch_combine<4>(grid, lds, s, cp); return true;
-    case 5: launch_combine<5>(grid, lds, s, cp); return true;
-    case 6: launch_combine<6>(grid, lds, s, cp); return true;
-    case 7: launch_combine<7>(grid, lds, s, cp); return true;
-    case 8: launch_combine<8>(grid, lds, s, cp); return true;
-    case 9: launch_combine<9>(grid, lds, s, cp); return true;
-    case 10: launch_combine<10>(grid, lds, s, cp); return true;
-    case 11: launch_combine<11>(grid, lds, s, cp); return true;
-    case 12: launch_combine<12>(grid, lds, s, cp); return true;
-    case 13: launch_combine<13>(grid, lds, s, cp); return true;
-    case 14: launch_combine<14>(grid, lds, s, cp); return true;
-    case 15: launch_combine<15>(grid, lds, s, cp); return true;
-    case 16: launch_combine<16>(grid, lds, s, cp); return true;
+    case 1: launch_combine<1>(gmix, grid, lds, s, cp); return true;
+    case 2: launch_combine<2>(gmix, grid, lds, s, cp); return true;
+    case 3: launch_combine<3>(gmix, grid, lds, s, cp); return true;
+    case 4: launch_combine<4>(gmix, grid, lds, s, cp); return true;
+    case 5: launch_combine<5>(gmix, grid, lds, s, cp); return true;
+    case 6: launch_combine<6>(gmix, grid, lds, s, cp); return true;
+    case 7: launch_combine<7>(gmix, grid, lds, s, cp); return true;
+    case 8: launch_combine<8>(gmix, grid, lds, s, cp); return true;
+    case 9: launch_combine<9>(gmix, grid, lds, s, cp); return true;
+    case 10: launch_combine<10>(gmix, grid, lds, s, cp); return true;
+    case 11: launch_combine<11>(gmix, grid, lds, s, cp); return true;
+    case 12: launch_combine<12>(gmix, grid, lds, s, cp); return true;
+    case 13: launch_combine<13>(gmix, grid, lds, s, cp); return true;
+    case 14: launch_combine<14>(gmix, grid, lds, s, cp); return true;
+    case 15: launch_combine<15>(gmix, grid, lds, s, cp); return true;
+    case 16: launch_combine<16>(gmix, grid, lds, s, cp); return true;
     default: return false;
   }
 }
@@ -1652,7 +1876,19 @@ struct pnamp_ctx {
   // captured launch sequence of the factored form, valid for (npairs, zval,
   // nval) and the current buffers; dropped whenever a buffer is reallocated
   hipGraphExec_t fac_graph = nullptr;
-  int g_np = -1, g_z = 0, g_n = 0;
+  int g_np = -1, g_z = 0, g_n = 0, g_form = -1;
+  // mixed forms: the dense V-contraction matrices (W_pp, W_nn, Q_pp, Q_nn,
+  // W_pn; kp x kp row-major FP64 on the host), their fragment-ordered device
+  // copies in the precision of va_mix, and the per-chunk GEMM operands
+  int kp = 0;
+  std::vector<double> vmat[NVPROB];
+  void *d_va[NVPROB] = {}, *d_va_lo[NVPROB] = {};
+  int va_mix = -1;
+  void *d_brho[2] = {}, *d_brho_lo[2] = {}, *d_ba10[2] = {}, *d_ba10_lo[2] = {};
+  double2 *d_a01 = nullptr;
+  float *d_gpn = nullptr;
+  size_t mix_cap = 0;  // pairs the mixed-form operands hold, in the precision mix_cap_mix
+  int mix_cap_mix = -1;
 };
 
 #define HIPCHK(ctx, x)                                                       \
@@ -1674,7 +1910,51 @@ static void drop_graph(pnamp_ctx *c) {
     (void)hipGraphExecDestroy(c->fac_graph);
     c->fac_graph = nullptr;
     c->g_np = -1;
+    c->g_form = -1;
   }
+}
+
+static void free_void(pnamp_ctx *c, void **p) {
+  drop_graph(c);
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+}
+
+// Fragment-ordered copy of a kp x kp row-major matrix for k_vgemm (A operand).
+static int upload_vmat(pnamp_ctx *c, int idx, int mix) {
+  const int kp = c->kp;
+  const std::vector<double> &A = c->vmat[idx];
+  free_void(c, &c->d_va[idx]);
+  free_void(c, &c->d_va_lo[idx]);
+  if (mix == MIX_FP32) {
+    const int kq_n = kp / 8;
+    std::vector<float> sw((size_t)kp * kp);
+    for (int rb = 0; rb < kp / 32; ++rb)
+      for (int kq = 0; kq < kq_n; ++kq)
+        for (int l = 0; l < 64; ++l)
+          for (int e = 0; e < 4; ++e)
+            sw[(((size_t)rb * kq_n + kq) * 64 + l) * 4 + e] =
+                (float)A[(size_t)(32 * rb + (l & 31)) * kp + (8 * kq + 2 * e + (l >> 5))];
+    HIPCHK(c, hipMalloc(&c->d_va[idx], sw.size() * sizeof(float)));
+    HIPCHK(c, hipMemcpy(c->d_va[idx], sw.data(), sw.size() * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    const int kb_n = kp / 16;
+    std::vector<unsigned short> hi((size_t)kp * kp), lo((size_t)kp * kp);
+    for (int rb = 0; rb < kp / 32; ++rb)
+      for (int kb = 0; kb < kb_n; ++kb)
+        for (int l = 0; l < 64; ++l)
+          for (int e = 0; e < 8; ++e) {
+            const size_t i = (((size_t)rb * kb_n + kb) * 64 + l) * 8 + e;
+            const double v = A[(size_t)(32 * rb + (l & 31)) * kp + (16 * kb + 8 * (l >> 5) + e)];
+            hi[i] = bf16_rne((float)v);
+            lo[i] = bf16_rne((float)(v - (double)bf16_float(hi[i])));
+          }
+    HIPCHK(c, hipMalloc(&c->d_va[idx], hi.size() * 2));
+    HIPCHK(c, hipMalloc(&c->d_va_lo[idx], lo.size() * 2));
+    HIPCHK(c, hipMemcpy(c->d_va[idx], hi.data(), hi.size() * 2, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_va_lo[idx], lo.data(), lo.size() * 2, hipMemcpyHostToDevice));
+  }
+  return 0;
 }
 
 template <class T>
@@ -1889,6 +2169,33 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
     if ((r = dupload(c, &c->d_wptr, wptr.data(), wptr.size())) || (r = dupload(c, &c->d_wcol, wcol.data(), wcol.size())) ||
         (r = dupload(c, &c->d_wval, wval.data(), wval.size())))
       return bail(r, c->err);
+    // dense forms of the mixed-precision contraction (rows x, columns y, x and
+    // y = species-local pair index c + a h of rho(c, a)): for each species entry
+    // (i j | V | k l) with d = [k + l != i + j]
+    //   W[(k,i),(l,j)] += V, W[(l,i),(k,j)] -= V, and with d W[(i,k),(j,l)] += V, W[(j,k),(i,l)] -= V,
+    //   Q[(i,j),(k,l)] += V/4, and with d Q[(k,l),(i,j)] += V/4,
+    // so that rho^T W rho + A01^T Q A10 is the regrouped eight-term sum; W_pn as the CSR above
+    const int kp = (hh + 31) / 32 * 32;
+    c->kp = kp;
+    for (auto &m : c->vmat) m.assign((size_t)kp * kp, 0.0);
+    auto at = [&](int x, int y) { return (size_t)x * kp + y; };
+    for (int sp = 0; sp < 2; ++sp) {
+      std::vector<double> &W = c->vmat[sp], &Q = c->vmat[2 + sp];
+      for (const VEntry &e : ls[sp]) {
+        const int i = e.idx & 255, j = (e.idx >> 8) & 255, k = (e.idx >> 16) & 255, l = e.idx >> 24;
+        const double v = e.me;
+        W[at(k + i * h, l + j * h)] += v;
+        W[at(l + i * h, k + j * h)] -= v;
+        Q[at(i + j * h, k + l * h)] += 0.25 * v;
+        if (k + l != i + j) {
+          W[at(i + k * h, j + l * h)] += v;
+          W[at(j + k * h, i + l * h)] -= v;
+          Q[at(k + l * h, i + j * h)] += 0.25 * v;
+        }
+      }
+    }
+    for (int x = 0; x < hh; ++x)
+      for (int q = wptr[x]; q < wptr[x + 1]; ++q) c->vmat[4][at(x, wcol[q])] = wval[q];
   }
   if ((r = dalloc(c, &c->d_rtab, (size_t)c->nbet * c->npack)) || (r = dalloc(c, &c->d_ptab, (size_t)c->nbet * 2 * c->npj)))
     return bail(r, c->err);
@@ -1971,7 +2278,7 @@ extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, cons
 
 extern "C" int pnamp_set_form(pnamp_ctx *c, int form) {
   if (!c) return PNAMP_E_ARG;
-  if (form != PNAMP_FORM_PLAIN && form != PNAMP_FORM_FACTORED) return fail(c, PNAMP_E_ARG, "unknown form");
+  if (form < PNAMP_FORM_PLAIN || form > PNAMP_FORM_MIXED_BF16) return fail(c, PNAMP_E_ARG, "unknown form");
   c->form = form;
   return PNAMP_OK;
 }
@@ -2020,26 +2327,54 @@ extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0
   pp.blk_off = c->d_blk_off; pp.blk_sz = c->d_blk_sz; pp.blk_base = c->d_blk_base; pp.twom = c->d_twom;
   pp.q0 = c->d_q0; pp.q1 = c->d_q1; pp.vlist = c->d_vlist; pp.t1b = c->d_t1b; pp.jpack = c->d_jpack;
   pp.zres = c->d_zres; pp.status = c->d_status;
-  if (c->form == PNAMP_FORM_FACTORED) {
+  if (c->form != PNAMP_FORM_PLAIN) {
     if (!c->separable)
-      return fail(c, PNAMP_E_ARG, "factored form needs states without proton-neutron mixing");
-    const int h = c->h, hh = h * h;
+      return fail(c, PNAMP_E_ARG, "factored and mixed forms need states without proton-neutron mixing");
+    const int mix = (c->form == PNAMP_FORM_FACTORED) ? MIX_FP64 : (c->form == PNAMP_FORM_MIXED_FP32 ? MIX_FP32 : MIX_BF16);
+    const int h = c->h, hh = h * h, kp = c->kp;
     if (c->nphi > NPHI_MAX) return fail(c, PNAMP_E_SIZE, "factored form supports nphi <= 16");
     const size_t comb_lds = sizeof(double2) * 2 * (size_t)c->nphi * hh;
     if (comb_lds > 150 * 1024) return fail(c, PNAMP_E_SIZE, "2 nphi h^2 complex exceeds the combine LDS");
     if (hh > 2 * NTC) return fail(c, PNAMP_E_SIZE, "h^2 exceeds the combine rows per workgroup");
     if (h > HMAX || (h & 1)) return fail(c, PNAMP_E_SIZE, "factored form: species size h must be even and <= 22");
     // per pair: rho [NE][2][nphi][h^2], setup blocks [NE][2][5][h^2], per-phi scalars and status;
-    // chunks of pairs up to 16 GiB of scratch
+    // mixed forms add the GEMM operands (4 x 2 NE nphi columns x kp, 4 bytes per element),
+    // A01 [NE][2][nphi][h^2] and G_pn [NE][nphi][h^2] (FP32 complex); chunks up to 24 GiB
     const size_t rho_pp = (size_t)c->NE * 2 * c->nphi * hh, sb_pp = (size_t)c->NE * 2 * 5 * hh;
     const size_t out_pp = (size_t)c->NE * 2 * c->nphi * FAC_OUT, st_pp = (size_t)c->NE * 2 * c->nphi;
-    const size_t bytes_pp = 16 * (rho_pp + sb_pp + out_pp) + 4 * st_pp;
-    const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)npairs, ((size_t)16 << 30) / bytes_pp));
+    const size_t cols_pp = (size_t)2 * c->NE * c->nphi;  // GEMM columns per pair and species
+    size_t bytes_pp = 16 * (rho_pp + sb_pp + out_pp) + 4 * st_pp;
+    if (mix != MIX_FP64) bytes_pp += 4 * 4 * cols_pp * kp + 16 * rho_pp + 8 * (size_t)c->NE * c->nphi * hh;
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)npairs, ((size_t)24 << 30) / bytes_pp));
     if ((size_t)chunk > c->fac_cap) {
       if ((r = dalloc(c, &c->d_frho, rho_pp * chunk)) || (r = dalloc(c, &c->d_sbuf, sb_pp * chunk)) ||
           (r = dalloc(c, &c->d_fout, out_pp * chunk)) || (r = dalloc(c, &c->d_fstatus, st_pp * chunk)))
         return r;
       c->fac_cap = chunk;
+      c->mix_cap = 0;
+    }
+    const size_t ncols_cap = (cols_pp * chunk + 127) / 128 * 128;
+    if (mix != MIX_FP64 && (c->mix_cap < (size_t)chunk || c->mix_cap_mix != mix)) {
+      // GEMM operands: rows k >= h^2 and columns past a chunk's last stay zero
+      const size_t eb = (mix == MIX_FP32) ? 4 : 2;
+      for (int q = 0; q < 2; ++q) {
+        void **bufs[4] = {&c->d_brho[q], &c->d_brho_lo[q], &c->d_ba10[q], &c->d_ba10_lo[q]};
+        for (int b = 0; b < 4; ++b) {
+          free_void(c, bufs[b]);
+          if (mix == MIX_FP32 && (b & 1)) continue;
+          HIPCHK(c, hipMalloc(bufs[b], ncols_cap * kp * eb));
+          HIPCHK(c, hipMemset(*bufs[b], 0, ncols_cap * kp * eb));
+        }
+      }
+      if ((r = dalloc(c, &c->d_a01, rho_pp * chunk)) || (r = dalloc(c, &c->d_gpn, (size_t)2 * c->NE * c->nphi * hh * chunk)))
+        return r;
+      c->mix_cap = chunk;
+      c->mix_cap_mix = mix;
+    }
+    if (mix != MIX_FP64 && c->va_mix != mix) {
+      for (int q = 0; q < NVPROB; ++q)
+        if ((r = upload_vmat(c, q, mix))) return r;
+      c->va_mix = mix;
     }
     if (!combine_attr(c->nphi, comb_lds)) return fail(c, PNAMP_E_SIZE, "factored form supports nphi <= 16");
     const size_t setup_lds = sizeof(double2) * 3 * (size_t)hh;
@@ -2053,17 +2388,38 @@ extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0
     fp.UfT = c->d_UfT; fp.VfT = c->d_VfT; fp.Z = c->d_Z; fp.t1b = c->d_t1b; fp.mtab = c->d_mtab;
     fp.vl0 = c->d_vs[0]; fp.vl1 = c->d_vs[1]; fp.nme0 = c->nme_s[0]; fp.nme1 = c->nme_s[1];
     fp.out = c->d_fout; fp.rho = c->d_frho; fp.status = c->d_fstatus;
+    for (int q = 0; q < 2; ++q) {
+      fp.brho[q] = c->d_brho[q]; fp.brho_lo[q] = c->d_brho_lo[q];
+      fp.ba10[q] = c->d_ba10[q]; fp.ba10_lo[q] = c->d_ba10_lo[q];
+    }
+    fp.a01 = c->d_a01;
+    VGemmParams vp;
+    vp.h = h; vp.nphi = c->nphi; vp.out = c->d_fout;
+    for (int q = 0; q < NVPROB; ++q) {
+      VProblem &P = vp.prob[q];
+      const int spc = (q == 4) ? 1 : (q & 1);
+      P.ahi = c->d_va[q]; P.alo = c->d_va_lo[q];
+      P.bhi = (q < 2 || q == 4) ? c->d_brho[spc] : c->d_ba10[spc];
+      P.blo = (q < 2 || q == 4) ? c->d_brho_lo[spc] : c->d_ba10_lo[spc];
+      P.dot = (q < 2) ? c->d_frho : c->d_a01;
+      P.g = c->d_gpn;
+      P.mode = (q == 4) ? 1 : 0;
+      P.species = spc;
+      P.slot = (q < 2) ? 1 : 8;
+      P.ncols = 0;
+    }
     CombParams cp;
     cp.h = h; cp.NE = c->NE; cp.zval = zval; cp.nval = nval;
     cp.vac_scale = c->vac_scale; cp.snpfaf = c->snpfaf; cp.sgn_h = (h & 1) ? -1.0 : 1.0; cp.e0 = c->e0;
     cp.det = c->d_det; cp.w_ptr = c->d_wptr; cp.w_col = c->d_wcol; cp.w_val = c->d_wval;
-    cp.out = c->d_fout; cp.rho = c->d_frho; cp.fstatus = c->d_fstatus;
-    if (c->fac_graph && (c->g_np != npairs || c->g_z != zval || c->g_n != nval)) drop_graph(c);
+    cp.out = c->d_fout; cp.rho = c->d_frho; cp.fstatus = c->d_fstatus; cp.gpn = (const float2 *)c->d_gpn;
+    if (c->fac_graph && (c->g_np != npairs || c->g_z != zval || c->g_n != nval || c->g_form != c->form))
+      drop_graph(c);
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     if (!c->fac_graph) {
-      // capture the species / combine launches of all chunks once; later calls
-      // with the same shape replay the graph (the pair lists are copied into
-      // the same device buffers before each replay)
+      // capture the launches of all chunks once; later calls with the same
+      // shape and form replay the graph (the pair lists are copied into the
+      // same device buffers before each replay)
       hipGraph_t graph = nullptr;
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       for (int p0 = 0; p0 < npairs; p0 += chunk) {
@@ -2076,8 +2432,13 @@ extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0
         const size_t nwg = (size_t)np * c->NE * 2 * c->nphi;
         fp.nwg = (int)nwg;
         hipLaunchKernelGGL(k_fac_setup, dim3(c->NE, np, 2), dim3(NTS), setup_lds, c->stream, sp);
-        launch_phi_h(h, (unsigned)((nwg + 7) / 8 * 8), c->stream, fp);
-        launch_combine_np(c->nphi, dim3(c->NE, np), comb_lds, c->stream, cp);
+        launch_phi_h(h, mix, (unsigned)((nwg + 7) / 8 * 8), c->stream, fp);
+        if (mix != MIX_FP64) {
+          const int ncols = (int)(cols_pp * np);
+          for (int q = 0; q < NVPROB; ++q) vp.prob[q].ncols = ncols;
+          launch_vgemm(h, mix, dim3((ncols + 127) / 128, NVPROB), c->stream, vp);
+        }
+        launch_combine_np(c->nphi, mix != MIX_FP64, dim3(c->NE, np), comb_lds, c->stream, cp);
       }
       const hipError_t cap = hipStreamEndCapture(c->stream, &graph);
       if (cap != hipSuccess) return fail(c, PNAMP_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(cap));
@@ -2090,6 +2451,7 @@ extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0
       c->g_np = npairs;
       c->g_z = zval;
       c->g_n = nval;
+      c->g_form = c->form;
     }
     HIPCHK(c, hipGraphLaunch(c->fac_graph, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
@@ -2228,6 +2590,17 @@ extern "C" void pnamp_destroy(pnamp_ctx *c) {
                   c->d_sbuf, c->d_fstatus, c->d_wptr, c->d_wcol, c->d_wval, c->d_jpack};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
+  for (int q = 0; q < NVPROB; ++q) {
+    if (c->d_va[q]) (void)hipFree(c->d_va[q]);
+    if (c->d_va_lo[q]) (void)hipFree(c->d_va_lo[q]);
+  }
+  for (int q = 0; q < 2; ++q) {
+    void *mb[4] = {c->d_brho[q], c->d_brho_lo[q], c->d_ba10[q], c->d_ba10_lo[q]};
+    for (void *b : mb)
+      if (b) (void)hipFree(b);
+  }
+  if (c->d_a01) (void)hipFree(c->d_a01);
+  if (c->d_gpn) (void)hipFree(c->d_gpn);
   for (auto e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpnamp_gpu.so")
 NOBS = 6
 OBS = ("norm", "Z", "N", "Ekin", "E", "J2")
-FORM_PLAIN, FORM_FACTORED = 0, 1
+FORM_PLAIN, FORM_FACTORED, FORM_MIXED_FP32, FORM_MIXED_BF16 = 0, 1, 2, 3
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
@@ -157,7 +157,8 @@ class PNAMP:
         self.nstates = U.shape[0]
 
     def set_form(self, form: int):
-        """FORM_PLAIN (general) or FORM_FACTORED (states without p-n mixing)."""
+        """FORM_PLAIN (general), FORM_FACTORED (states without p-n mixing, FP64) or
+        FORM_MIXED_FP32 / FORM_MIXED_BF16 (factored, V contraction on MFMA)."""
         self._check(self._L.pnamp_set_form(self._h, int(form)))
 
     @property

@@ -224,3 +224,61 @@ def test_factored_graph_replays_match():
         again = g.kernels_batched(w.pairs, w.zval, w.nval)
         assert np.array_equal(again, first), rep
     g.close()
+
+
+# Mixed forms: the V contraction on MFMA in FP32 or split BF16.  Observables
+# without the two-body contraction (norm, Z, N, E_kin, <J^2>) stay FP64 and
+# match the factored form to 1e-10; the energy kernel carries the GEMM
+# rounding: FP32 operands (2^-24) and split BF16 (about 2^-17 per operand)
+# with FP32 accumulation over h^2 terms.
+MIX_TOL = {2: 2e-5, 3: 5e-4}
+
+
+@pytest.mark.parametrize("form", [2, 3])
+@pytest.mark.parametrize("space,zn,mesh_args", [
+    ("sd", (2, 2), (3, 4, 3, 5, 6)),
+    ("pf", (2, 4), (2, 3, 3, 9, 8)),
+    ("f5pg9", (4, 2), (2, 2, 2, 3, 4)),
+])
+def test_mixed_forms_match_factored(space, zn, mesh_args, form):
+    from sccm_amd.pnamp import FORM_FACTORED
+    b = make_basis(space)
+    inter = synthetic_interaction(b, 1.2, 201, 202)
+    mesh = make_mesh(*mesh_args)
+    Us, Vs = zip(*[synthetic_state(b, *zn, seed=s) for s in (21, 22, 23)])
+    g = _gpu(b, inter, mesh)
+    g.set_states(np.array(Us), np.array(Vs))
+    pairs = [(0, 1), (1, 2), (2, 2), (0, 0)]
+    g.set_form(FORM_FACTORED)
+    zf, kf = g.kernels_batched(pairs, *zn, want_zres=True)
+    g.set_form(form)
+    zm, km = g.kernels_batched(pairs, *zn, want_zres=True)
+    worst_e = 0.0
+    for i in range(len(pairs)):
+        for io in (0, 1, 2, 3, 5):
+            assert _rel(zm[i][:, io], zf[i][:, io]) < REL, (space, pairs[i], io)
+        e = _rel(zm[i][:, 4], zf[i][:, 4])
+        worst_e = max(worst_e, e)
+        assert e < MIX_TOL[form], (space, pairs[i], e)
+        for io in (0, 1, 2, 3, 5):
+            assert _blockwise(km[i][io:io + 1], kf[i][io:io + 1]) < REL
+        assert _blockwise(km[i][4:5], kf[i][4:5]) < MIX_TOL[form]
+    print(f"{space} form {form}: max rel error of the energy ZRES {worst_e:.3e}")
+    g.close()
+
+
+@pytest.mark.parametrize("form", [2, 3])
+def test_mixed_graph_replays_match(form):
+    from sccm_amd.pnamp import FORM_PLAIN
+    w = configs.pf_gcm(nstates=4, nalp=2, nbet=3, nphi=5, jmax=4)
+    g = _gpu(w.basis, w.inter, w.mesh)
+    g.set_states(w.U, w.V)
+    g.set_form(form)
+    first = g.kernels_batched(w.pairs, w.zval, w.nval)
+    for rep in range(2):
+        g.set_form(FORM_PLAIN)
+        g.kernels_batched(w.pairs[:2], w.zval, w.nval)
+        g.set_form(form)
+        again = g.kernels_batched(w.pairs, w.zval, w.nval)
+        assert np.array_equal(again, first), rep
+    g.close()
