@@ -134,6 +134,20 @@ int pnamp_kernel_pair(pnamp_ctx *ctx, int q0, int q1, int zval, int nval, double
 int pnamp_kernels_batched(pnamp_ctx *ctx, int npairs, const int32_t *q0, const int32_t *q1,
                           int zval, int nval, double *zres, double *kern);
 
+/* Norm and energy of every gauge point: samples [npairs][NE][nphi][nphi][2]
+ * complex = (norm, E + E0) per (pair, Euler point, phi_p, phi_n), norm as
+ * summed into ZRES slot 1 before the 1/nphi^2 (PNAMP_Kernel3D.f90:294-324).
+ * Factored and mixed forms only.  No counterpart in the reference: training
+ * data of the learned surrogate (include/pnamp_mlp.h). */
+int pnamp_mesh_samples(pnamp_ctx *ctx, int npairs, const int32_t *q0, const int32_t *q1, int zval, int nval,
+                       double *samples);
+
+/* J-projection of caller-supplied ZRES [npairs][NE][PNAMP_NOBS] complex into
+ * kern (layout of pnamp_kernels_batched): the projection half of
+ * Kernel4Triaxial (PNAMP_Kernel3D.f90:415-526), for kernels assembled from
+ * surrogate and exact mesh points. */
+int pnamp_project_zres(pnamp_ctx *ctx, int npairs, const double *zres, double *kern);
+
 /* Hill-Wheeler solve of one J block: HNDIAG (Codes/GCM/GCM_HNDiag.f90:2-103).
  * N, H real symmetric dim x dim column-major (H norm-weighted); keeps norm
  * eigenvalues with EN(i)/EN(1) >= eps, at most nos of them (nos <= 0: all).

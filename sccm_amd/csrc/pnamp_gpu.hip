@@ -1268,6 +1268,7 @@ struct CombParams {
   const double2 *out, *rho;
   const int *fstatus;               // [npairs][NE][2][nphi]
   const float2 *gpn;                // mixed forms: G = W_pn rho_n from k_vgemm, [npairs][NE][nphi][h*h]
+  double2 *samples;                 // optional: [npairs][NE][nphi][nphi][2] = norm, energy per gauge pair
   double2 *zres;                    // [npairs][NE][NOBS]
   int *status;                      // [npairs][NE]
 };
@@ -1368,6 +1369,11 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
     const double2 tzs = cadd(a[7], b[7]);
     const double2 J2 = cadd(cadd(cadd(a[4], b[4]), cmul(tzs, tzs)), cmul(cadd(a[5], b[5]), cadd(a[6], b[6])));
     const double2 o[NOBS] = {zn, cmul(zn, a[3]), cmul(zn, b[3]), cmul(zn, ekin), cmul(zn, E), cmul(zn, J2)};
+    if (p.samples) {
+      double2 *sm = p.samples + (((size_t)ip * p.NE + iang) * NP * NP + t) * 2;
+      sm[0] = zn;
+      sm[1] = E;
+    }
 #pragma unroll
     for (int k = 0; k < NOBS; ++k) {
       v[2 * k] += o[k].x;
@@ -2304,8 +2310,11 @@ static int ensure_pairs(pnamp_ctx *c, int np) {
   return 0;
 }
 
-extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0, const int32_t *q1, int zval,
-                                     int nval, double *zres, double *kern) {
+// The batched kernels; with samples (device, [npairs][NE][nphi][nphi][2]) the
+// factored forms also write the norm and energy of every gauge pair, and run
+// their launches directly instead of through the captured graph.
+static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32_t *q1, int zval, int nval,
+                        double *zres, double *kern, double2 *samples) {
   if (!c || npairs <= 0 || !q0 || !q1) return PNAMP_E_ARG;
   if (!c->nstates) return fail(c, PNAMP_E_STATE, "no states set");
   for (int i = 0; i < npairs; ++i)
@@ -2413,15 +2422,17 @@ extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0
     cp.vac_scale = c->vac_scale; cp.snpfaf = c->snpfaf; cp.sgn_h = (h & 1) ? -1.0 : 1.0; cp.e0 = c->e0;
     cp.det = c->d_det; cp.w_ptr = c->d_wptr; cp.w_col = c->d_wcol; cp.w_val = c->d_wval;
     cp.out = c->d_fout; cp.rho = c->d_frho; cp.fstatus = c->d_fstatus; cp.gpn = (const float2 *)c->d_gpn;
+    cp.samples = nullptr;
     if (c->fac_graph && (c->g_np != npairs || c->g_z != zval || c->g_n != nval || c->g_form != c->form))
       drop_graph(c);
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    if (!c->fac_graph) {
-      // capture the launches of all chunks once; later calls with the same
-      // shape and form replay the graph (the pair lists are copied into the
-      // same device buffers before each replay)
-      hipGraph_t graph = nullptr;
-      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const bool direct = samples != nullptr;
+    if (direct || !c->fac_graph) {
+      // the launches of all chunks: captured once into a graph that later
+      // calls of the same shape and form replay (the pair lists are copied
+      // into the same device buffers before each replay), or issued directly
+      // when gauge-point samples are requested
+      if (!direct) HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       for (int p0 = 0; p0 < npairs; p0 += chunk) {
         const int np = std::min(chunk, npairs - p0);
         sp.q0 = c->d_q0 + p0; sp.q1 = c->d_q1 + p0;
@@ -2429,6 +2440,7 @@ extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0
         cp.q0 = c->d_q0 + p0; cp.q1 = c->d_q1 + p0;
         cp.zres = c->d_zres + (size_t)p0 * c->NE * NOBS;
         cp.status = c->d_status + (size_t)p0 * c->NE;
+        if (direct) cp.samples = samples + (size_t)p0 * c->NE * c->nphi * c->nphi * 2;
         const size_t nwg = (size_t)np * c->NE * 2 * c->nphi;
         fp.nwg = (int)nwg;
         hipLaunchKernelGGL(k_fac_setup, dim3(c->NE, np, 2), dim3(NTS), setup_lds, c->stream, sp);
@@ -2440,6 +2452,9 @@ extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0
         }
         launch_combine_np(c->nphi, mix != MIX_FP64, dim3(c->NE, np), comb_lds, c->stream, cp);
       }
+    }
+    if (!direct && !c->fac_graph) {
+      hipGraph_t graph = nullptr;
       const hipError_t cap = hipStreamEndCapture(c->stream, &graph);
       if (cap != hipSuccess) return fail(c, PNAMP_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(cap));
       const hipError_t ins = hipGraphInstantiate(&c->fac_graph, graph, nullptr, nullptr, 0);
@@ -2453,7 +2468,8 @@ extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0
       c->g_n = nval;
       c->g_form = c->form;
     }
-    HIPCHK(c, hipGraphLaunch(c->fac_graph, c->stream));
+    if (!direct) HIPCHK(c, hipGraphLaunch(c->fac_graph, c->stream));
+    HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   } else {
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
@@ -2480,6 +2496,46 @@ extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0
   HIPCHK(c, hipEventElapsedTime(&c->ms_total, c->ev[0], c->ev[3]));
   for (int i = 0; i < npairs; ++i)
     if (pst[i]) return fail(c, pst[i], "singular T (ROKAPPAphiomega) in pair " + std::to_string(i));
+  return PNAMP_OK;
+}
+
+extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0, const int32_t *q1, int zval,
+                                     int nval, double *zres, double *kern) {
+  return kernels_impl(c, npairs, q0, q1, zval, nval, zres, kern, nullptr);
+}
+
+extern "C" int pnamp_mesh_samples(pnamp_ctx *c, int npairs, const int32_t *q0, const int32_t *q1, int zval, int nval,
+                                  double *samples) {
+  if (!c || !samples || npairs <= 0) return PNAMP_E_ARG;
+  if (c->form == PNAMP_FORM_PLAIN) return fail(c, PNAMP_E_ARG, "mesh samples need a factored or mixed form");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t cnt = (size_t)npairs * c->NE * c->nphi * c->nphi * 2;
+  double2 *d = nullptr;
+  HIPCHK(c, hipMalloc((void **)&d, cnt * sizeof(double2)));
+  const int r = kernels_impl(c, npairs, q0, q1, zval, nval, nullptr, nullptr, d);
+  hipError_t e = hipSuccess;
+  if (r == PNAMP_OK) e = hipMemcpy(samples, d, cnt * sizeof(double2), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(c, PNAMP_E_DEVICE, std::string("samples copy: ") + hipGetErrorString(e));
+  return r;
+}
+
+extern "C" int pnamp_project_zres(pnamp_ctx *c, int npairs, const double *zres, double *kern) {
+  if (!c || npairs <= 0 || !zres || !kern) return PNAMP_E_ARG;
+  if (npairs > 65535) return fail(c, PNAMP_E_SIZE, "at most 65535 pairs per call");
+  int r = ensure_pairs(c, npairs);
+  if (r) return r;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(c->d_zres, zres, sizeof(double2) * (size_t)npairs * c->NE * NOBS, hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_status, 0, sizeof(int) * (size_t)npairs * c->NE, c->stream));
+  ProjParams pr{c->nalp, c->nbet, c->ngam, c->NE, c->jmax, c->fold, c->npj, c->d_alp, c->d_walp, c->d_wbet,
+                c->d_gam, c->d_wgam, c->d_ptab, c->d_zres, c->d_status, c->d_kern, c->d_pair_status};
+  hipLaunchKernelGGL(k_project, dim3(npairs, NOBS), dim3(256), 0, c->stream, pr);
+  HIPCHK(c, hipGetLastError());
+  const size_t kper = (size_t)NOBS * (c->jmax + 1) * (2 * c->jmax + 1) * (2 * c->jmax + 1);
+  HIPCHK(c, hipMemcpyAsync(kern, c->d_kern, sizeof(double2) * kper * npairs, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return PNAMP_OK;
 }
 

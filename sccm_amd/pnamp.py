@@ -70,6 +70,10 @@ def load_library(path: str = LIB_PATH):
     L.pnamp_kernel_pair.restype = C.c_int
     L.pnamp_kernels_batched.argtypes = [vp, C.c_int, _i32p, _i32p, C.c_int, C.c_int, _f64p, _f64p]
     L.pnamp_kernels_batched.restype = C.c_int
+    L.pnamp_mesh_samples.argtypes = [vp, C.c_int, _i32p, _i32p, C.c_int, C.c_int, _f64p]
+    L.pnamp_mesh_samples.restype = C.c_int
+    L.pnamp_project_zres.argtypes = [vp, C.c_int, _f64p, _f64p]
+    L.pnamp_project_zres.restype = C.c_int
     L.pnamp_hw_solve.argtypes = [vp, C.c_int, _f64p, _f64p, C.c_double, C.c_int, _f64p, _f64p, _f64p,
                                  C.POINTER(C.c_int)]
     L.pnamp_hw_solve.restype = C.c_int
@@ -193,6 +197,25 @@ class PNAMP:
                                                   _p(zres) if zres is not None else None,
                                                   _p(kern) if kern is not None else None))
         return (zres, kern) if want_zres else kern
+
+    def mesh_samples(self, pairs, zval: int, nval: int) -> np.ndarray:
+        """Norm and energy of every gauge point: [npairs, NE, nphi, nphi, 2]
+        complex (factored and mixed forms)."""
+        pairs = np.asarray(pairs, dtype=np.int32).reshape(-1, 2)
+        q0 = np.ascontiguousarray(pairs[:, 0])
+        q1 = np.ascontiguousarray(pairs[:, 1])
+        f = self.mesh.nphi
+        out = np.zeros((len(pairs), self.n_euler, f, f, 2), dtype=np.complex128)
+        self._check(self._L.pnamp_mesh_samples(self._h, len(pairs), _p(q0, C.c_int32), _p(q1, C.c_int32),
+                                               int(zval), int(nval), _p(out)))
+        return out
+
+    def project_zres(self, zres: np.ndarray) -> np.ndarray:
+        """J-projection of ZRES [npairs, NE, NOBS] complex into kernels."""
+        z = np.ascontiguousarray(zres, dtype=np.complex128)
+        kern = np.zeros((z.shape[0],) + self._kshape(), dtype=np.complex128)
+        self._check(self._L.pnamp_project_zres(self._h, z.shape[0], _p(z), _p(kern)))
+        return kern
 
     def timing(self) -> tuple[float, float]:
         a, b = C.c_float(), C.c_float()

@@ -1,0 +1,108 @@
+"""Learned surrogate: the HIP network kernels against the NumPy reference
+(gradients by backpropagation, checked also by central differences of the
+reference), and what the trained network gains over no network on a small
+pf GCM."""
+import numpy as np
+import pytest
+
+import mlp_ref
+
+
+def test_reference_gradient_matches_finite_differences():
+    rng = np.random.default_rng(5)
+    sizes = [4, 7, 6, 3]
+    npar = sum(sizes[i] * sizes[i + 1] + sizes[i + 1] for i in range(3))
+    p = rng.normal(0, 0.5, npar)
+    X, Y = rng.normal(size=(9, 4)), rng.normal(size=(9, 3))
+    w = np.array([1.0, 0.5, 2.0])
+    _, g = mlp_ref.loss_grad(p, sizes, X, Y, w)
+    h = 1e-6
+    for k in range(0, npar, 3):
+        e = np.zeros(npar)
+        e[k] = h
+        fd = (mlp_ref.loss_grad(p + e, sizes, X, Y, w)[0] - mlp_ref.loss_grad(p - e, sizes, X, Y, w)[0]) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-7 + 1e-6 * abs(fd), k
+
+
+def test_features_and_targets_roundtrip():
+    from sccm_amd import configs, surrogate
+    w = configs.pf_gcm(nstates=3, nalp=2, nbet=3, nphi=3, jmax=2)
+    X = surrogate.mesh_features(w, w.pairs[:2])
+    assert X.shape == (2, w.mesh.n_euler, 3, 3, surrogate.NIN)
+    assert np.allclose(X[..., 4] ** 2 + X[..., 5] ** 2, 1.0)
+    rng = np.random.default_rng(1)
+    s = rng.normal(size=(2, 4, 3, 3, 2)) + 1j * rng.normal(size=(2, 4, 3, 3, 2))
+    N, E = surrogate.targets_to_values(surrogate.mesh_targets(s))
+    assert np.allclose(N, s[..., 0]) and np.allclose(E, s[..., 1])
+
+
+def test_assemble_zres_without_surrogate_points_is_exact():
+    from sccm_amd import surrogate
+    rng = np.random.default_rng(2)
+    s = rng.normal(size=(2, 5, 3, 3, 2)) + 1j * rng.normal(size=(2, 5, 3, 3, 2))
+    z = np.zeros((2, 5, 6), dtype=complex)
+    z[..., 0] = s[..., 0].reshape(2, 5, 9).mean(-1)
+    z[..., 4] = (s[..., 0] * s[..., 1]).reshape(2, 5, 9).mean(-1)
+    za, frac = surrogate.assemble_zres(s, z, np.zeros_like(s[..., 0]), np.zeros_like(s[..., 1]), 1e-3)
+    assert frac == 0.0 and np.allclose(za, z)
+
+
+@pytest.mark.gpu
+def test_mlp_gradient_matches_reference():
+    from sccm_amd.surrogate import MLP
+    rng = np.random.default_rng(7)
+    sizes = [14, 32, 24, 5]
+    net = MLP(sizes, seed=11)
+    p = net.get_params().astype(np.float64)
+    X = rng.normal(size=(300, 14)).astype(np.float32)
+    Y = rng.normal(size=(300, 5)).astype(np.float32)
+    net.set_data(X, Y)
+    idx = rng.permutation(300)[:200]
+    loss, g = net.loss_grad(idx)
+    lr, gr = mlp_ref.loss_grad(p, sizes, X[idx], Y[idx])
+    assert abs(loss - lr) <= 1e-5 * abs(lr)
+    assert np.abs(g - gr).max() <= 1e-4 * np.abs(gr).max()
+    P = net.predict(X[:50])
+    assert np.abs(P - mlp_ref.forward(p, sizes, X[:50])).max() <= 1e-5 * np.abs(P).max()
+    Pb = net.predict(X[:50], bf16=True)
+    assert np.abs(Pb - P).max() <= 3e-2 * np.abs(P).max()
+    net.close()
+
+
+@pytest.mark.gpu
+def test_mlp_training_fits_a_smooth_function():
+    from sccm_amd.surrogate import MLP
+    rng = np.random.default_rng(8)
+    X = rng.uniform(-1, 1, size=(8192, 3)).astype(np.float32)
+    Y = np.stack([np.sin(2 * X[:, 0]) * X[:, 1], X[:, 2] ** 2], 1).astype(np.float32)
+    net = MLP([3, 64, 64, 2], seed=3)
+    net.set_data(X, Y)
+    losses = net.train(40, 256, 3e-3, 0.97, seed=4)
+    assert losses[-1] < 0.02 * losses[0]
+    net.close()
+
+
+@pytest.mark.gpu
+def test_surrogate_gain_on_small_pf_gcm():
+    """The network trained on exact mesh samples predicts log|N| on held-out
+    points about three times better than the best constant (the method
+    without it); the Hill-Wheeler energies of the surrogate + FP64-fallback
+    kernels are reported for both policies (their |dE| is far above 1 keV on
+    this GCM, which is why the chosen form evaluates every point exactly)."""
+    from sccm_amd import configs, surrogate
+    from sccm_amd.pnamp import FORM_FACTORED, PNAMP
+    w = configs.pf_gcm(nstates=5, nalp=4, nbet=6, nphi=5, jmax=4)
+    ctx = PNAMP(w.basis, w.inter, w.mesh)
+    ctx.set_states(w.U, w.V)
+    ctx.set_form(FORM_FACTORED)
+    n = len(w.pairs) * w.mesh.n_points
+    r = surrogate.run(w, ctx, n_train=int(0.6 * n), n_test=int(0.3 * n), width=64, hidden=3, epochs=40,
+                      batch=512, lr=3e-3, decay=0.95, log=lambda *a: None)
+    t = r["test_fp32"]
+    print(r)
+    assert r["loss_last"] < 0.6 * r["loss_first"]
+    assert t["rmse_log_abs_N"] < 0.35 * t["rmse_log_abs_N_mean_predictor"]
+    for policy in ("replace", "screen"):
+        assert 0.0 <= r[policy]["fraction_of_mesh_points_from_surrogate"] <= 1.0
+        assert r[policy]["max_abs_dE_keV"] is not None
+    ctx.close()
