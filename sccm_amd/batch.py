@@ -1,0 +1,134 @@
+"""Uneven batch of GCM instances across model spaces (config C4).
+
+64 GCM instances drawn (seeded) from the sd (n=24), pf (n=40) and f5pg9
+(n=44) spaces, valence Z, N in {2, 4, 6} (even: the synthetic HFB vacua keep
+even number parity, so U is invertible as zbareover needs), 5-30 configurations
+each on a (beta2, gamma) grid, all pairs q <= q', mesh alpha=gamma 10, beta 20,
+gauge 9 per species, J <= 8.  Interaction per space with the sd recipe
+(J-coupled TBMEs N(0, 1.5) MeV, single-particle energies U[-5, 0] MeV).
+
+One device context per model space holds that space's basis, interaction
+and mesh; each instance sets its states and runs all its pairs in one
+batched call (chunked, graph-captured launches of the factored form).  The
+plain FP64 path is timed on a seeded sample of pairs of every instance and
+its whole-batch time extrapolated per space (it runs at ~0.6 M mesh-pts/s).
+"""
+from __future__ import annotations
+
+import json
+import sys
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import configs
+from .basis import make_basis
+from .interaction import synthetic_interaction
+from .mesh import make_mesh
+
+SPACES = ("sd", "pf", "f5pg9")
+SEEDS = {"sd": (1001, 1002), "pf": (1021, 1022), "f5pg9": (1031, 1032)}
+
+
+@dataclass
+class Instance:
+    space: str
+    zval: int
+    nval: int
+    nconf: int
+    seed: int
+
+    @property
+    def npairs(self) -> int:
+        return self.nconf * (self.nconf + 1) // 2
+
+
+def draw_instances(n: int = 64, seed: int = 4001) -> list[Instance]:
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        out.append(Instance(SPACES[int(rng.integers(3))], int(2 * rng.integers(1, 4)), int(2 * rng.integers(1, 4)),
+                            int(rng.integers(5, 31)), seed + 10 * (i + 1)))
+    return out
+
+
+def instance_states(inst: Instance, basis):
+    coords = configs.gcm_grid(inst.nconf)
+    U, V = configs.make_states(basis, inst.zval, inst.nval, coords, seed=inst.seed)
+    pairs = np.array([(q, q1) for q in range(inst.nconf) for q1 in range(q, inst.nconf)], dtype=np.int32)
+    return U, V, pairs
+
+
+def run(n: int = 64, seed: int = 4001, nalp: int = 10, nbet: int = 20, nphi: int = 9, jmax: int = 8,
+        form: str = "factored", plain_pairs: int = 2, log=print) -> dict:
+    from .pnamp import FORM_FACTORED, FORM_MIXED_BF16, FORM_MIXED_FP32, FORM_PLAIN, PNAMP
+    fcode = {"factored": FORM_FACTORED, "mixed-fp32": FORM_MIXED_FP32, "mixed-bf16": FORM_MIXED_BF16}[form]
+    insts = draw_instances(n, seed)
+    mesh = make_mesh(nalp, nbet, nalp, nphi, jmax)
+    ctxs = {}
+    for sp in SPACES:
+        b = make_basis(sp)
+        ctxs[sp] = (b, PNAMP(b, synthetic_interaction(b, 1.5, *SEEDS[sp]), mesh))
+    rng = np.random.default_rng(seed + 1)
+    tot_pts, t_chosen = 0, 0.0
+    per_space = {sp: {"instances": 0, "pairs": 0, "mesh_points": 0, "chosen_s": 0.0, "plain_s": 0.0,
+                      "plain_points": 0} for sp in SPACES}
+    worst_rel = 0.0
+    for k, inst in enumerate(insts):
+        b, ctx = ctxs[inst.space]
+        U, V, pairs = instance_states(inst, b)
+        ctx.set_states(U, V)
+        ctx.set_form(fcode)
+        t = time.perf_counter()
+        kern = ctx.kernels_batched(pairs, inst.zval, inst.nval)
+        dt = time.perf_counter() - t
+        pts = len(pairs) * mesh.n_points
+        s = per_space[inst.space]
+        s["instances"] += 1
+        s["pairs"] += len(pairs)
+        s["mesh_points"] += pts
+        s["chosen_s"] += dt
+        tot_pts += pts
+        t_chosen += dt
+        # plain FP64 on a seeded sample of this instance's pairs
+        sel = rng.choice(len(pairs), size=min(plain_pairs, len(pairs)), replace=False)
+        ctx.set_form(FORM_PLAIN)
+        t = time.perf_counter()
+        kp = ctx.kernels_batched(pairs[sel], inst.zval, inst.nval)
+        s["plain_s"] += time.perf_counter() - t
+        s["plain_points"] += len(sel) * mesh.n_points
+        rel = float(np.abs(kern[sel] - kp).max() / np.abs(kp).max())
+        worst_rel = max(worst_rel, rel)
+        log(f"batch: {k + 1}/{n} {inst.space} Z={inst.zval} N={inst.nval} {inst.nconf} configs "
+            f"{len(pairs)} pairs {pts / dt / 1e6:.1f} M mesh-pts/s, rel diff vs plain {rel:.1e}")
+    t_plain_est = sum(s["plain_s"] / s["plain_points"] * s["mesh_points"] for s in per_space.values()
+                      if s["plain_points"])
+    for c in ctxs.values():
+        c[1].close()
+    return {
+        "metric": "mesh-pts/s, uneven GCM batch (sd/pf/f5pg9)", "form": form,
+        "instances": n, "mesh_points": tot_pts, "chosen_seconds": t_chosen,
+        "chosen_mesh_pts_per_s": tot_pts / t_chosen,
+        "plain_fp64_mesh_pts_per_s_extrapolated": tot_pts / t_plain_est,
+        "speedup_vs_plain": t_plain_est / t_chosen,
+        "max_rel_kernel_diff_vs_plain_on_sampled_pairs": worst_rel,
+        "per_space": per_space,
+        "plain_sample": f"{plain_pairs} seeded pairs per instance, whole-batch plain time extrapolated per space",
+    }
+
+
+def main():
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--form", default="factored", choices=("factored", "mixed-fp32", "mixed-bf16"))
+    ap.add_argument("--plain-pairs", type=int, default=2)
+    a = ap.parse_args()
+    r = run(a.instances, form=a.form, plain_pairs=a.plain_pairs,
+            log=lambda *x: print(*x, file=sys.stderr, flush=True))
+    print(json.dumps(r), flush=True)
+
+
+if __name__ == "__main__":
+    main()
