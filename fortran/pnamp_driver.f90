@@ -5,8 +5,10 @@
 ! each pair is written in the post_general layout (Codes/PNAMP/PNAMP_IO.f90:39-94,
 ! formats (10i5) / (8e15.8)) with the CHOP rule of PNAMP_Kernel3D.f90:729-737.
 !
-! usage: pnamp_driver <case.bin> <outdir>
+! usage: pnamp_driver <case.bin> <outdir> [plain|factored]
 !   case.bin is an unformatted stream file (see sccm_amd/driver.py:write_case).
+!   factored: the factored FP64 form when the states have no proton-neutron
+!   mixing (same kernels to rounding), else the plain form.
 program pnamp_driver
   use iso_c_binding
   use pnamp_gpu
@@ -25,11 +27,13 @@ program pnamp_driver
   type(c_ptr) :: ctx
   integer(c_int) :: rc
   real(c_float) :: ms_pts, ms_tot
-  character(len=512) :: casefile, outdir
+  character(len=512) :: casefile, outdir, formarg
   integer :: ip, u_in
 
   call get_command_argument(1, casefile)
   call get_command_argument(2, outdir)
+  formarg = 'plain'
+  if (command_argument_count() >= 3) call get_command_argument(3, formarg)
   open(newunit=u_in, file=trim(casefile), access='stream', form='unformatted', status='old')
   read(u_in) hdr, dh
   n = hdr(1); nme = hdr(2); nalp = hdr(3); nbet = hdr(4); ngam = hdr(5); nphi = hdr(6)
@@ -51,6 +55,11 @@ program pnamp_driver
   call check(rc, 'pnamp_create')
   rc = pnamp_set_states(ctx, int(nstates, c_int), u, v)
   call check(rc, 'pnamp_set_states')
+  if (trim(formarg) == 'factored' .and. pnamp_states_separable(ctx) == 1) then
+    rc = pnamp_set_form(ctx, PNAMP_FORM_FACTORED)
+    call check(rc, 'pnamp_set_form')
+    write(*, '(a)') ' pnamp_driver: factored form'
+  end if
   allocate(kern(-jmax:jmax, -jmax:jmax, 0:jmax, PNAMP_NOBS, npairs))
   rc = pnamp_kernels_batched(ctx, int(npairs, c_int), q0, q1, int(zval, c_int), int(nval, c_int), &
                              c_null_ptr, c_loc(kern))

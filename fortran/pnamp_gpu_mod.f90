@@ -12,9 +12,12 @@ module pnamp_gpu
   public :: pnamp_basis_t, pnamp_interaction_t, pnamp_mesh_t
   public :: pnamp_create, pnamp_set_states, pnamp_kernel_pair, pnamp_kernels_batched
   public :: pnamp_hw_solve, pnamp_last_timing, pnamp_last_error_f, pnamp_destroy
-  public :: PNAMP_NOBS
+  public :: pnamp_set_form, pnamp_states_separable, pnamp_project_zres
+  public :: PNAMP_NOBS, PNAMP_FORM_PLAIN, PNAMP_FORM_FACTORED, PNAMP_FORM_MIXED_FP32, PNAMP_FORM_MIXED_BF16
 
   integer(c_int), parameter :: PNAMP_NOBS = 6
+  integer(c_int), parameter :: PNAMP_FORM_PLAIN = 0, PNAMP_FORM_FACTORED = 1
+  integer(c_int), parameter :: PNAMP_FORM_MIXED_FP32 = 2, PNAMP_FORM_MIXED_BF16 = 3
 
   type, bind(C) :: pnamp_basis_t
     integer(c_int) :: n
@@ -51,6 +54,24 @@ module pnamp_gpu
       type(c_ptr), value :: ctx
       integer(c_int), value :: nstates
       real(c_double), intent(in) :: u(*), v(*)
+    end function
+
+    integer(c_int) function pnamp_set_form(ctx, form) bind(C, name="pnamp_set_form")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: form
+    end function
+
+    integer(c_int) function pnamp_states_separable(ctx) bind(C, name="pnamp_states_separable")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function pnamp_project_zres(ctx, npairs, zres, kern) bind(C, name="pnamp_project_zres")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npairs
+      type(c_ptr), value :: zres, kern
     end function
 
     integer(c_int) function pnamp_kernel_pair(ctx, q0, q1, zval, nval, zres, kern) bind(C, name="pnamp_kernel_pair")
