@@ -994,8 +994,14 @@ __device__ int wave_gauss_jordan(double2 (&col)[H], double2 *fc, int *qmap) {
 //   (1/4) V [r_ijkl - r_jikl - r_ijlk + r_jilk + d (r_klij - r_klji - r_lkij + r_lkji)]
 //   = V [rho_ki rho_lj - rho_li rho_kj + A01_ij A10_kl / 4
 //        + d (rho_ik rho_jl - rho_jk rho_il + A01_kl A10_ij / 4)].
-template <int H, int MIX>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_fac_phi(PhiParams p) {
+//
+// The work is split in two launches over the same grid: PART_PF the Pfaffian
+// (a few KB of LDS and ~100 VGPRs, so many waves per SIMD hide its serial
+// steps), PART_REST the rest (21 KB of LDS for rho / kappa at h = 20).
+constexpr int PART_PF = 1, PART_REST = 2;
+
+template <int H, int MIX, int PART>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART_PF ? 4 : 2))) void k_fac_phi(PhiParams p) {
   constexpr int N2 = 2 * H, NU = N2 * (N2 - 1) / 2, SL = (NU + 63) / 64, HH = H * H;
   constexpr int KP = (HH + 31) / 32 * 32;  // V-contraction K padded to the MFMA tile
   constexpr int PASSES = (3 * H <= 64) ? 1 : 2;
@@ -1023,6 +1029,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   const double phi = (nphi == 1) ? 0.0 : (2.0 * M_PI / nphi) * f;
   const double2 e1 = cexpi(phi), e2 = cexpi(2.0 * phi);
 
+  if constexpr (PART == PART_PF) {
   // ---- Pfaffian
   double2 pf = make_double2(1.0, 0.0);
   {
@@ -1111,6 +1118,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     }
   }
 
+  if (lane == 0) p.out[obase * FAC_OUT] = pf;
+  } else {
   // ---- Gauss-Jordan and the density products
   int info = 0;
   const bool w1lane = lane >= H && lane < 2 * H;
@@ -1250,11 +1259,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   for (int q = 0; q < 14; ++q) v[q] = wave_sum(v[q]);
   if (lane == 0) {
     double2 *o = p.out + obase * FAC_OUT;
-    o[0] = pf;
 #pragma unroll
     for (int q = 0; q < 7; ++q) o[1 + q] = make_double2(v[2 * q], v[2 * q + 1]);
     o[8] = cz();
     p.status[obase] = info;
+  }
   }
 }
 
@@ -1550,9 +1559,10 @@ static bool launch_vgemm(int h, int mix, dim3 grid, hipStream_t s, const VGemmPa
 
 template <int H>
 static void launch_phi(int mix, unsigned grid, hipStream_t s, const PhiParams &pp) {
-  if (mix == MIX_FP32) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP32>), dim3(grid), dim3(64), 0, s, pp);
-  else if (mix == MIX_BF16) hipLaunchKernelGGL((k_fac_phi<H, MIX_BF16>), dim3(grid), dim3(64), 0, s, pp);
-  else hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64>), dim3(grid), dim3(64), 0, s, pp);
+  hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF>), dim3(grid), dim3(64), 0, s, pp);
+  if (mix == MIX_FP32) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP32, PART_REST>), dim3(grid), dim3(64), 0, s, pp);
+  else if (mix == MIX_BF16) hipLaunchKernelGGL((k_fac_phi<H, MIX_BF16, PART_REST>), dim3(grid), dim3(64), 0, s, pp);
+  else hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_REST>), dim3(grid), dim3(64), 0, s, pp);
 }
 
 // Species sizes h = n / 2 with an instantiated k_fac_phi (every h is even:
