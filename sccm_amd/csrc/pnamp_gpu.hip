@@ -1162,6 +1162,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
     const bool isW1 = w1lane && (PASSES == 1 || pass == 0);
     const bool isW2 = (PASSES == 1) ? w2lane1 : (pass == 1 && w1lane);
     // V0s sweep: rho (W1 lanes), kappa01 (W2 lanes)
+#pragma unroll 2
     for (int j = 0; j < H; ++j) {
       double2 acc = cz();
 #pragma unroll
@@ -1178,6 +1179,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
     }
     if (PASSES == 1 || pass == 0) {
       // U0s sweep: kappa10 (W1 lanes)
+#pragma unroll 2
       for (int j = 0; j < H; ++j) {
         double2 acc = cz();
 #pragma unroll
@@ -1243,16 +1245,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
   // ---- species list
   const VEntry *vl = sp ? p.vl1 : p.vl0;
   const int nme = (MIX == MIX_FP64) ? (sp ? p.nme1 : p.nme0) : 0;
-  for (int q = lane; q < nme; q += 64) {
-    const VEntry ve = vl[q];
-    const int i = ve.idx & 255, j = (ve.idx >> 8) & 255, k = (ve.idx >> 16) & 255, l = ve.idx >> 24;
-    double2 s = csub(cmul(ro[k + i * H], ro[l + j * H]), cmul(ro[l + i * H], ro[k + j * H]));
-    s = cadd(s, cscale(0.25, cmul(A01[i + j * H], A10[k + l * H])));
-    if (k + l != i + j) {
-      s = cadd(s, csub(cmul(ro[i + k * H], ro[j + l * H]), cmul(ro[j + k * H], ro[i + l * H])));
-      s = cadd(s, cscale(0.25, cmul(A01[k + l * H], A10[i + j * H])));
+  for (int q0 = lane; q0 < nme; q0 += 4 * 64) {
+    VEntry vq[4];  // four entries loaded ahead of their LDS reads (same summation order)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = q0 + 64 * u;
+      vq[u] = (q < nme) ? vl[q] : VEntry{0.0, 0u, 0u};
     }
-    e2b = cadd(e2b, cscale(ve.me, s));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (q0 + 64 * u >= nme) break;
+      const VEntry ve = vq[u];
+      const int i = ve.idx & 255, j = (ve.idx >> 8) & 255, k = (ve.idx >> 16) & 255, l = ve.idx >> 24;
+      double2 s = csub(cmul(ro[k + i * H], ro[l + j * H]), cmul(ro[l + i * H], ro[k + j * H]));
+      s = cadd(s, cscale(0.25, cmul(A01[i + j * H], A10[k + l * H])));
+      if (k + l != i + j) {
+        s = cadd(s, csub(cmul(ro[i + k * H], ro[j + l * H]), cmul(ro[j + k * H], ro[i + l * H])));
+        s = cadd(s, cscale(0.25, cmul(A01[k + l * H], A10[i + j * H])));
+      }
+      e2b = cadd(e2b, cscale(ve.me, s));
+    }
   }
   double v[14] = {e2b.x, e2b.y, ek.x, ek.y, tr.x, tr.y, aj.x, aj.y, tp.x, tp.y, tm.x, tm.y, tz.x, tz.y};
 #pragma unroll
@@ -1293,16 +1305,15 @@ struct CombParams {
 // stay in registers.  With GMIX (mixed forms) G comes from k_vgemm instead.
 template <int NP, bool GMIX>
 __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
-  extern __shared__ double2 dsm[];  // rho_p [NP][h^2], rho_n -> G [NP][h^2]
+  extern __shared__ double2 dsm[];  // rho_n -> G [NP][h^2]; rho_p is read from global memory
   __shared__ double2 epn[NP * NP];
   __shared__ double red[NTC / 64 * 2 * NOBS];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int iang = blockIdx.x, ip = blockIdx.y;
   const int h = p.h, hh = h * h;
   const size_t base = ((size_t)ip * p.NE + iang) * 2;
-  double2 *rp = dsm, *rn = dsm + (size_t)NP * hh;
+  double2 *rn = dsm;
   for (int e = tid; e < NP * hh; e += NTC) {
-    rp[e] = p.rho[base * NP * hh + e];
     if (GMIX) {
       const float2 gv = p.gpn[((size_t)ip * p.NE + iang) * NP * hh + e];
       rn[e] = make_double2(gv.x, gv.y);
@@ -1321,7 +1332,22 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
     for (int f = 0; f < NP; ++f) g[r][f] = cz();
     const int x = tid + r * NTC;
     if (x < hh) {
-      for (int q = p.w_ptr[x]; q < p.w_ptr[x + 1]; ++q) {
+      const int qe = p.w_ptr[x + 1];
+      int q = p.w_ptr[x];
+      for (; q + 4 <= qe; q += 4) {  // four CSR entries loaded ahead of their use
+        int y[4];
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          y[u] = p.w_col[q + u];
+          v[u] = p.w_val[q + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int f = 0; f < NP; ++f) rfma(g[r][f], v[u], rn[(size_t)f * hh + y[u]]);
+      }
+      for (; q < qe; ++q) {
         const int y = p.w_col[q];
         const double v = p.w_val[q];
 #pragma unroll
@@ -1345,7 +1371,7 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
     double2 acc[NP];
 #pragma unroll
     for (int f = 0; f < NP; ++f) acc[f] = cz();
-    const double2 *rpf = rp + (size_t)fp * hh;
+    const double2 *rpf = p.rho + (base * NP + fp) * hh;
     for (int x = gl; x < hh; x += 32) {
       const double2 a = rpf[x];
 #pragma unroll
@@ -2352,8 +2378,8 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
     const int mix = (c->form == PNAMP_FORM_FACTORED) ? MIX_FP64 : (c->form == PNAMP_FORM_MIXED_FP32 ? MIX_FP32 : MIX_BF16);
     const int h = c->h, hh = h * h, kp = c->kp;
     if (c->nphi > NPHI_MAX) return fail(c, PNAMP_E_SIZE, "factored form supports nphi <= 16");
-    const size_t comb_lds = sizeof(double2) * 2 * (size_t)c->nphi * hh;
-    if (comb_lds > 150 * 1024) return fail(c, PNAMP_E_SIZE, "2 nphi h^2 complex exceeds the combine LDS");
+    const size_t comb_lds = sizeof(double2) * (size_t)c->nphi * hh;
+    if (comb_lds > 150 * 1024) return fail(c, PNAMP_E_SIZE, "nphi h^2 complex exceeds the combine LDS");
     if (hh > 2 * NTC) return fail(c, PNAMP_E_SIZE, "h^2 exceeds the combine rows per workgroup");
     if (h > HMAX || (h & 1)) return fail(c, PNAMP_E_SIZE, "factored form: species size h must be even and <= 22");
     // per pair: rho [NE][2][nphi][h^2], setup blocks [NE][2][5][h^2], per-phi scalars and status;
