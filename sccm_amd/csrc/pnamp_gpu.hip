@@ -1001,14 +1001,22 @@ __device__ int wave_gauss_jordan(double2 (&col)[H], double2 *fc, int *qmap) {
 constexpr int PART_PF = 1, PART_REST = 2;
 
 template <int H, int MIX, int PART>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART_PF ? 4 : 2))) void k_fac_phi(PhiParams p) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART_PF ? 5 : 2))) void k_fac_phi(PhiParams p) {
   constexpr int N2 = 2 * H, NU = N2 * (N2 - 1) / 2, SL = (NU + 63) / 64, HH = H * H;
   constexpr int KP = (HH + 31) / 32 * 32;  // V-contraction K padded to the MFMA tile
   constexpr int PASSES = (3 * H <= 64) ? 1 : 2;
-  __shared__ double2 sA[3 * HH];  // rho, kappa10, kappa01; Gauss-Jordan un-permute buffer before
+  constexpr int HP = H + 1, MS = H * HP;  // LDS leading dimension h + 1: transposed reads avoid bank conflicts
+  __shared__ double2 sA[3 * MS];  // rho, kappa10, kappa01 (ld h + 1); Gauss-Jordan un-permute buffer before
   __shared__ double2 colR[N2], colA[N2], colC[N2];
-  __shared__ double2 fc[H];
-  __shared__ int qmap[H];
+  // Gauss-Jordan pivot column and unknown map: inside the rho block of sA in
+  // one pass (rho is written after they are last read), separate in two
+  // passes (rho of pass 0 must survive pass 1); keeps h = 20 within 20 KB,
+  // eight waves per CU
+  constexpr bool GJ_IN_SA = (PASSES == 1);
+  __shared__ double2 fc_sep[GJ_IN_SA ? 1 : H];
+  __shared__ int qmap_sep[GJ_IN_SA ? 1 : H];
+  double2 *fc = GJ_IN_SA ? sA : fc_sep;
+  int *qmap = GJ_IN_SA ? (int *)(sA + H) : qmap_sep;
   const int lane = threadIdx.x;
   // XCD-aware order: consecutive logical indices (the nphi waves of one setup
   // block) stay on one XCD and its L2; gridDim.x is a multiple of 8.
@@ -1148,16 +1156,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
     if (inf && !info) info = inf;
     // solution rows in unknown order through LDS: unknown qmap[k] is row k
     constexpr int NB = (PASSES == 1) ? 2 * H : H;
-    double2 *ub = (PASSES == 1) ? sA + HH : sA + 2 * HH;
+    double2 *ub = (PASSES == 1) ? sA + MS : sA + 2 * MS;
     const bool blane = lane >= H && lane < H + NB;
     if (blane) {
 #pragma unroll
-      for (int r = 0; r < H; ++r) ub[(lane - H) * H + qmap[r]] = col[r];
+      for (int r = 0; r < H; ++r) ub[(lane - H) * HP + qmap[r]] = col[r];
     }
     __syncthreads();
     double2 w[H];
 #pragma unroll
-    for (int k = 0; k < H; ++k) w[k] = blane ? ub[(lane - H) * H + k] : cz();
+    for (int k = 0; k < H; ++k) w[k] = blane ? ub[(lane - H) * HP + k] : cz();
     __syncthreads();
     const bool isW1 = w1lane && (PASSES == 1 || pass == 0);
     const bool isW2 = (PASSES == 1) ? w2lane1 : (pass == 1 && w1lane);
@@ -1168,14 +1176,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
 #pragma unroll
       for (int k = 0; k < H; ++k) rfma(acc, V0T[j * H + k], w[k]);
       if (isW1) {
-        sA[irow + j * H] = acc;
+        sA[irow + j * HP] = acc;
         rho_out[irow + j * H] = acc;
         if (MIX != MIX_FP64) {
           put_b<MIX, KP>(p.brho[sp], p.brho_lo[sp], irow + j * H, 2 * mcol, acc.x);
           put_b<MIX, KP>(p.brho[sp], p.brho_lo[sp], irow + j * H, 2 * mcol + 1, acc.y);
         }
       }
-      if (isW2) sA[2 * HH + irow + j * H] = cneg(acc);
+      if (isW2) sA[2 * MS + irow + j * HP] = cneg(acc);
     }
     if (PASSES == 1 || pass == 0) {
       // U0s sweep: kappa10 (W1 lanes)
@@ -1184,7 +1192,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
         double2 acc = cz();
 #pragma unroll
         for (int k = 0; k < H; ++k) rfma(acc, U0T[j * H + k], w[k]);
-        if (isW1) sA[HH + irow + j * H] = acc;
+        if (isW1) sA[MS + irow + j * HP] = acc;
       }
     }
     __syncthreads();
@@ -1192,21 +1200,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
 
   // ---- traces, <J^2> pieces, kinetic energy
   const double2 *ro = sA;
-  double2 *k10 = sA + HH, *k01 = sA + 2 * HH;
+  double2 *k10 = sA + MS, *k01 = sA + 2 * MS;
   const double *mz = p.mtab, *cp = p.mtab + H, *jj1 = p.mtab + 2 * H;
   const double *t1 = p.t1b + (size_t)sp * H * (n + 1);  // species block, ld n
   double2 tr = cz(), ek = cz(), aj = cz(), tp = cz(), tm = cz(), tz = cz();
   for (int e = lane; e < HH; e += 64) {
     const int a = e % H, b = e / H;
     const double ma = mz[a], mb = mz[b], ca = cp[a], cb = cp[b];
-    const double2 rab = ro[a + b * H];
+    const double2 rab = ro[a + b * HP];
     ek = cadd(ek, cscale(t1[b + (size_t)a * n], rab));
-    aj = csub(aj, cscale(ma * mb, cmul(rab, ro[b + a * H])));
-    aj = cadd(aj, cscale(ma * mb, cmul(k10[a + b * H], k01[a + b * H])));
+    aj = csub(aj, cscale(ma * mb, cmul(rab, ro[b + a * HP])));
+    aj = cadd(aj, cscale(ma * mb, cmul(k10[a + b * HP], k01[a + b * HP])));
     if (a > 0 && b > 0 && ca != 0.0 && cb != 0.0) {
-      aj = csub(aj, cscale(ca * cb, cmul(rab, ro[(b - 1) + (a - 1) * H])));
-      const double2 kk = cadd(cmul(k10[a + (b - 1) * H], k01[(a - 1) + b * H]),
-                              cmul(k10[(a - 1) + b * H], k01[a + (b - 1) * H]));
+      aj = csub(aj, cscale(ca * cb, cmul(rab, ro[(b - 1) + (a - 1) * HP])));
+      const double2 kk = cadd(cmul(k10[a + (b - 1) * HP], k01[(a - 1) + b * HP]),
+                              cmul(k10[(a - 1) + b * HP], k01[a + (b - 1) * HP]));
       aj = cadd(aj, cscale(0.5 * ca * cb, kk));
     }
     if (a == b) {
@@ -1222,12 +1230,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
   for (int e = lane; e < HH; e += 64) {
     const int a = e % H, b = e / H;
     if (a < b) {
-      const double2 x10 = csub(k10[a + b * H], k10[b + a * H]);
-      const double2 x01 = csub(k01[a + b * H], k01[b + a * H]);
-      k10[a + b * H] = x10;
-      k10[b + a * H] = cneg(x10);
-      k01[a + b * H] = x01;
-      k01[b + a * H] = cneg(x01);
+      const double2 x10 = csub(k10[a + b * HP], k10[b + a * HP]);
+      const double2 x01 = csub(k01[a + b * HP], k01[b + a * HP]);
+      k10[a + b * HP] = x10;
+      k10[b + a * HP] = cneg(x10);
+      k01[a + b * HP] = x01;
+      k01[b + a * HP] = cneg(x01);
     }
   }
   __syncthreads();
@@ -1237,9 +1245,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
     // the species list moves to k_vgemm: A01 (dot vector) and A10 (operand)
     double2 *a01 = p.a01 + obase * HH;
     for (int e = lane; e < HH; e += 64) {
-      a01[e] = A01[e];
-      put_b<MIX, KP>(p.ba10[sp], p.ba10_lo[sp], e, 2 * mcol, A10[e].x);
-      put_b<MIX, KP>(p.ba10[sp], p.ba10_lo[sp], e, 2 * mcol + 1, A10[e].y);
+      const int el = (e % H) + (e / H) * HP;
+      a01[e] = A01[el];
+      put_b<MIX, KP>(p.ba10[sp], p.ba10_lo[sp], e, 2 * mcol, A10[el].x);
+      put_b<MIX, KP>(p.ba10[sp], p.ba10_lo[sp], e, 2 * mcol + 1, A10[el].y);
     }
   }
   // ---- species list
@@ -1257,11 +1266,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
       if (q0 + 64 * u >= nme) break;
       const VEntry ve = vq[u];
       const int i = ve.idx & 255, j = (ve.idx >> 8) & 255, k = (ve.idx >> 16) & 255, l = ve.idx >> 24;
-      double2 s = csub(cmul(ro[k + i * H], ro[l + j * H]), cmul(ro[l + i * H], ro[k + j * H]));
-      s = cadd(s, cscale(0.25, cmul(A01[i + j * H], A10[k + l * H])));
+      double2 s = csub(cmul(ro[k + i * HP], ro[l + j * HP]), cmul(ro[l + i * HP], ro[k + j * HP]));
+      s = cadd(s, cscale(0.25, cmul(A01[i + j * HP], A10[k + l * HP])));
       if (k + l != i + j) {
-        s = cadd(s, csub(cmul(ro[i + k * H], ro[j + l * H]), cmul(ro[j + k * H], ro[i + l * H])));
-        s = cadd(s, cscale(0.25, cmul(A01[k + l * H], A10[i + j * H])));
+        s = cadd(s, csub(cmul(ro[i + k * HP], ro[j + l * HP]), cmul(ro[j + k * HP], ro[i + l * HP])));
+        s = cadd(s, cscale(0.25, cmul(A01[k + l * HP], A10[i + j * HP])));
       }
       e2b = cadd(e2b, cscale(ve.me, s));
     }
@@ -1679,9 +1688,17 @@ struct ProjParams {
   int *pair_status;          // [npairs]
 };
 
+// The phases e^{i K alpha_a} and e^{i K gamma_g} (K = -jmax..jmax) are tabled
+// in LDS (dynamic, (2 jmax + 1)(nalp + ngam) complex), so a term of the sum is
+// one complex product instead of a sine and cosine.
 __global__ __launch_bounds__(256) void k_project(ProjParams p) {
+  extern __shared__ double2 ktab[];
   const int ip = blockIdx.x, io = blockIdx.y, tid = threadIdx.x;
   const int J2m = 2 * p.jmax + 1;
+  double2 *ea = ktab, *eg = ktab + (size_t)J2m * p.nalp;
+  for (int i = tid; i < J2m * p.nalp; i += blockDim.x) ea[i] = cexpi((i / p.nalp - p.jmax) * p.alp[i % p.nalp]);
+  for (int i = tid; i < J2m * p.ngam; i += blockDim.x) eg[i] = cexpi((i / p.ngam - p.jmax) * p.gam[i % p.ngam]);
+  __syncthreads();
   const double2 *z = p.zres + (size_t)ip * p.NE * NOBS;
   double2 *kout = p.kern + ((size_t)ip * NOBS + io) * (p.jmax + 1) * J2m * J2m;
   const bool onept = (p.nalp * p.ngam == 1);
@@ -1706,15 +1723,15 @@ __global__ __launch_bounds__(256) void k_project(ProjParams p) {
           for (int ig = 0; ig < p.ngam; ++ig) {
             const double w = p.walp[ia] * p.wbet[ib] * p.wgam[ig];
             const int i1 = (ib * p.nalp + ia) * p.ngam + ig;
+            const double2 zap = ea[(K1 + p.jmax) * p.nalp + ia], zgp = eg[(K2 + p.jmax) * p.ngam + ig];
             if (p.fold == PNAMP_FOLD_PLAIN) {
-              const double2 D = cscale(dp * w * wJ, cexpi(K1 * p.alp[ia] + K2 * p.gam[ig]));
+              const double2 D = cscale(dp * w * wJ, cmul(zap, zgp));
               cfma(acc, D, z[(size_t)i1 * NOBS + io]);
             } else {
               const int i2 = (ib * p.nalp + ia) * p.ngam + (p.ngam - 1 - ig);
               const int i3 = (ib * p.nalp + (p.nalp - 1 - ia)) * p.ngam + ig;
               const int i4 = (ib * p.nalp + (p.nalp - 1 - ia)) * p.ngam + (p.ngam - 1 - ig);
-              const double2 zap = cexpi(K1 * p.alp[ia]), zam = cexpi(-K1 * p.alp[ia]);
-              const double2 zgp = cexpi(K2 * p.gam[ig]);
+              const double2 zam = cconj(zap);
               const double2 za1 = cscale(dp, cmul(zap, zgp)), za2 = cscale(dp, cmul(zam, zgp));
               const double2 za3 = cscale(dm, cmul(zap, zgp)), za4 = cscale(dm, cmul(zam, zgp));
               double2 s = cmul(za1, z[(size_t)i1 * NOBS + io]);
@@ -2097,6 +2114,8 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
   if (mesh->nalp < 1 || mesh->nbet < 1 || mesh->ngam < 1 || mesh->nphi < 1 || mesh->jmax < 0)
     return bail(PNAMP_E_ARG, "bad mesh sizes");
   if (mesh->jmax > JMAX_MAX) return bail(PNAMP_E_SIZE, "jmax too large");
+  if ((2 * (size_t)mesh->jmax + 1) * (mesh->nalp + mesh->ngam) * sizeof(double2) > 150 * 1024)
+    return bail(PNAMP_E_SIZE, "alpha and gamma meshes too large for the projection phase tables");
   if (mesh->fold != PNAMP_FOLD_PLAIN && mesh->fold != PNAMP_FOLD_LITERAL) return bail(PNAMP_E_ARG, "bad fold");
   if (inter->nme < 0 || inter->nme > (1 << 30)) return bail(PNAMP_E_ARG, "bad interaction size");
   c->device = device;
@@ -2334,6 +2353,16 @@ extern "C" int pnamp_state_overlaps(pnamp_ctx *c, double *det) {
   return PNAMP_OK;
 }
 
+static hipError_t launch_project(pnamp_ctx *c, int npairs, const ProjParams &pr) {
+  const size_t lds = sizeof(double2) * (2 * (size_t)c->jmax + 1) * (c->nalp + c->ngam);
+  if (lds > 48 * 1024) {
+    const hipError_t e = hipFuncSetAttribute((const void *)k_project, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_project, dim3(npairs, NOBS), dim3(256), lds, c->stream, pr);
+  return hipGetLastError();
+}
+
 static int ensure_pairs(pnamp_ctx *c, int np) {
   if (np <= c->pair_cap) return 0;
   int r = 0;
@@ -2515,8 +2544,7 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
   }
   ProjParams pr{c->nalp, c->nbet, c->ngam, c->NE, c->jmax, c->fold, c->npj, c->d_alp, c->d_walp, c->d_wbet,
                 c->d_gam, c->d_wgam, c->d_ptab, c->d_zres, c->d_status, c->d_kern, c->d_pair_status};
-  hipLaunchKernelGGL(k_project, dim3(npairs, NOBS), dim3(256), 0, c->stream, pr);
-  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, launch_project(c, npairs, pr));
   HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
   const size_t kper = (size_t)NOBS * (c->jmax + 1) * (2 * c->jmax + 1) * (2 * c->jmax + 1);
   if (kern)  // k_project writes every slot of the dense layout, zeros where |K| > J
@@ -2567,8 +2595,7 @@ extern "C" int pnamp_project_zres(pnamp_ctx *c, int npairs, const double *zres, 
   HIPCHK(c, hipMemsetAsync(c->d_status, 0, sizeof(int) * (size_t)npairs * c->NE, c->stream));
   ProjParams pr{c->nalp, c->nbet, c->ngam, c->NE, c->jmax, c->fold, c->npj, c->d_alp, c->d_walp, c->d_wbet,
                 c->d_gam, c->d_wgam, c->d_ptab, c->d_zres, c->d_status, c->d_kern, c->d_pair_status};
-  hipLaunchKernelGGL(k_project, dim3(npairs, NOBS), dim3(256), 0, c->stream, pr);
-  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, launch_project(c, npairs, pr));
   const size_t kper = (size_t)NOBS * (c->jmax + 1) * (2 * c->jmax + 1) * (2 * c->jmax + 1);
   HIPCHK(c, hipMemcpyAsync(kern, c->d_kern, sizeof(double2) * kper * npairs, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
