@@ -1293,8 +1293,13 @@ struct CombParams {
   double vac_scale, snpfaf, sgn_h, e0;
   const double *det;
   const int *q0, *q1;
-  const int *w_ptr, *w_col;         // W_pn in CSR over x = c + a h (rows), y = d + b h (columns)
-  const double *w_val;
+  // W_pn (rows x = c + a h, columns y = d + b h) in dense blocks of equal
+  // Delta m = m_c - m_a = -(m_d - m_b): the interaction conserves M
+  const int2 *pn_rows;              // [h^2] by block order: (x, block)
+  const int *pn_blk;                // [nblk][5]: first row position, rows, first column (in pn_ycol), columns, offset in pn_w
+  const int *pn_ycol;               // [h^2] column lists of the blocks
+  const double *pn_w;               // blocks, column-major
+  int pn_ncol;                      // length of pn_ycol
   const double2 *out, *rho;
   const int *fstatus;               // [npairs][NE][2][nphi]
   const float2 *gpn;                // mixed forms: G = W_pn rho_n from k_vgemm, [npairs][NE][nphi][h*h]
@@ -1314,7 +1319,7 @@ struct CombParams {
 // stay in registers.  With GMIX (mixed forms) G comes from k_vgemm instead.
 template <int NP, bool GMIX>
 __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
-  extern __shared__ double2 dsm[];  // rho_n -> G [NP][h^2]; rho_p is read from global memory
+  extern __shared__ double2 dsm[];  // rho_n -> G [NP][h^2], then the block column lists; rho_p from global memory
   __shared__ double2 epn[NP * NP];
   __shared__ double red[NTC / 64 * 2 * NOBS];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1322,6 +1327,8 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
   const int h = p.h, hh = h * h;
   const size_t base = ((size_t)ip * p.NE + iang) * 2;
   double2 *rn = dsm;
+  int *ycol = (int *)(dsm + (size_t)NP * hh);
+  for (int e = tid; e < p.pn_ncol; e += NTC) ycol[e] = p.pn_ycol[e];
   for (int e = tid; e < NP * hh; e += NTC) {
     if (GMIX) {
       const float2 gv = p.gpn[((size_t)ip * p.NE + iang) * NP * hh + e];
@@ -1332,33 +1339,40 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
   }
   __syncthreads();
   if (!GMIX) {
-  // G rows x = tid, tid + NTC (h^2 <= 2 NTC), all phi_n, in registers
+  // G rows in block order (row positions tid, tid + NTC; h^2 <= 2 NTC), all
+  // phi_n, in registers.  The lanes of one block walk the same column list,
+  // so each rho_n read is an LDS broadcast.
   constexpr int RPT = 2;
   double2 g[RPT][NP];
+  int xrow[RPT];
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
 #pragma unroll
     for (int f = 0; f < NP; ++f) g[r][f] = cz();
-    const int x = tid + r * NTC;
-    if (x < hh) {
-      const int qe = p.w_ptr[x + 1];
-      int q = p.w_ptr[x];
-      for (; q + 4 <= qe; q += 4) {  // four CSR entries loaded ahead of their use
-        int y[4];
+    const int px = tid + r * NTC;
+    xrow[r] = -1;
+    if (px < hh) {
+      const int2 rw = p.pn_rows[px];
+      const int *bk = p.pn_blk + 5 * rw.y;
+      const int nrows = bk[1], ncol = bk[3];
+      const int *yc = ycol + bk[2];
+      const double *wb = p.pn_w + bk[4] + (px - bk[0]);
+      xrow[r] = rw.x;
+      int k = 0;
+      for (; k + 4 <= ncol; k += 4) {  // four block entries loaded ahead of their use
         double v[4];
 #pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = wb[(size_t)(k + u) * nrows];
+#pragma unroll
         for (int u = 0; u < 4; ++u) {
-          y[u] = p.w_col[q + u];
-          v[u] = p.w_val[q + u];
+          const int y = yc[k + u];
+#pragma unroll
+          for (int f = 0; f < NP; ++f) rfma(g[r][f], v[u], rn[(size_t)f * hh + y]);
         }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int f = 0; f < NP; ++f) rfma(g[r][f], v[u], rn[(size_t)f * hh + y[u]]);
       }
-      for (; q < qe; ++q) {
-        const int y = p.w_col[q];
-        const double v = p.w_val[q];
+      for (; k < ncol; ++k) {
+        const double v = wb[(size_t)k * nrows];
+        const int y = yc[k];
 #pragma unroll
         for (int f = 0; f < NP; ++f) rfma(g[r][f], v, rn[(size_t)f * hh + y]);
       }
@@ -1367,8 +1381,8 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
-    const int x = tid + r * NTC;
-    if (x < hh) {
+    const int x = xrow[r];
+    if (x >= 0) {
 #pragma unroll
       for (int f = 0; f < NP; ++f) rn[(size_t)f * hh + x] = g[r][f];
     }
@@ -1928,8 +1942,10 @@ struct pnamp_ctx {
   double *d_mtab = nullptr;         // [3][h]: m, <m+1|J_+|m>, j(j+1) per species-local state
   double2 *d_fout = nullptr, *d_frho = nullptr, *d_sbuf = nullptr;
   int *d_fstatus = nullptr;
-  int *d_wptr = nullptr, *d_wcol = nullptr;
-  double *d_wval = nullptr;
+  int2 *d_pn_rows = nullptr;
+  int *d_pn_blk = nullptr, *d_pn_ycol = nullptr;
+  double *d_pn_w = nullptr;
+  int pn_ncol = 0;
   double2 *d_jpack = nullptr;
   size_t fac_cap = 0;  // pairs the factored scratch holds
   // captured launch sequence of the factored form, valid for (npairs, zval,
@@ -2207,7 +2223,7 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
       c->nme_s[k] = (int)ls[k].size();
       if ((r = dupload(c, &c->d_vs[k], ls[k].data(), ls[k].size()))) return bail(r, c->err);
     }
-    // W_pn as CSR: rows x = c + a h, columns y = d + b h (species-local indices)
+    // W_pn by rows x = c + a h, columns y = d + b h (species-local indices)
     const int hh = h * h;
     std::vector<std::vector<std::pair<int, double>>> rows(hh);
     auto add = [&](int x, int y, double v) {
@@ -2227,9 +2243,42 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
       for (auto &e : rows[x]) { wcol.push_back(e.first); wval.push_back(e.second); }
       wptr[x + 1] = (int)wcol.size();
     }
-    if ((r = dupload(c, &c->d_wptr, wptr.data(), wptr.size())) || (r = dupload(c, &c->d_wcol, wcol.data(), wcol.size())) ||
-        (r = dupload(c, &c->d_wval, wval.data(), wval.size())))
-      return bail(r, c->err);
+    // dense Delta-m blocks of W_pn for k_fac_combine: row x = c + a h has
+    // Delta m = m_c - m_a, column y = d + b h has m_d - m_b, and a non-zero
+    // needs Delta m(x) = -Delta m(y)
+    {
+      auto dm = [&](int idx) { return basis->twom[idx % h] - basis->twom[idx / h]; };
+      std::vector<int> dvals;
+      for (int x = 0; x < hh; ++x) dvals.push_back(dm(x));
+      std::sort(dvals.begin(), dvals.end());
+      dvals.erase(std::unique(dvals.begin(), dvals.end()), dvals.end());
+      std::vector<int2> prow;
+      std::vector<int> blk, ycol;
+      std::vector<double> wblk;
+      for (size_t bi = 0; bi < dvals.size(); ++bi) {
+        const int D = dvals[bi];
+        std::vector<int> xs, ys;
+        for (int x = 0; x < hh; ++x) if (dm(x) == D) xs.push_back(x);
+        for (int y = 0; y < hh; ++y) if (dm(y) == -D) ys.push_back(y);
+        blk.insert(blk.end(), {(int)prow.size(), (int)xs.size(), (int)ycol.size(), (int)ys.size(), (int)wblk.size()});
+        const size_t w0 = wblk.size();
+        wblk.resize(w0 + xs.size() * ys.size(), 0.0);
+        for (size_t i = 0; i < xs.size(); ++i) {
+          for (const auto &e : rows[xs[i]]) {
+            const auto it = std::lower_bound(ys.begin(), ys.end(), e.first);
+            if (it == ys.end() || *it != e.first)
+              return bail(PNAMP_E_ARG, "proton-neutron interaction entry does not conserve M");
+            wblk[w0 + (size_t)(it - ys.begin()) * xs.size() + i] = e.second;
+          }
+          prow.push_back(make_int2(xs[i], (int)bi));
+        }
+        ycol.insert(ycol.end(), ys.begin(), ys.end());
+      }
+      c->pn_ncol = (int)ycol.size();
+      if ((r = dupload(c, &c->d_pn_rows, prow.data(), prow.size())) || (r = dupload(c, &c->d_pn_blk, blk.data(), blk.size())) ||
+          (r = dupload(c, &c->d_pn_ycol, ycol.data(), ycol.size())) || (r = dupload(c, &c->d_pn_w, wblk.data(), wblk.size())))
+        return bail(r, c->err);
+    }
     // dense forms of the mixed-precision contraction (rows x, columns y, x and
     // y = species-local pair index c + a h of rho(c, a)): for each species entry
     // (i j | V | k l) with d = [k + l != i + j]
@@ -2407,7 +2456,7 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
     const int mix = (c->form == PNAMP_FORM_FACTORED) ? MIX_FP64 : (c->form == PNAMP_FORM_MIXED_FP32 ? MIX_FP32 : MIX_BF16);
     const int h = c->h, hh = h * h, kp = c->kp;
     if (c->nphi > NPHI_MAX) return fail(c, PNAMP_E_SIZE, "factored form supports nphi <= 16");
-    const size_t comb_lds = sizeof(double2) * (size_t)c->nphi * hh;
+    const size_t comb_lds = sizeof(double2) * (size_t)c->nphi * hh + sizeof(int) * (size_t)c->pn_ncol;
     if (comb_lds > 150 * 1024) return fail(c, PNAMP_E_SIZE, "nphi h^2 complex exceeds the combine LDS");
     if (hh > 2 * NTC) return fail(c, PNAMP_E_SIZE, "h^2 exceeds the combine rows per workgroup");
     if (h > HMAX || (h & 1)) return fail(c, PNAMP_E_SIZE, "factored form: species size h must be even and <= 22");
@@ -2485,7 +2534,8 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
     CombParams cp;
     cp.h = h; cp.NE = c->NE; cp.zval = zval; cp.nval = nval;
     cp.vac_scale = c->vac_scale; cp.snpfaf = c->snpfaf; cp.sgn_h = (h & 1) ? -1.0 : 1.0; cp.e0 = c->e0;
-    cp.det = c->d_det; cp.w_ptr = c->d_wptr; cp.w_col = c->d_wcol; cp.w_val = c->d_wval;
+    cp.det = c->d_det; cp.pn_rows = c->d_pn_rows; cp.pn_blk = c->d_pn_blk; cp.pn_ycol = c->d_pn_ycol;
+    cp.pn_w = c->d_pn_w; cp.pn_ncol = c->pn_ncol;
     cp.out = c->d_fout; cp.rho = c->d_frho; cp.fstatus = c->d_fstatus; cp.gpn = (const float2 *)c->d_gpn;
     cp.samples = nullptr;
     if (c->fac_graph && (c->g_np != npairs || c->g_z != zval || c->g_n != nval || c->g_form != c->form))
@@ -2706,7 +2756,8 @@ extern "C" void pnamp_destroy(pnamp_ctx *c) {
                   c->d_vlist, c->d_t1b, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status, c->d_q0,
                   c->d_q1, c->d_zres, c->d_kern, c->d_status, c->d_pair_status, c->d_vs[0], c->d_vs[1],
                   c->d_vs[2], c->d_Uf, c->d_Vf, c->d_UfT, c->d_VfT, c->d_mtab, c->d_fout, c->d_frho,
-                  c->d_sbuf, c->d_fstatus, c->d_wptr, c->d_wcol, c->d_wval, c->d_jpack};
+                  c->d_sbuf, c->d_fstatus, c->d_jpack, c->d_pn_rows,
+                  c->d_pn_blk, c->d_pn_ycol, c->d_pn_w};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (int q = 0; q < NVPROB; ++q) {
