@@ -63,6 +63,12 @@ __device__ __forceinline__ void cfma(double2 &acc, double2 a, double2 b) {
   acc.y = fma(a.x, b.y, acc.y);
   acc.y = fma(a.y, b.x, acc.y);
 }
+__device__ __forceinline__ void cfms(double2 &acc, double2 a, double2 b) {  // acc -= a * b
+  acc.x = fma(-a.x, b.x, acc.x);
+  acc.x = fma(a.y, b.y, acc.x);
+  acc.y = fma(-a.x, b.y, acc.y);
+  acc.y = fma(-a.y, b.x, acc.y);
+}
 __device__ __forceinline__ void rfma(double2 &acc, double a, double2 b) {  // acc += a * b, a real
   acc.x = fma(a, b.x, acc.x);
   acc.y = fma(a, b.y, acc.y);
@@ -70,6 +76,17 @@ __device__ __forceinline__ void rfma(double2 &acc, double a, double2 b) {  // ac
 __device__ __forceinline__ double2 crecip(double2 a) {
   double d = a.x * a.x + a.y * a.y;
   return make_double2(a.x / d, -a.y / d);
+}
+__device__ __forceinline__ double2 crecip1(double2 a) {  // one division
+  const double r = 1.0 / (a.x * a.x + a.y * a.y);
+  return make_double2(a.x * r, -a.y * r);
+}
+// Read-only view in the constant address space: wave-uniform reads through it
+// become scalar loads (a generic pointer that stores in the kernel might
+// alias is read with per-lane vector loads)
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T *ro_const(const T *q) {
+  return (const __attribute__((address_space(4))) T *)q;
 }
 __device__ __forceinline__ double cabs1(double2 a) { return fabs(a.x) + fabs(a.y); }
 __device__ __forceinline__ double2 cexpi(double t) {
@@ -833,8 +850,13 @@ struct PhiParams {
   const double *Z;                  // [nstates][n*n]
   const double *t1b;                // n x n
   const double *mtab;               // [3][h]: m, <m+1|J_+|m> (0 at m = j), j(j+1) per species-local state
-  const VEntry *vl0, *vl1;          // species lists with local indices (p, n)
-  int nme0, nme1;
+  // species lists (p, n), one term per entry: LDS byte offsets of rho(k,i),
+  // rho(l,j), rho(l,i), rho(k,j), A01(i,j), A10(k,l)/4 as 16-bit pairs in
+  // x, y, z; the matrix element in lme; padded with zero terms to a multiple
+  // of 256
+  const uint4 *lofs[2];
+  const double *lme[2];
+  int nl[2];
   const int *q0, *q1;
   double2 *out;                     // [npairs][NE][2][nphi][FAC_OUT]
   double2 *rho;                     // [npairs][NE][2][nphi][h*h], column-major
@@ -948,11 +970,13 @@ __device__ int wave_gauss_jordan(double2 (&col)[H], double2 *fc, int *qmap) {
     if (lane == 0 && k > 0) qmap[k] = q;
     __syncthreads();
     const double2 pv = fc[k];
-    if (pv.x == 0.0 && pv.y == 0.0 && info == 0) info = k + 1;
-    const double2 prow = cmul(col[k], crecip(pv));
+    // tested here (uniform) rather than at the INFO store: a test sunk to
+    // the use keeps every pivot alive in registers
+    if (__builtin_amdgcn_readfirstlane((pv.x == 0.0 && pv.y == 0.0) ? 1 : 0) && info == 0) info = k + 1;
+    const double2 prow = cmul(col[k], crecip1(pv));
 #pragma unroll
     for (int r = 0; r < H; ++r)
-      if (r != k) col[r] = csub(col[r], cmul(fc[r], prow));
+      if (r != k) cfms(col[r], fc[r], prow);
     col[k] = prow;
     __syncthreads();
   }
@@ -997,26 +1021,29 @@ __device__ int wave_gauss_jordan(double2 (&col)[H], double2 *fc, int *qmap) {
 //
 // The work is split in two launches over the same grid: PART_PF the Pfaffian
 // (a few KB of LDS and ~100 VGPRs, so many waves per SIMD hide its serial
-// steps), PART_REST the rest (21 KB of LDS for rho / kappa at h = 20).
+// steps), PART_REST the rest (13 KB of LDS for rho and one kappa / A block at
+// h = 20, so three waves per SIMD).
 constexpr int PART_PF = 1, PART_REST = 2;
 
 template <int H, int MIX, int PART>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART_PF ? 5 : 2))) void k_fac_phi(PhiParams p) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART_PF ? 5 : (3 * H <= 64 ? 3 : 2)))) void k_fac_phi(PhiParams p) {
   constexpr int N2 = 2 * H, NU = N2 * (N2 - 1) / 2, SL = (NU + 63) / 64, HH = H * H;
   constexpr int KP = (HH + 31) / 32 * 32;  // V-contraction K padded to the MFMA tile
   constexpr int PASSES = (3 * H <= 64) ? 1 : 2;
   constexpr int HP = H + 1, MS = H * HP;  // LDS leading dimension h + 1: transposed reads avoid bank conflicts
-  __shared__ double2 sA[3 * MS];  // rho, kappa10, kappa01 (ld h + 1); Gauss-Jordan un-permute buffer before
+  // B0 = sA[0, MS): rho (ld h + 1; transposed reads avoid bank conflicts);
+  // B1 = sA[MS, MS + h^2) (ld h): Gauss-Jordan pivot column and unknown map
+  // (one pass), kappa01, kappa10, then the A-block A01 (strict upper) and
+  // A10 / 4 (strict lower triangle); 13 KB at h = 20, twelve waves per CU
+  __shared__ double2 sA[MS + HH];
   __shared__ double2 colR[N2], colA[N2], colC[N2];
-  // Gauss-Jordan pivot column and unknown map: inside the rho block of sA in
-  // one pass (rho is written after they are last read), separate in two
-  // passes (rho of pass 0 must survive pass 1); keeps h = 20 within 20 KB,
-  // eight waves per CU
+  // in two passes kappa01 of pass 0 holds B1 through pass 1
   constexpr bool GJ_IN_SA = (PASSES == 1);
   __shared__ double2 fc_sep[GJ_IN_SA ? 1 : H];
   __shared__ int qmap_sep[GJ_IN_SA ? 1 : H];
-  double2 *fc = GJ_IN_SA ? sA : fc_sep;
-  int *qmap = GJ_IN_SA ? (int *)(sA + H) : qmap_sep;
+  double2 *const B0 = sA, *const B1 = sA + MS;
+  double2 *fc = GJ_IN_SA ? B1 : fc_sep;
+  int *qmap = GJ_IN_SA ? (int *)(B1 + H) : qmap_sep;
   const int lane = threadIdx.x;
   // XCD-aware order: consecutive logical indices (the nphi waves of one setup
   // block) stay on one XCD and its L2; gridDim.x is a multiple of 8.
@@ -1032,7 +1059,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
   const size_t obase = g * nphi + f;
   const double2 *X = p.sbuf + g * 5 * HH, *Y = X + HH, *RV1 = X + 2 * HH, *RcU1 = X + 3 * HH, *M0 = X + 4 * HH;
   const int q0 = p.q0[ip];
-  const double *V0T = p.VfT + ((size_t)q0 * 2 + sp) * HH, *U0T = p.UfT + ((size_t)q0 * 2 + sp) * HH;
+  const auto V0T = ro_const(p.VfT + ((size_t)q0 * 2 + sp) * HH), U0T = ro_const(p.UfT + ((size_t)q0 * 2 + sp) * HH);
   const double *Z0 = p.Z + (size_t)q0 * n * n + (size_t)sp * H * (n + 1);  // species block, ld n
   const double phi = (nphi == 1) ? 0.0 : (2.0 * M_PI / nphi) * f;
   const double2 e1 = cexpi(phi), e2 = cexpi(2.0 * phi);
@@ -1129,23 +1156,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
   if (lane == 0) p.out[obase * FAC_OUT] = pf;
   } else {
   // ---- Gauss-Jordan and the density products
+  // One pass: T^T in lanes 0..H-1, RV1 (-> W1) in H..2H-1, RcU1 (-> W2) in
+  // 2H..3H-1.  Two passes: pass 0 RcU1 (-> W2), pass 1 RV1 (-> W1), both in
+  // lanes H..2H-1.  W1 rows give rho (B0) and kappa10, W2 rows kappa01 (B1).
   int info = 0;
+  constexpr int K01L = (PASSES == 1) ? 2 * H : H;  // first lane of the W2 rows
   const bool w1lane = lane >= H && lane < 2 * H;
-  const bool w2lane1 = lane >= 2 * H && lane < 3 * H;  // PASSES == 1 only
-  const int irow = w2lane1 ? lane - 2 * H : lane - H;
+  const int irow = (lane >= K01L && lane < K01L + H && PASSES == 1) ? lane - K01L : lane - H;
   double2 *rho_out = p.rho + obase * HH;
   const size_t mcol = ((size_t)ip * p.NE + iang) * nphi + f;  // GEMM column pair of this phi in species sp
-#pragma unroll 1
+  constexpr int NE64 = (HH + 63) / 64;
+  double2 a01r[NE64];  // A01 = kappa01 - kappa01^T, strict upper triangle
+  double2 ajk = cz();  // kappa part of <J^2>
+  const auto mz = ro_const(p.mtab), cp = ro_const(p.mtab + H), jj1 = ro_const(p.mtab + 2 * H);
+#pragma unroll
   for (int pass = 0; pass < PASSES; ++pass) {
+    const bool isW1 = w1lane && (PASSES == 1 || pass == 1);
+    const bool isW2 = (PASSES == 1) ? (lane >= K01L && lane < K01L + H) : (pass == 0 && w1lane);
     double2 col[H];
     if (lane < H) {
 #pragma unroll
       for (int r = 0; r < H; ++r)
         col[r] = cadd(cmul(e1, X[lane + r * H]), cmul(cconj(e1), Y[lane + r * H]));
-    } else if (w1lane || (PASSES == 1 && w2lane1)) {
-      const bool second = (PASSES == 1) ? w2lane1 : (pass == 1);
-      const double2 *src = second ? RcU1 : RV1;
-      const double2 ph = second ? cconj(e1) : e1;
+    } else if (isW1 || isW2) {
+      const double2 *src = isW2 ? RcU1 : RV1;
+      const double2 ph = isW2 ? cconj(e1) : e1;
 #pragma unroll
       for (int r = 0; r < H; ++r) col[r] = cmul(ph, src[irow + r * H]);
     } else {
@@ -1154,69 +1189,85 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
     }
     const int inf = wave_gauss_jordan<H>(col, fc, qmap);
     if (inf && !info) info = inf;
-    // solution rows in unknown order through LDS: unknown qmap[k] is row k
-    constexpr int NB = (PASSES == 1) ? 2 * H : H;
-    double2 *ub = (PASSES == 1) ? sA + MS : sA + 2 * MS;
-    const bool blane = lane >= H && lane < H + NB;
-    if (blane) {
+    // solution rows in unknown order through B0, h rows at a time (unknown
+    // qmap[k] is row k), back into col: w[k] is W(irow, k) on the W lanes
 #pragma unroll
-      for (int r = 0; r < H; ++r) ub[(lane - H) * HP + qmap[r]] = col[r];
+    for (int g = 0; g < ((PASSES == 1) ? 2 : 1); ++g) {
+      const bool gl = lane >= H * (g + 1) && lane < H * (g + 2);
+      if (gl) {
+#pragma unroll
+        for (int r = 0; r < H; ++r) B0[(lane - H * (g + 1)) * HP + qmap[r]] = col[r];
+      }
+      __syncthreads();
+      if (gl) {
+#pragma unroll
+        for (int k = 0; k < H; ++k) col[k] = B0[(lane - H * (g + 1)) * HP + k];
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    double2 w[H];
-#pragma unroll
-    for (int k = 0; k < H; ++k) w[k] = blane ? ub[(lane - H) * HP + k] : cz();
-    __syncthreads();
-    const bool isW1 = w1lane && (PASSES == 1 || pass == 0);
-    const bool isW2 = (PASSES == 1) ? w2lane1 : (pass == 1 && w1lane);
-    // V0s sweep: rho (W1 lanes), kappa01 (W2 lanes)
+    const double2 (&w)[H] = col;
+    // V0s sweep: rho (W1 rows) -> B0 and HBM, kappa01 (W2 rows) -> B1
 #pragma unroll 2
     for (int j = 0; j < H; ++j) {
       double2 acc = cz();
 #pragma unroll
       for (int k = 0; k < H; ++k) rfma(acc, V0T[j * H + k], w[k]);
       if (isW1) {
-        sA[irow + j * HP] = acc;
+        B0[irow + j * HP] = acc;
         rho_out[irow + j * H] = acc;
         if (MIX != MIX_FP64) {
           put_b<MIX, KP>(p.brho[sp], p.brho_lo[sp], irow + j * H, 2 * mcol, acc.x);
           put_b<MIX, KP>(p.brho[sp], p.brho_lo[sp], irow + j * H, 2 * mcol + 1, acc.y);
         }
       }
-      if (isW2) sA[2 * MS + irow + j * HP] = cneg(acc);
+      if (isW2) B1[irow + j * H] = cneg(acc);
     }
-    if (PASSES == 1 || pass == 0) {
-      // U0s sweep: kappa10 (W1 lanes)
+    __syncthreads();
+    if (PASSES == 1 || pass == 1) {
+#pragma unroll
+      for (int u = 0; u < NE64; ++u) {
+        const int e = lane + 64 * u, a = e % H, b = e / H;
+        a01r[u] = (e < HH && a < b) ? csub(B1[a + b * H], B1[b + a * H]) : cz();
+      }
+      // U0s sweep: kappa10 (W1 rows) with the kappa part of <J^2>,
+      //   sum_ab kappa10(a,b) [m_a m_b kappa01(a,b) + c_a c_{b+1} kappa01(a-1,b+1) / 2
+      //                        + c_{a+1} c_b kappa01(a+1,b-1) / 2]
+      // (c_a = <m_a + 1|J_+|m_a>, 0 at m_a = j_a, so a - 1 is in a's orbit
+      // when c_a != 0); kappa10(a, b - 1) replaces kappa01(a, b - 1) in B1 one
+      // column late, after the last read of that column
+      const int a = w1lane ? irow : 0;
+      const double ma = mz[a], ca = cp[a], ca1 = (a + 1 < H) ? cp[a + 1] : 0.0;
+      const int am = (a > 0) ? a - 1 : 0, ap = (a + 1 < H) ? a + 1 : a;
+      double2 prev = cz();
 #pragma unroll 2
       for (int j = 0; j < H; ++j) {
         double2 acc = cz();
 #pragma unroll
         for (int k = 0; k < H; ++k) rfma(acc, U0T[j * H + k], w[k]);
-        if (isW1) sA[MS + irow + j * HP] = acc;
+        if (w1lane) {
+          double2 x = cscale(ma * mz[j], B1[a + j * H]);
+          if (j + 1 < H) x = cadd(x, cscale(0.5 * ca * cp[j + 1], B1[am + (j + 1) * H]));
+          if (j > 0) x = cadd(x, cscale(0.5 * ca1 * cp[j], B1[ap + (j - 1) * H]));
+          cfma(ajk, acc, x);
+          if (j > 0) B1[a + (j - 1) * H] = prev;
+          prev = acc;
+        }
       }
+      if (w1lane) B1[a + (H - 1) * H] = prev;
+      __syncthreads();
     }
-    __syncthreads();
   }
 
-  // ---- traces, <J^2> pieces, kinetic energy
-  const double2 *ro = sA;
-  double2 *k10 = sA + MS, *k01 = sA + 2 * MS;
-  const double *mz = p.mtab, *cp = p.mtab + H, *jj1 = p.mtab + 2 * H;
+  // ---- traces, the rho part of <J^2>, kinetic energy
   const double *t1 = p.t1b + (size_t)sp * H * (n + 1);  // species block, ld n
-  double2 tr = cz(), ek = cz(), aj = cz(), tp = cz(), tm = cz(), tz = cz();
+  double2 tr = cz(), ek = cz(), aj = ajk, tp = cz(), tm = cz(), tz = cz();
   for (int e = lane; e < HH; e += 64) {
     const int a = e % H, b = e / H;
     const double ma = mz[a], mb = mz[b], ca = cp[a], cb = cp[b];
-    const double2 rab = ro[a + b * HP];
+    const double2 rab = B0[a + b * HP];
     ek = cadd(ek, cscale(t1[b + (size_t)a * n], rab));
-    aj = csub(aj, cscale(ma * mb, cmul(rab, ro[b + a * HP])));
-    aj = cadd(aj, cscale(ma * mb, cmul(k10[a + b * HP], k01[a + b * HP])));
-    if (a > 0 && b > 0 && ca != 0.0 && cb != 0.0) {
-      aj = csub(aj, cscale(ca * cb, cmul(rab, ro[(b - 1) + (a - 1) * HP])));
-      const double2 kk = cadd(cmul(k10[a + (b - 1) * HP], k01[(a - 1) + b * HP]),
-                              cmul(k10[(a - 1) + b * HP], k01[a + (b - 1) * HP]));
-      aj = cadd(aj, cscale(0.5 * ca * cb, kk));
-    }
+    aj = csub(aj, cscale(ma * mb, cmul(rab, B0[b + a * HP])));
+    if (a > 0 && b > 0 && ca != 0.0 && cb != 0.0) aj = csub(aj, cscale(ca * cb, cmul(rab, B0[(b - 1) + (a - 1) * HP])));
     if (a == b) {
       tr = cadd(tr, rab);
       tz = cadd(tz, cscale(ma, rab));
@@ -1225,54 +1276,59 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
     if (a == b + 1) tp = cadd(tp, cscale(ca, rab));
     if (b == a + 1) tm = cadd(tm, cscale(cb, rab));
   }
+  // A10 / 4 = (kappa10 - kappa10^T) / 4, then the A-block
+  double2 a10r[NE64];
+#pragma unroll
+  for (int u = 0; u < NE64; ++u) {
+    const int e = lane + 64 * u, a = e % H, b = e / H;
+    a10r[u] = (e < HH && a < b) ? cscale(0.25, csub(B1[a + b * H], B1[b + a * H])) : cz();
+  }
   __syncthreads();
-  // kappa -> A = kappa - kappa^T in place
-  for (int e = lane; e < HH; e += 64) {
-    const int a = e % H, b = e / H;
-    if (a < b) {
-      const double2 x10 = csub(k10[a + b * HP], k10[b + a * HP]);
-      const double2 x01 = csub(k01[a + b * HP], k01[b + a * HP]);
-      k10[a + b * HP] = x10;
-      k10[b + a * HP] = cneg(x10);
-      k01[a + b * HP] = x01;
-      k01[b + a * HP] = cneg(x01);
+#pragma unroll
+  for (int u = 0; u < NE64; ++u) {
+    const int e = lane + 64 * u, a = e % H, b = e / H;
+    if (e < HH && a < b) {
+      B1[a + b * H] = a01r[u];
+      B1[b + a * H] = a10r[u];
     }
   }
   __syncthreads();
   double2 e2b = cz();
-  const double2 *A10 = k10, *A01 = k01;
   if (MIX != MIX_FP64) {
     // the species list moves to k_vgemm: A01 (dot vector) and A10 (operand)
     double2 *a01 = p.a01 + obase * HH;
     for (int e = lane; e < HH; e += 64) {
-      const int el = (e % H) + (e / H) * HP;
-      a01[e] = A01[el];
-      put_b<MIX, KP>(p.ba10[sp], p.ba10_lo[sp], e, 2 * mcol, A10[el].x);
-      put_b<MIX, KP>(p.ba10[sp], p.ba10_lo[sp], e, 2 * mcol + 1, A10[el].y);
+      const int a = e % H, b = e / H;
+      const double2 x01 = (a < b) ? B1[a + b * H] : (a > b ? cneg(B1[b + a * H]) : cz());
+      const double2 x10 = (a < b) ? cscale(4.0, B1[b + a * H]) : (a > b ? cscale(-4.0, B1[a + b * H]) : cz());
+      a01[e] = x01;
+      put_b<MIX, KP>(p.ba10[sp], p.ba10_lo[sp], e, 2 * mcol, x10.x);
+      put_b<MIX, KP>(p.ba10[sp], p.ba10_lo[sp], e, 2 * mcol + 1, x10.y);
     }
-  }
-  // ---- species list
-  const VEntry *vl = sp ? p.vl1 : p.vl0;
-  const int nme = (MIX == MIX_FP64) ? (sp ? p.nme1 : p.nme0) : 0;
-  for (int q0 = lane; q0 < nme; q0 += 4 * 64) {
-    VEntry vq[4];  // four entries loaded ahead of their LDS reads (same summation order)
+  } else {
+    // ---- species list: V [rho_ki rho_lj - rho_li rho_kj + A01_ij A10_kl / 4]
+    const uint4 *lo = p.lofs[sp];
+    const double *lm = p.lme[sp];
+    const int nl = p.nl[sp];
+    const char *lds = (const char *)sA;
+    auto ld = [&](unsigned off) { return *(const double2 *)(lds + off); };
+    for (int q0 = lane; q0 < nl; q0 += 4 * 64) {
+      uint4 o[4];  // four entries loaded ahead of their LDS reads
+      double me[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int q = q0 + 64 * u;
-      vq[u] = (q < nme) ? vl[q] : VEntry{0.0, 0u, 0u};
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (q0 + 64 * u >= nme) break;
-      const VEntry ve = vq[u];
-      const int i = ve.idx & 255, j = (ve.idx >> 8) & 255, k = (ve.idx >> 16) & 255, l = ve.idx >> 24;
-      double2 s = csub(cmul(ro[k + i * HP], ro[l + j * HP]), cmul(ro[l + i * HP], ro[k + j * HP]));
-      s = cadd(s, cscale(0.25, cmul(A01[i + j * HP], A10[k + l * HP])));
-      if (k + l != i + j) {
-        s = cadd(s, csub(cmul(ro[i + k * HP], ro[j + l * HP]), cmul(ro[j + k * HP], ro[i + l * HP])));
-        s = cadd(s, cscale(0.25, cmul(A01[k + l * HP], A10[i + j * HP])));
+      for (int u = 0; u < 4; ++u) {
+        o[u] = lo[q0 + 64 * u];
+        me[u] = lm[q0 + 64 * u];
       }
-      e2b = cadd(e2b, cscale(ve.me, s));
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double2 r1 = ld(o[u].x & 0xffffu), r2 = ld(o[u].x >> 16), r3 = ld(o[u].y & 0xffffu),
+                      r4 = ld(o[u].y >> 16), x = ld(o[u].z & 0xffffu), y = ld(o[u].z >> 16);
+        double2 t = cmul(r1, r2);
+        cfms(t, r3, r4);
+        cfma(t, x, y);
+        rfma(e2b, me[u], t);
+      }
     }
   }
   double v[14] = {e2b.x, e2b.y, ek.x, ek.y, tr.x, tr.y, aj.x, aj.y, tp.x, tp.y, tm.x, tm.y, tz.x, tz.y};
@@ -1937,7 +1993,11 @@ struct pnamp_ctx {
   int form = PNAMP_FORM_PLAIN;
   bool separable = false;
   int nme_s[3] = {0, 0, 0};
-  VEntry *d_vs[3] = {nullptr, nullptr, nullptr};  // pp, nn (local indices), pn
+  VEntry *d_vs[3] = {nullptr, nullptr, nullptr};  // unused
+  uint4 *d_lofs[2] = {nullptr, nullptr};            // species lists of k_fac_phi (p, n)
+  double *d_lme[2] = {nullptr, nullptr};
+  int nl[2] = {0, 0};
+  bool fac_list_ok = false;
   double *d_Uf = nullptr, *d_Vf = nullptr, *d_UfT = nullptr, *d_VfT = nullptr;
   double *d_mtab = nullptr;         // [3][h]: m, <m+1|J_+|m>, j(j+1) per species-local state
   double2 *d_fout = nullptr, *d_frho = nullptr, *d_sbuf = nullptr;
@@ -2219,9 +2279,37 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
         return bail(PNAMP_E_ARG, "interaction entry violates isospin-projection conservation");
       }
     }
-    for (int k = 0; k < 3; ++k) {
-      c->nme_s[k] = (int)ls[k].size();
-      if ((r = dupload(c, &c->d_vs[k], ls[k].data(), ls[k].size()))) return bail(r, c->err);
+    for (int k = 0; k < 3; ++k) c->nme_s[k] = (int)ls[k].size();
+    // pp and nn lists for k_fac_phi: one term per entry, the (kl)(ij) partner
+    // of an entry with a + b != c + d as an entry of its own, LDS byte
+    // offsets precomputed (rho at ld h + 1, then the A-block at ld h)
+    c->fac_list_ok = true;
+    for (int k = 0; k < 2; ++k) {
+      std::vector<uint4> lo;
+      std::vector<double> lm;
+      const unsigned hp = h + 1, msz = h * (h + 1);
+      auto rb = [&](int row, int col) { return (unsigned)(row + col * hp) * 16u; };
+      auto ab = [&](int row, int col) { return (msz + (unsigned)(row + col * h)) * 16u; };
+      auto term = [&](int i, int j, int kk, int l, double me) {
+        lo.push_back(make_uint4(rb(kk, i) | (rb(l, j) << 16), rb(l, i) | (rb(kk, j) << 16), ab(i, j) | (ab(l, kk) << 16), 0u));
+        lm.push_back(me);
+      };
+      for (const VEntry &e : ls[k]) {
+        const int i = e.idx & 255, j = (e.idx >> 8) & 255, kk = (e.idx >> 16) & 255, l = e.idx >> 24;
+        if (!(i < j && kk < l)) {  // the A-block holds A01 and A10 for index pairs in increasing order only
+          c->fac_list_ok = false;
+          break;
+        }
+        term(i, j, kk, l, e.me);
+        if (kk + l != i + j) term(kk, l, i, j, e.me);
+      }
+      while (lo.size() % 256) {
+        lo.push_back(make_uint4(0u, 0u, 0u, 0u));
+        lm.push_back(0.0);
+      }
+      c->nl[k] = (int)lo.size();
+      if ((r = dupload(c, &c->d_lofs[k], lo.data(), lo.size())) || (r = dupload(c, &c->d_lme[k], lm.data(), lm.size())))
+        return bail(r, c->err);
     }
     // W_pn by rows x = c + a h, columns y = d + b h (species-local indices)
     const int hh = h * h;
@@ -2453,6 +2541,8 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
   if (c->form != PNAMP_FORM_PLAIN) {
     if (!c->separable)
       return fail(c, PNAMP_E_ARG, "factored and mixed forms need states without proton-neutron mixing");
+    if (!c->fac_list_ok)
+      return fail(c, PNAMP_E_ARG, "factored and mixed forms need list entries with a < b and c < d");
     const int mix = (c->form == PNAMP_FORM_FACTORED) ? MIX_FP64 : (c->form == PNAMP_FORM_MIXED_FP32 ? MIX_FP32 : MIX_BF16);
     const int h = c->h, hh = h * h, kp = c->kp;
     if (c->nphi > NPHI_MAX) return fail(c, PNAMP_E_SIZE, "factored form supports nphi <= 16");
@@ -2509,7 +2599,9 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
     PhiParams fp;
     fp.n = c->n; fp.nphi = c->nphi; fp.NE = c->NE; fp.cscale = c->cscale; fp.sbuf = c->d_sbuf;
     fp.UfT = c->d_UfT; fp.VfT = c->d_VfT; fp.Z = c->d_Z; fp.t1b = c->d_t1b; fp.mtab = c->d_mtab;
-    fp.vl0 = c->d_vs[0]; fp.vl1 = c->d_vs[1]; fp.nme0 = c->nme_s[0]; fp.nme1 = c->nme_s[1];
+    for (int q = 0; q < 2; ++q) {
+      fp.lofs[q] = c->d_lofs[q]; fp.lme[q] = c->d_lme[q]; fp.nl[q] = c->nl[q];
+    }
     fp.out = c->d_fout; fp.rho = c->d_frho; fp.status = c->d_fstatus;
     for (int q = 0; q < 2; ++q) {
       fp.brho[q] = c->d_brho[q]; fp.brho_lo[q] = c->d_brho_lo[q];
@@ -2755,7 +2847,8 @@ extern "C" void pnamp_destroy(pnamp_ctx *c) {
                   c->d_pk_a, c->d_pk_b, c->d_blk_off, c->d_blk_sz, c->d_blk_base, c->d_twoj, c->d_twom,
                   c->d_vlist, c->d_t1b, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status, c->d_q0,
                   c->d_q1, c->d_zres, c->d_kern, c->d_status, c->d_pair_status, c->d_vs[0], c->d_vs[1],
-                  c->d_vs[2], c->d_Uf, c->d_Vf, c->d_UfT, c->d_VfT, c->d_mtab, c->d_fout, c->d_frho,
+                  c->d_vs[2], c->d_lofs[0], c->d_lofs[1], c->d_lme[0], c->d_lme[1], c->d_Uf, c->d_Vf,
+                  c->d_UfT, c->d_VfT, c->d_mtab, c->d_fout, c->d_frho,
                   c->d_sbuf, c->d_fstatus, c->d_jpack, c->d_pn_rows,
                   c->d_pn_blk, c->d_pn_ycol, c->d_pn_w};
   for (void *b : bufs)
