@@ -1036,7 +1036,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
   // (one pass), kappa01, kappa10, then the A-block A01 (strict upper) and
   // A10 / 4 (strict lower triangle); 13 KB at h = 20, twelve waves per CU
   __shared__ double2 sA[MS + HH];
-  __shared__ double2 colR[N2], colA[N2], colC[N2];
   // in two passes kappa01 of pass 0 holds B1 through pass 1
   constexpr bool GJ_IN_SA = (PASSES == 1);
   __shared__ double2 fc_sep[GJ_IN_SA ? 1 : H];
@@ -1069,11 +1068,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
   double2 pf = make_double2(1.0, 0.0);
   {
     double2 sv[SL];
-    int sij[SL];
+    // byte offsets 16 i | 16 j << 16 of each entry's row and column, in LDS
+    // (as registers they would spill at five waves per SIMD); padding lanes
+    // get i = j = 2H, which matches no row or column
+    __shared__ unsigned sijL[SL * 64];
+    // columns r (colR), r + 1 (colA) and c (colC) of the step and the pivot
+    __shared__ double2 colbuf[3 * N2 + 1];
+    constexpr unsigned OFF_R = 0, OFF_A = 16 * N2, OFF_C = 32 * N2, OFF_P = 48 * N2;
+    auto at = [&](unsigned off) -> double2 & { return *(double2 *)((char *)colbuf + off); };
 #pragma unroll
     for (int s = 0; s < SL; ++s) {
       const int e = s * 64 + lane;
-      int i = 0, j = 0;
+      int i = N2, j = N2;
       double2 v = cz();
       if (e < NU) {
         int tt = (int)((1.0 + sqrt(1.0 + 8.0 * e)) * 0.5);
@@ -1087,60 +1093,77 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
         v = cscale(p.cscale, v);
       }
       sv[s] = v;
-      sij[s] = i | (j << 8);
+      sijL[s * 64 + lane] = (unsigned)(16 * i) | ((unsigned)(16 * j) << 16);
     }
+    // Entries that left the trailing matrix may be overwritten: they are
+    // never read again (the last one, S(2H-2, 2H-1), is entry 0).
     bool zero = false;
     for (int k = 0; k < H - 1; ++k) {
       const int r = 2 * k, m = N2 - r;
       const int cnt = m * (m - 1) / 2, lo = (m - 1) * (m - 2) / 2, cnt2 = (m - 2) * (m - 3) / 2;
+      const unsigned r16 = 16u * r, r16b = r16 + 16u;
       unsigned key = 0u;
 #pragma unroll
       for (int s = 0; s < SL; ++s) {
         if (s * 64 < cnt && (s + 1) * 64 > lo) {
-          const int e = s * 64 + lane;
-          if (e >= lo && e < cnt) key = max(key, pivot_key<6>(sv[s].x * sv[s].x + sv[s].y * sv[s].y, sij[s] >> 8));
+          const unsigned ij = sijL[s * 64 + lane];
+          const unsigned pk = pivot_key<6>(sv[s].x * sv[s].x + sv[s].y * sv[s].y, (int)(ij >> 20));
+          key = ((ij & 0xffffu) == r16) ? max(key, pk) : key;
         }
       }
       key = wave_umax(key);
       const int c = 63 - (int)(key & 63u);
+      const unsigned c16 = 16u * c;
+      const bool swap = c != r + 1;
+      // columns of the step with the exchange c <-> r + 1 of the trailing
+      // indices folded in (only indices >= r + 2 are read later):
+      //   colR[x] = S(x, r), colR[c] <- S(r + 1, r), pivot S(r, c) aside;
+      //   colA[x] = S(x, r + 1);  colC[x] = S(x, c), colC[c] <- S(r + 1, c).
+      // Rows r, r + 1 are entries [cnt2, cnt), row and column c [C(2H-1-c, 2), cnt).
+      const int loc = min(cnt2, (N2 - 1 - c) * (N2 - 2 - c) / 2);
 #pragma unroll
       for (int s = 0; s < SL; ++s) {
-        if (s * 64 < cnt) {
-          const int e = s * 64 + lane;
-          if (e < cnt) {
-            const int i = sij[s] & 255, j = sij[s] >> 8;
-            const double2 v = sv[s];
-            if (i == r) colR[j] = cneg(v);
-            if (i == r + 1) colA[j] = cneg(v);
-            if (j == r + 1) colA[i] = v;
-            if (i == c) colC[j] = cneg(v);
-            if (j == c) colC[i] = v;
+        if (s * 64 < cnt && (s + 1) * 64 > loc) {
+          const unsigned ij = sijL[s * 64 + lane], i16 = ij & 0xffffu, j16 = ij >> 16;
+          const double2 v = sv[s];
+          if (i16 == r16) {
+            if (swap && j16 == c16) at(OFF_P) = v;
+            else at(OFF_R + ((swap && j16 == r16b) ? c16 : j16)) = cneg(v);
           }
+          if (i16 == r16b) at(OFF_A + j16) = cneg(v);
+          if (i16 == c16) at(OFF_C + j16) = cneg(v);
+          if (j16 == c16 && i16 > r16) at(OFF_C + ((i16 == r16b) ? c16 : i16)) = v;
         }
       }
       __syncthreads();
-      const double2 spiv = cneg(colR[c]);  // S(r, c)
+      const double2 spiv = swap ? at(OFF_P) : cneg(at(OFF_R + c16));  // S(r, c)
       if (!(spiv.x * spiv.x + spiv.y * spiv.y > 1.0e-26)) {  // |pivot| <= 1e-13
         zero = true;
         break;
       }
-      const double2 inv = crecip(spiv);
-      pf = cmul(pf, (c != r + 1) ? cneg(spiv) : spiv);
-      // trailing entries i >= r + 2 after exchanging c <-> r + 1:
-      // S'(x, r) = colR[s(x)], S'(x, r + 1) = colC[s(x)], s = the exchange
+      const double2 inv = crecip1(spiv);
+      pf = cmul(pf, swap ? cneg(spiv) : spiv);
+      // the entries of row and column c take those of r + 1 (S'(x, c) =
+      // S(x, r + 1) = colA[x]), then the Schur update of the trailing entries
+      // i, j >= r + 2: S' = S + (S'(i,r+1) S'(j,r) - S'(i,r) S'(j,r+1)) / S(r, c)
+      if (c >= r + 2) {
+        const int lof = (N2 - 1 - c) * (N2 - 2 - c) / 2;
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          if (s * 64 < cnt2 && (s + 1) * 64 > lof) {
+            const unsigned ij = sijL[s * 64 + lane], i16 = ij & 0xffffu, j16 = ij >> 16;
+            if (j16 == c16) sv[s] = at(OFF_A + i16);
+            else if (i16 == c16) sv[s] = cneg(at(OFF_A + j16));
+          }
+        }
+      }
 #pragma unroll
       for (int s = 0; s < SL; ++s) {
         if (s * 64 < cnt2) {
-          const int e = s * 64 + lane;
-          if (e < cnt2) {
-            const int i = sij[s] & 255, j = sij[s] >> 8;
-            double2 v = sv[s];
-            if (j == c) v = colA[i];
-            else if (i == c) v = cneg(colA[j]);
-            const int si = (i == c) ? r + 1 : i, sj = (j == c) ? r + 1 : j;
-            const double2 tu = csub(cmul(colC[si], colR[sj]), cmul(colR[si], colC[sj]));
-            sv[s] = cadd(v, cmul(tu, inv));
-          }
+          const unsigned ij = sijL[s * 64 + lane], i16 = ij & 0xffffu, j16 = ij >> 16;
+          double2 t = cmul(at(OFF_C + i16), at(OFF_R + j16));
+          cfms(t, at(OFF_R + i16), at(OFF_C + j16));
+          cfma(sv[s], t, inv);
         }
       }
       __syncthreads();
