@@ -908,11 +908,6 @@ __device__ __forceinline__ void put_b(void *hi, void *lo, int k, size_t col, dou
   }
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-
 template <int CTRL, int RM>
 __device__ __forceinline__ unsigned dpp_umax(unsigned v) {
   return max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, RM, 0xF, false));
@@ -928,6 +923,27 @@ __device__ __forceinline__ unsigned wave_umax(unsigned v) {
   v = dpp_umax<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
   v = dpp_umax<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
   return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp_f64(double v) {  // DPP move of a double (0 where no source)
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, RM, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, RM, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum of v over the wavefront by the DPP steps of wave_umax, wave-uniform.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_f64<0xB1, 0xF>(v);
+  v += dpp_f64<0x4E, 0xF>(v);
+  v += dpp_f64<0x141, 0xF>(v);
+  v += dpp_f64<0x140, 0xF>(v);
+  v += dpp_f64<0x142, 0xA>(v);
+  v += dpp_f64<0x143, 0xC>(v);
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
 // Pivot key: a non-negative magnitude as FP32 bits with the low KB bits
@@ -1356,7 +1372,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
   }
   double v[14] = {e2b.x, e2b.y, ek.x, ek.y, tr.x, tr.y, aj.x, aj.y, tp.x, tp.y, tm.x, tm.y, tz.x, tz.y};
 #pragma unroll
-  for (int q = 0; q < 14; ++q) v[q] = wave_sum(v[q]);
+  for (int q = 0; q < 14; ++q) v[q] = wave_sum_dpp(v[q]);
   if (lane == 0) {
     double2 *o = p.out + obase * FAC_OUT;
 #pragma unroll
