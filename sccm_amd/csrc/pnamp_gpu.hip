@@ -962,12 +962,13 @@ __device__ __forceinline__ unsigned pivot_key(double mag, int idx) {
 // unused coefficient column q with the largest |M(k, q)| (|re| + |im|, ties to
 // the smallest q): for M = T^T this is the row pivoting of T by ZGETRF's
 // IZAMAX.  The row index is static, so nothing is picked out of registers by
-// a run-time index; column q is broadcast through LDS.  On return the unknown
-// q_k = qmap[k] is row k of each right-hand-side column; INFO as LAPACK for an
-// exactly-zero pivot.  The workgroup is this one wavefront, so
-// __syncthreads() only orders LDS.
+// a run-time index; column q reaches every lane by v_readlane from lane q (a
+// wave-uniform operand of the update; through LDS the 64-lane broadcast reads
+// made the step LDS-bound).  On return the unknown q_k = qmap[k] is row k of
+// each right-hand-side column; INFO as LAPACK for an exactly-zero pivot.  The
+// workgroup is this one wavefront, so __syncthreads() only orders LDS.
 template <int H>
-__device__ int wave_gauss_jordan(double2 (&col)[H], double2 *fc, int *qmap) {
+__device__ int wave_gauss_jordan(double2 (&col)[H], int *qmap) {
   const int lane = threadIdx.x;
   unsigned used = 0u;
   int info = 0;
@@ -978,24 +979,23 @@ __device__ int wave_gauss_jordan(double2 (&col)[H], double2 *fc, int *qmap) {
     key = wave_umax(key);
     const int q = 31 - (int)(key & 31u);
     used |= 1u << q;
-    if (lane == q) {
-#pragma unroll
-      for (int r = 0; r < H; ++r) fc[r] = col[r];
-      if (k == 0) qmap[0] = q;
-    }
-    if (lane == 0 && k > 0) qmap[k] = q;
-    __syncthreads();
-    const double2 pv = fc[k];
+    if (lane == 0) qmap[k] = q;
+    auto rl = [&](double x) {
+      const long long bb = __double_as_longlong(x);
+      return __longlong_as_double(((long long)__builtin_amdgcn_readlane((int)(bb >> 32), q) << 32) |
+                                  (long long)(unsigned)__builtin_amdgcn_readlane((int)bb, q));
+    };
+    const double2 pv = make_double2(rl(col[k].x), rl(col[k].y));
     // tested here (uniform) rather than at the INFO store: a test sunk to
     // the use keeps every pivot alive in registers
     if (__builtin_amdgcn_readfirstlane((pv.x == 0.0 && pv.y == 0.0) ? 1 : 0) && info == 0) info = k + 1;
     const double2 prow = cmul(col[k], crecip1(pv));
 #pragma unroll
     for (int r = 0; r < H; ++r)
-      if (r != k) cfms(col[r], fc[r], prow);
+      if (r != k) cfms(col[r], make_double2(rl(col[r].x), rl(col[r].y)), prow);
     col[k] = prow;
-    __syncthreads();
   }
+  __syncthreads();
   return info;
 }
 
@@ -1048,17 +1048,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
   constexpr int PASSES = (3 * H <= 64) ? 1 : 2;
   constexpr int HP = H + 1, MS = H * HP;  // LDS leading dimension h + 1: transposed reads avoid bank conflicts
   // B0 = sA[0, MS): rho (ld h + 1; transposed reads avoid bank conflicts);
-  // B1 = sA[MS, MS + h^2) (ld h): Gauss-Jordan pivot column and unknown map
-  // (one pass), kappa01, kappa10, then the A-block A01 (strict upper) and
-  // A10 / 4 (strict lower triangle); 13 KB at h = 20, twelve waves per CU
+  // B1 = sA[MS, MS + h^2) (ld h): Gauss-Jordan unknown map (one pass),
+  // kappa01, kappa10, then the A-block A01 (strict upper) and A10 / 4
+  // (strict lower triangle); 13 KB at h = 20, twelve waves per CU
   __shared__ double2 sA[MS + HH];
   // in two passes kappa01 of pass 0 holds B1 through pass 1
   constexpr bool GJ_IN_SA = (PASSES == 1);
-  __shared__ double2 fc_sep[GJ_IN_SA ? 1 : H];
   __shared__ int qmap_sep[GJ_IN_SA ? 1 : H];
   double2 *const B0 = sA, *const B1 = sA + MS;
-  double2 *fc = GJ_IN_SA ? B1 : fc_sep;
-  int *qmap = GJ_IN_SA ? (int *)(B1 + H) : qmap_sep;
+  int *qmap = GJ_IN_SA ? (int *)B1 : qmap_sep;
   const int lane = threadIdx.x;
   // XCD-aware order: consecutive logical indices (the nphi waves of one setup
   // block) stay on one XCD and its L2; gridDim.x is a multiple of 8.
@@ -1226,7 +1224,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
 #pragma unroll
       for (int r = 0; r < H; ++r) col[r] = cz();
     }
-    const int inf = wave_gauss_jordan<H>(col, fc, qmap);
+    const int inf = wave_gauss_jordan<H>(col, qmap);
     if (inf && !info) info = inf;
     // solution rows in unknown order through B0, h rows at a time (unknown
     // qmap[k] is row k), back into col: w[k] is W(irow, k) on the W lanes
