@@ -1041,6 +1041,28 @@ __device__ int wave_gauss_jordan(double2 (&col)[H], int *qmap) {
 // h = 20, so three waves per SIMD).
 constexpr int PART_PF = 1, PART_REST = 2;
 
+// Row / column byte offsets 16 i | 16 j << 16 of the packed strict upper
+// triangle of an n2 x n2 matrix, entries numbered from the bottom-right
+// corner (entry e is in row n2 - 1 - t with t(t-1)/2 <= e < t(t+1)/2);
+// padding entries get i = j = n2.  Built at compile time.
+template <int N2>
+struct PfIndex {
+  static constexpr int SIZE = ((N2 * (N2 - 1) / 2 + 63) / 64) * 64;
+  unsigned v[SIZE];
+  constexpr PfIndex() : v() {
+    for (int e = 0; e < SIZE; ++e) {
+      int i = N2, j = N2;
+      if (e < N2 * (N2 - 1) / 2) {
+        int t = 1;
+        while ((t + 1) * t / 2 <= e) ++t;
+        i = N2 - 1 - t;
+        j = i + 1 + (e - t * (t - 1) / 2);
+      }
+      v[e] = (unsigned)(16 * i) | ((unsigned)(16 * j) << 16);
+    }
+  }
+};
+
 template <int H, int MIX, int PART>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART_PF ? 5 : (3 * H <= 64 ? 3 : 2)))) void k_fac_phi(PhiParams p) {
   constexpr int N2 = 2 * H, NU = N2 * (N2 - 1) / 2, SL = (NU + 63) / 64, HH = H * H;
@@ -1090,24 +1112,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
     __shared__ double2 colbuf[3 * N2 + 1];
     constexpr unsigned OFF_R = 0, OFF_A = 16 * N2, OFF_C = 32 * N2, OFF_P = 48 * N2;
     auto at = [&](unsigned off) -> double2 & { return *(double2 *)((char *)colbuf + off); };
+    static constexpr PfIndex<N2> pf_index{};
 #pragma unroll
     for (int s = 0; s < SL; ++s) {
       const int e = s * 64 + lane;
-      int i = N2, j = N2;
+      const unsigned ij = pf_index.v[e];
+      const int i = (int)((ij & 0xffffu) >> 4), j = (int)(ij >> 20);
       double2 v = cz();
       if (e < NU) {
-        int tt = (int)((1.0 + sqrt(1.0 + 8.0 * e)) * 0.5);
-        while (tt * (tt - 1) / 2 > e) --tt;
-        while ((tt + 1) * tt / 2 <= e) ++tt;
-        i = N2 - 1 - tt;
-        j = i + 1 + (e - tt * (tt - 1) / 2);
         if (j < H) v = cmul(e2, M0[i + j * H]);
         else if (i < H) v = make_double2((j - H == i) ? -1.0 : 0.0, 0.0);
         else v = make_double2(-Z0[(i - H) + (size_t)(j - H) * n], 0.0);
         v = cscale(p.cscale, v);
       }
       sv[s] = v;
-      sijL[s * 64 + lane] = (unsigned)(16 * i) | ((unsigned)(16 * j) << 16);
+      sijL[s * 64 + lane] = ij;
     }
     // Entries that left the trailing matrix may be overwritten: they are
     // never read again (the last one, S(2H-2, 2H-1), is entry 0).
