@@ -1889,6 +1889,83 @@ __global__ __launch_bounds__(256) void k_project(ProjParams p) {
   }
 }
 
+// Plain fold on the product mesh, separable: per beta node the gamma sum
+// (per K2), then the alpha sum (per K1, K2), then the d^J_{K1K2}(beta) term,
+// (2J+1) / (8 pi^2) sum_abg w_a w_b w_g e^{iK1 a} d^J_{K1K2}(b) e^{iK2 g} ZRES
+// in nbet (ngam + nalp (2jmax+1)) (2jmax+1) + nbet npj complex products
+// instead of nalp nbet ngam npj.  Up to PROJ_MAXO outputs per thread.
+constexpr int PROJ_MAXO = 16;
+__global__ __launch_bounds__(256) void k_project_sep(ProjParams p) {
+  extern __shared__ double2 ktab[];
+  const int ip = blockIdx.x, io = blockIdx.y, tid = threadIdx.x;
+  const int J2m = 2 * p.jmax + 1, na = p.nalp, ng = p.ngam;
+  double2 *ea = ktab, *eg = ea + (size_t)J2m * na, *zb = eg + (size_t)J2m * ng, *zg = zb + (size_t)na * ng,
+          *zag = zg + (size_t)na * J2m;
+  for (int i = tid; i < J2m * na; i += blockDim.x) ea[i] = cexpi((i / na - p.jmax) * p.alp[i % na]);
+  for (int i = tid; i < J2m * ng; i += blockDim.x) eg[i] = cexpi((i / ng - p.jmax) * p.gam[i % ng]);
+  const double2 *z = p.zres + (size_t)ip * p.NE * NOBS;
+  double2 *kout = p.kern + ((size_t)ip * NOBS + io) * (p.jmax + 1) * J2m * J2m;
+  const int ntot = (p.jmax + 1) * J2m * J2m;
+  double2 acc[PROJ_MAXO];
+#pragma unroll
+  for (int m = 0; m < PROJ_MAXO; ++m) acc[m] = cz();
+  for (int ib = 0; ib < p.nbet; ++ib) {
+    __syncthreads();
+    for (int i = tid; i < na * ng; i += blockDim.x) zb[i] = z[((size_t)ib * na * ng + i) * NOBS + io];
+    __syncthreads();
+    for (int i = tid; i < na * J2m; i += blockDim.x) {
+      const int a = i / J2m, k2 = i % J2m;
+      double2 sacc = cz();
+      for (int g = 0; g < ng; ++g) cfma(sacc, cscale(p.wgam[g], eg[k2 * ng + g]), zb[a * ng + g]);
+      zg[i] = sacc;
+    }
+    __syncthreads();
+    for (int i = tid; i < J2m * J2m; i += blockDim.x) {
+      const int k1 = i / J2m, k2 = i % J2m;
+      double2 sacc = cz();
+      for (int a = 0; a < na; ++a) cfma(sacc, cscale(p.walp[a], ea[k1 * na + a]), zg[a * J2m + k2]);
+      zag[i] = sacc;
+    }
+    __syncthreads();
+    const double wb = p.wbet[ib];
+#pragma unroll
+    for (int m = 0; m < PROJ_MAXO; ++m) {
+      const int o = tid + m * (int)blockDim.x;
+      if (o >= ntot) break;
+      const int J = o / (J2m * J2m), K1 = (o / J2m) % J2m - p.jmax, K2 = o % J2m - p.jmax;
+      if (K1 < -J || K1 > J || K2 < -J || K2 > J) continue;
+      const int e = J * (4 * J * J - 1) / 3 + (K1 + J) * (2 * J + 1) + (K2 + J);
+      const double dp = p.ptab[((size_t)ib * 2 + 0) * p.npj + e];
+      acc[m] = cadd(acc[m], cscale(wb * dp, zag[(K1 + p.jmax) * J2m + (K2 + p.jmax)]));
+    }
+  }
+  const bool onept = (na * ng == 1);
+#pragma unroll
+  for (int m = 0; m < PROJ_MAXO; ++m) {
+    const int o = tid + m * (int)blockDim.x;
+    if (o >= ntot) break;
+    const int J = o / (J2m * J2m);
+    const double wJ = onept ? (2.0 * J + 1.0) / 2.0 : (2.0 * J + 1.0) / (8.0 * M_PI * M_PI);
+    kout[o] = cscale(wJ, acc[m]);
+  }
+  if (io == 0) {
+    __shared__ int st[256];
+    int sv = 0;
+    for (int i = tid; i < p.NE; i += blockDim.x) {
+      const int v = p.status[(size_t)ip * p.NE + i];
+      if (v && !sv) sv = v;
+    }
+    st[tid] = sv;
+    __syncthreads();
+    if (tid == 0) {
+      int r = 0;
+      for (int i = 0; i < (int)blockDim.x; ++i)
+        if (st[i]) { r = st[i]; break; }
+      p.pair_status[ip] = r;
+    }
+  }
+}
+
 // ------------------------------------------------------------------ Hill-Wheeler
 // Cyclic Jacobi on a real symmetric matrix in global memory, one workgroup,
 // round-robin pairing so that dim/2 rotations run in parallel per step.
@@ -2547,6 +2624,17 @@ extern "C" int pnamp_state_overlaps(pnamp_ctx *c, double *det) {
 }
 
 static hipError_t launch_project(pnamp_ctx *c, int npairs, const ProjParams &pr) {
+  const size_t J2m = 2 * (size_t)c->jmax + 1;
+  if (c->fold == PNAMP_FOLD_PLAIN && (size_t)(c->jmax + 1) * J2m * J2m <= 256 * (size_t)PROJ_MAXO) {
+    const size_t lds = sizeof(double2) * (J2m * (c->nalp + c->ngam) + (size_t)c->nalp * c->ngam +
+                                          (size_t)c->nalp * J2m + J2m * J2m);
+    if (lds > 48 * 1024) {
+      const hipError_t e = hipFuncSetAttribute((const void *)k_project_sep, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_project_sep, dim3(npairs, NOBS), dim3(256), lds, c->stream, pr);
+    return hipGetLastError();
+  }
   const size_t lds = sizeof(double2) * (2 * (size_t)c->jmax + 1) * (c->nalp + c->ngam);
   if (lds > 48 * 1024) {
     const hipError_t e = hipFuncSetAttribute((const void *)k_project, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
