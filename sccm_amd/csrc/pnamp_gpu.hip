@@ -2131,6 +2131,7 @@ struct pnamp_ctx {
   double *d_lme[2] = {nullptr, nullptr};
   int nl[2] = {0, 0};
   bool fac_list_ok = false;
+  std::string fac_reason;
   double *d_Uf = nullptr, *d_Vf = nullptr, *d_UfT = nullptr, *d_VfT = nullptr;
   double *d_mtab = nullptr;         // [3][h]: m, <m+1|J_+|m>, j(j+1) per species-local state
   double2 *d_fout = nullptr, *d_frho = nullptr, *d_sbuf = nullptr;
@@ -2431,6 +2432,7 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
         const int i = e.idx & 255, j = (e.idx >> 8) & 255, kk = (e.idx >> 16) & 255, l = e.idx >> 24;
         if (!(i < j && kk < l)) {  // the A-block holds A01 and A10 for index pairs in increasing order only
           c->fac_list_ok = false;
+          c->fac_reason = "factored and mixed forms need list entries with a < b and c < d";
           break;
         }
         term(i, j, kk, l, e.me);
@@ -2487,8 +2489,11 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
         for (size_t i = 0; i < xs.size(); ++i) {
           for (const auto &e : rows[xs[i]]) {
             const auto it = std::lower_bound(ys.begin(), ys.end(), e.first);
-            if (it == ys.end() || *it != e.first)
-              return bail(PNAMP_E_ARG, "proton-neutron interaction entry does not conserve M");
+            if (it == ys.end() || *it != e.first) {  // the plain form takes the list as it is
+              c->fac_list_ok = false;
+              c->fac_reason = "factored and mixed forms need proton-neutron list entries that conserve M";
+              continue;
+            }
             wblk[w0 + (size_t)(it - ys.begin()) * xs.size() + i] = e.second;
           }
           prow.push_back(make_int2(xs[i], (int)bi));
@@ -2685,8 +2690,7 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
   if (c->form != PNAMP_FORM_PLAIN) {
     if (!c->separable)
       return fail(c, PNAMP_E_ARG, "factored and mixed forms need states without proton-neutron mixing");
-    if (!c->fac_list_ok)
-      return fail(c, PNAMP_E_ARG, "factored and mixed forms need list entries with a < b and c < d");
+    if (!c->fac_list_ok) return fail(c, PNAMP_E_ARG, c->fac_reason);
     const int mix = (c->form == PNAMP_FORM_FACTORED) ? MIX_FP64 : (c->form == PNAMP_FORM_MIXED_FP32 ? MIX_FP32 : MIX_BF16);
     const int h = c->h, hh = h * h, kp = c->kp;
     if (c->nphi > NPHI_MAX) return fail(c, PNAMP_E_SIZE, "factored form supports nphi <= 16");

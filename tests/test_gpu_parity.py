@@ -207,6 +207,46 @@ def test_factored_form_rejects_mixed_states():
     g.close()
 
 
+def _modified_sd_interaction(kind):
+    """sd interaction with one list entry changed: "unordered" swaps a and b
+    of a proton-proton entry (a > b), "pn_m" moves d of a proton-neutron
+    entry to a state of another m (the entry no longer conserves M)."""
+    import dataclasses
+    b = make_basis("sd")
+    inter = synthetic_interaction(b, 1.5, 1001, 1002)
+    ka, kb, kc, kd = (np.array(x, copy=True) for x in (inter.ka, inter.kb, inter.kc, inter.kd))
+    h = b.n // 2
+    if kind == "unordered":
+        q = int(np.flatnonzero((ka <= h) & (kb <= h) & (kc <= h) & (kd <= h))[0])
+        ka[q], kb[q] = kb[q], ka[q]
+    else:
+        q = int(np.flatnonzero((ka <= h) & (kb > h) & (kc <= h) & (kd > h))[0])
+        d = int(kd[q]) - 1
+        other = [s for s in range(h, b.n) if b.twom[s] != b.twom[d]]
+        kd[q] = other[0] + 1
+    return b, dataclasses.replace(inter, ka=ka, kb=kb, kc=kc, kd=kd)
+
+
+@pytest.mark.parametrize("kind", ["unordered", "pn_m"])
+def test_factored_form_refuses_lists_it_cannot_split(kind):
+    """The factored forms need list entries with a < b, c < d and proton-
+    neutron entries that conserve M; other lists run in the plain form and
+    are refused by the factored one with an error, never computed wrongly."""
+    from sccm_amd.pnamp import FORM_FACTORED, FORM_PLAIN, PnampError
+    b, inter = _modified_sd_interaction(kind)
+    g = _gpu(b, inter, make_mesh(2, 3, 2, 3, 2))
+    U, V = configs.make_states(b, 2, 2, configs.gcm_grid(3))
+    g.set_states(U, V)
+    assert g.separable
+    g.set_form(FORM_PLAIN)
+    zr, kern = g.kernel_pair(0, 1, 2, 2, want_zres=True)
+    assert np.all(np.isfinite(zr))
+    g.set_form(FORM_FACTORED)
+    with pytest.raises(PnampError):
+        g.kernel_pair(0, 1, 2, 2)
+    g.close()
+
+
 def test_factored_graph_replays_match():
     """The factored form replays a captured hipGraph on calls of the same
     shape; every replay, with eager work of the plain form in between, must
