@@ -112,7 +112,10 @@ int pnamp_set_states(pnamp_ctx *ctx, int nstates, const double *U, const double 
 
 /* Selects the evaluation form (default PNAMP_FORM_PLAIN).  No counterpart in
  * the reference, which has one form; callers choosing FACTORED or a MIXED
- * form must set states for which pnamp_states_separable returns 1. */
+ * form must set states for which pnamp_states_separable returns 1, and an
+ * interaction list in the reference's order (a < b and c < d in every entry)
+ * whose proton-neutron entries conserve M; otherwise the kernel calls of
+ * those forms return PNAMP_E_ARG (the PLAIN form takes any list). */
 int pnamp_set_form(pnamp_ctx *ctx, int form);
 
 /* 1 if every state set has no proton-neutron mixing (U, V block-diagonal by
