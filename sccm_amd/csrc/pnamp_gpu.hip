@@ -1425,7 +1425,7 @@ struct CombParams {
 // W[(c,a),(d,b)] = V_abcd and W[(a,c),(b,d)] += V_abcd for list entries with
 // a + b != c + d (for a, c protons and b, d neutrons the eight-term sum of
 // HFB_Energy_COMPLEX reduces to V [rho_p(c,a) rho_n(d,b) + rho_p(a,c) rho_n(b,d)]).
-// G = W rho_n(phi_n) is formed once per phi_n from the CSR rows of W, then
+// G = W rho_n(phi_n) is formed once per phi_n from the dense Delta-m blocks of W, then
 // the nphi^2 dot products with rho_p(phi_p); the gauge points are summed into
 // ZRES.  NP = nphi is a template argument so the per-gauge register arrays
 // stay in registers.  With GMIX (mixed forms) G comes from k_vgemm instead.
@@ -2126,7 +2126,6 @@ struct pnamp_ctx {
   int form = PNAMP_FORM_PLAIN;
   bool separable = false;
   int nme_s[3] = {0, 0, 0};
-  VEntry *d_vs[3] = {nullptr, nullptr, nullptr};  // unused
   uint4 *d_lofs[2] = {nullptr, nullptr};            // species lists of k_fac_phi (p, n)
   double *d_lme[2] = {nullptr, nullptr};
   int nl[2] = {0, 0};
@@ -2510,7 +2509,7 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
     // (i j | V | k l) with d = [k + l != i + j]
     //   W[(k,i),(l,j)] += V, W[(l,i),(k,j)] -= V, and with d W[(i,k),(j,l)] += V, W[(j,k),(i,l)] -= V,
     //   Q[(i,j),(k,l)] += V/4, and with d Q[(k,l),(i,j)] += V/4,
-    // so that rho^T W rho + A01^T Q A10 is the regrouped eight-term sum; W_pn as the CSR above
+    // so that rho^T W rho + A01^T Q A10 is the regrouped eight-term sum; W_pn from the rows above
     const int kp = (hh + 31) / 32 * 32;
     c->kp = kp;
     for (auto &m : c->vmat) m.assign((size_t)kp * kp, 0.0);
@@ -2994,8 +2993,8 @@ extern "C" void pnamp_destroy(pnamp_ctx *c) {
   void *bufs[] = {c->d_alp, c->d_walp, c->d_bet, c->d_wbet, c->d_gam, c->d_wgam, c->d_rtab, c->d_ptab,
                   c->d_pk_a, c->d_pk_b, c->d_blk_off, c->d_blk_sz, c->d_blk_base, c->d_twoj, c->d_twom,
                   c->d_vlist, c->d_t1b, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status, c->d_q0,
-                  c->d_q1, c->d_zres, c->d_kern, c->d_status, c->d_pair_status, c->d_vs[0], c->d_vs[1],
-                  c->d_vs[2], c->d_lofs[0], c->d_lofs[1], c->d_lme[0], c->d_lme[1], c->d_Uf, c->d_Vf,
+                  c->d_q1, c->d_zres, c->d_kern, c->d_status, c->d_pair_status,
+                  c->d_lofs[0], c->d_lofs[1], c->d_lme[0], c->d_lme[1], c->d_Uf, c->d_Vf,
                   c->d_UfT, c->d_VfT, c->d_mtab, c->d_fout, c->d_frho,
                   c->d_sbuf, c->d_fstatus, c->d_jpack, c->d_pn_rows,
                   c->d_pn_blk, c->d_pn_ycol, c->d_pn_w};
