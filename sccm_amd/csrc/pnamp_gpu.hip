@@ -1387,13 +1387,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
       }
     }
   }
+  // the 14 sums: quad sums by DPP, the 16 quad sums of each value through
+  // LDS (sA is free after the list), lane q < 14 adds those of value q and
+  // writes it (Re, Im of output slots 1..7)
   double v[14] = {e2b.x, e2b.y, ek.x, ek.y, tr.x, tr.y, aj.x, aj.y, tp.x, tp.y, tm.x, tm.y, tz.x, tz.y};
 #pragma unroll
-  for (int q = 0; q < 14; ++q) v[q] = wave_sum_dpp(v[q]);
-  if (lane == 0) {
-    double2 *o = p.out + obase * FAC_OUT;
+  for (int q = 0; q < 14; ++q) {
+    v[q] += dpp_f64<0xB1, 0xF>(v[q]);  // quad_perm [1,0,3,2]
+    v[q] += dpp_f64<0x4E, 0xF>(v[q]);  // quad_perm [2,3,0,1]
+  }
+  double *red = (double *)sA;
+  __syncthreads();
+  if ((lane & 3) == 0) {
 #pragma unroll
-    for (int q = 0; q < 7; ++q) o[1 + q] = make_double2(v[2 * q], v[2 * q + 1]);
+    for (int q = 0; q < 14; ++q) red[(lane >> 2) * 14 + q] = v[q];
+  }
+  __syncthreads();
+  double2 *o = p.out + obase * FAC_OUT;
+  if (lane < 14) {
+    double tot = 0.0;
+#pragma unroll
+    for (int qd = 0; qd < 16; ++qd) tot += red[qd * 14 + lane];
+    ((double *)o)[2 + lane] = tot;
+  }
+  if (lane == 0) {
     o[8] = cz();
     p.status[obase] = info;
   }
