@@ -850,6 +850,7 @@ struct PhiParams {
   const double *Z;                  // [nstates][n*n]
   const double *t1b;                // n x n
   const double *mtab;               // [3][h]: m, <m+1|J_+|m> (0 at m = j), j(j+1) per species-local state
+  const double2 *jtab;              // [2][h^2][4]: coefficients of rho(a,b) in Ekin and the <J^2> pieces (see pnamp_create)
   // species lists (p, n), one term per entry: LDS byte offsets of rho(k,i),
   // rho(l,j), rho(l,i), rho(k,j), A01(i,j), A10(k,l)/4 as 16-bit pairs in
   // x, y, z; the matrix element in lme; padded with zero terms to a multiple
@@ -1224,7 +1225,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
   constexpr int NE64 = (HH + 63) / 64;
   double2 a01r[NE64];  // A01 = kappa01 - kappa01^T, strict upper triangle
   double2 ajk = cz();  // kappa part of <J^2>
-  const auto mz = ro_const(p.mtab), cp = ro_const(p.mtab + H), jj1 = ro_const(p.mtab + 2 * H);
+  const auto mz = ro_const(p.mtab), cp = ro_const(p.mtab + H);
 #pragma unroll
   for (int pass = 0; pass < PASSES; ++pass) {
     const bool isW1 = w1lane && (PASSES == 1 || pass == 1);
@@ -1315,22 +1316,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
   }
 
   // ---- traces, the rho part of <J^2>, kinetic energy
-  const double *t1 = p.t1b + (size_t)sp * H * (n + 1);  // species block, ld n
+  // per element rho(a, b): Ekin t1(b, a); <J^2> -m_a m_b rho(b, a) - c_a c_b
+  // rho(b-1, a-1) + j_a(j_a+1) d_ab; traces d_ab, m_a d_ab, c_a d_{a,b+1},
+  // c_b d_{b,a+1} -- eight coefficients tabled per element, no branches
   double2 tr = cz(), ek = cz(), aj = ajk, tp = cz(), tm = cz(), tz = cz();
+  const double2 *jt = p.jtab + (size_t)sp * HH * 4;
   for (int e = lane; e < HH; e += 64) {
     const int a = e % H, b = e / H;
-    const double ma = mz[a], mb = mz[b], ca = cp[a], cb = cp[b];
+    const double2 c01 = jt[4 * e], c23 = jt[4 * e + 1], c45 = jt[4 * e + 2], c67 = jt[4 * e + 3];
     const double2 rab = B0[a + b * HP];
-    ek = cadd(ek, cscale(t1[b + (size_t)a * n], rab));
-    aj = csub(aj, cscale(ma * mb, cmul(rab, B0[b + a * HP])));
-    if (a > 0 && b > 0 && ca != 0.0 && cb != 0.0) aj = csub(aj, cscale(ca * cb, cmul(rab, B0[(b - 1) + (a - 1) * HP])));
-    if (a == b) {
-      tr = cadd(tr, rab);
-      tz = cadd(tz, cscale(ma, rab));
-      aj = cadd(aj, cscale(jj1[a], rab));
-    }
-    if (a == b + 1) tp = cadd(tp, cscale(ca, rab));
-    if (b == a + 1) tm = cadd(tm, cscale(cb, rab));
+    const double2 rba = B0[b + a * HP];
+    const double2 rl = B0[(b > 0 ? b - 1 : 0) + (a > 0 ? a - 1 : 0) * HP];
+    rfma(ek, c01.x, rab);
+    rfma(aj, -c01.y, cmul(rab, rba));
+    rfma(aj, -c23.x, cmul(rab, rl));
+    rfma(tr, c23.y, rab);
+    rfma(tz, c45.x, rab);
+    rfma(aj, c45.y, rab);
+    rfma(tp, c67.x, rab);
+    rfma(tm, c67.y, rab);
   }
   // A10 / 4 = (kappa10 - kappa10^T) / 4, then the A-block
   double2 a10r[NE64];
@@ -2134,6 +2138,7 @@ struct pnamp_ctx {
       *d_blk_base = nullptr, *d_twoj = nullptr, *d_twom = nullptr;
   VEntry *d_vlist = nullptr;
   double *d_t1b = nullptr;
+  double *d_jtab = nullptr;
   double *d_U = nullptr, *d_V = nullptr, *d_Z = nullptr, *d_det = nullptr;
   int *d_state_status = nullptr;
   int *d_q0 = nullptr, *d_q1 = nullptr;
@@ -2389,6 +2394,25 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
       mt[2 * h + a] = j * (j + 1);
     }
     if ((r = dupload(c, &c->d_mtab, mt.data(), mt.size()))) return bail(r, c->err);
+    // per species and element (a, b) of rho, e = a + b h: t1(b, a), m_a m_b,
+    // c_a c_b (0 unless a, b > 0), d_ab, m_a d_ab, j_a(j_a+1) d_ab,
+    // c_a d_{a,b+1}, c_b d_{b,a+1} (the kinetic energy and <J^2> loop of k_fac_phi)
+    std::vector<double> jt(2 * (size_t)h * h * 8, 0.0);
+    for (int sp = 0; sp < 2; ++sp)
+      for (int b = 0; b < h; ++b)
+        for (int a = 0; a < h; ++a) {
+          double *q = jt.data() + ((size_t)sp * h * h + a + (size_t)b * h) * 8;
+          const double ma = mt[a], mb = mt[b], ca = mt[h + a], cb = mt[h + b];
+          q[0] = inter->t1b[(size_t)(sp * h + b) + (size_t)(sp * h + a) * n];
+          q[1] = ma * mb;
+          q[2] = (a > 0 && b > 0) ? ca * cb : 0.0;
+          q[3] = (a == b) ? 1.0 : 0.0;
+          q[4] = (a == b) ? ma : 0.0;
+          q[5] = (a == b) ? mt[2 * h + a] : 0.0;
+          q[6] = (a == b + 1) ? ca : 0.0;
+          q[7] = (b == a + 1) ? cb : 0.0;
+        }
+    if ((r = dupload(c, &c->d_jtab, jt.data(), jt.size()))) return bail(r, c->err);
   }
   if ((r = dupload(c, &c->d_pk_a, pa.data(), pa.size())) || (r = dupload(c, &c->d_pk_b, pb.data(), pb.size())) ||
       (r = dupload(c, &c->d_blk_off, off.data(), h)) || (r = dupload(c, &c->d_blk_sz, sz.data(), h)) ||
@@ -2763,6 +2787,7 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
     PhiParams fp;
     fp.n = c->n; fp.nphi = c->nphi; fp.NE = c->NE; fp.cscale = c->cscale; fp.sbuf = c->d_sbuf;
     fp.UfT = c->d_UfT; fp.VfT = c->d_VfT; fp.Z = c->d_Z; fp.t1b = c->d_t1b; fp.mtab = c->d_mtab;
+    fp.jtab = (const double2 *)c->d_jtab;
     for (int q = 0; q < 2; ++q) {
       fp.lofs[q] = c->d_lofs[q]; fp.lme[q] = c->d_lme[q]; fp.nl[q] = c->nl[q];
     }
@@ -3009,7 +3034,7 @@ extern "C" void pnamp_destroy(pnamp_ctx *c) {
   drop_graph(c);
   void *bufs[] = {c->d_alp, c->d_walp, c->d_bet, c->d_wbet, c->d_gam, c->d_wgam, c->d_rtab, c->d_ptab,
                   c->d_pk_a, c->d_pk_b, c->d_blk_off, c->d_blk_sz, c->d_blk_base, c->d_twoj, c->d_twom,
-                  c->d_vlist, c->d_t1b, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status, c->d_q0,
+                  c->d_vlist, c->d_t1b, c->d_jtab, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status, c->d_q0,
                   c->d_q1, c->d_zres, c->d_kern, c->d_status, c->d_pair_status,
                   c->d_lofs[0], c->d_lofs[1], c->d_lme[0], c->d_lme[1], c->d_Uf, c->d_Vf,
                   c->d_UfT, c->d_VfT, c->d_mtab, c->d_fout, c->d_frho,
