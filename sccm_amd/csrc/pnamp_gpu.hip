@@ -795,15 +795,21 @@ __global__ __launch_bounds__(NTS) void k_fac_setup(SetupParams p) {
   const double *U0 = p.Uf + ((size_t)q0 * 2 + sp) * hh, *V0 = p.Vf + ((size_t)q0 * 2 + sp) * hh;
   const double *U1 = p.Uf + ((size_t)q1 * 2 + sp) * hh, *V1 = p.Vf + ((size_t)q1 * 2 + sp) * hh;
   const double *Z1 = p.Z + (size_t)q1 * n * n + (size_t)sp * h * (n + 1);  // species block, ld n
+  // R_ab = e^{-i alpha m_a} d_ab(beta) e^{-i gamma m_b}: the two phases
+  // tabled per state (2h sines and cosines instead of one per packed entry)
+  __shared__ double2 ea_s[HMAX], eg_s[HMAX];
   for (int i = tid; i < h; i += NTS) {
     m_off[i] = (short)p.blk_off[i];
     m_sz[i] = (short)p.blk_sz[i];
     m_base[i] = (short)p.blk_base[i];
+    ea_s[i] = cexpi(-alp * 0.5 * p.twom[i]);
+    eg_s[i] = cexpi(-gam * 0.5 * p.twom[i]);
   }
+  __syncthreads();
   for (int e = tid; e < p.npack; e += NTS) {
     const int a = p.pk_a[e], b = p.pk_b[e];
     const double d = p.rtab[(size_t)ib * p.npack + e];
-    sR[e] = cscale(d, cexpi(-alp * 0.5 * p.twom[a] - gam * 0.5 * p.twom[b]));
+    sR[e] = cscale(d, cmul(ea_s[a], eg_s[b]));
   }
   __syncthreads();
   double2 *RV1 = sm, *RcU1 = sm + hh, *RZ1 = sm + 2 * hh;
