@@ -32,7 +32,7 @@ class OrCfg(C.Structure):
         ("gam", _f64p), ("wgam", _f64p),
         ("nphi", C.c_int), ("jmax", C.c_int), ("fold", C.c_int), ("norm_method", C.c_int),
         ("cscale", C.c_double), ("iscale", C.c_int),
-        ("zval", C.c_int), ("nval", C.c_int), ("nthreads", C.c_int),
+        ("zval", C.c_int), ("nval", C.c_int), ("nthreads", C.c_int), ("energy_method", C.c_int),
     ]
 
 
@@ -70,7 +70,9 @@ class Oracle:
     """Holds one (basis, interaction, mesh, Z, N) configuration; keeps the
     numpy buffers alive for the C side."""
 
-    def __init__(self, basis, inter, mesh, zval: int, nval: int, nthreads: int = 0):
+    def __init__(self, basis, inter, mesh, zval: int, nval: int, nthreads: int = 0, energy_method: int = 0):
+        """energy_method 0: the list sum of HFB_Energy_COMPLEX (Method II, the
+        reference's path); 1: the field form (Method I)."""
         self._keep = []
         k = self._keep
 
@@ -94,7 +96,7 @@ class Oracle:
             mesh.alp.size, mesh.bet.size, mesh.gam.size,
             f64(mesh.alp), f64(mesh.walp), f64(mesh.bet), f64(mesh.wbet), f64(mesh.gam), f64(mesh.wgam),
             mesh.nphi, mesh.jmax, mesh.fold, mesh.norm_method, float(mesh.cscale), int(mesh.iscale),
-            int(zval), int(nval), int(nthreads))
+            int(zval), int(nval), int(nthreads), int(energy_method))
 
     @property
     def n(self):

@@ -256,6 +256,11 @@ typedef struct {
   int iscale;
   int zval, nval;
   int nthreads;
+  /* two-body energy per mesh point: 0 the list sum HFB_Energy_COMPLEX
+   * (Method II, what Kernel4Triaxial calls, PNAMP_Kernel3D.f90:316), 1 the
+   * field form HFB_FIELD_COMPLEX + HFB_ENER_comp (Method I, the commented
+   * alternative at PNAMP_Kernel3D.f90:309-310) */
+  int energy_method;
 } or_cfg;
 
 /* ROTATION_MAT (Codes/PNAMP/PNAMP_Proj.f90:83-136). */
@@ -612,7 +617,8 @@ static int euler_point(const or_cfg *c, double alp, double bet, double gam, cons
       if (e && !info) info = e;
       zc zp, zn, ek, eh;
       zn0(n, ro, &zp, &zn);
-      or_energy(c, ro, k10, k01, &ek, &eh);
+      if (c->energy_method == 1) or_energy_field(c, ro, k10, k01, &ek, &eh);
+      else or_energy(c, ro, k10, k01, &ek, &eh);
       sum[0] += znorm;
       sum[1] += znorm * zp;
       sum[2] += znorm * zn;
