@@ -137,6 +137,24 @@ int pnamp_kernel_pair(pnamp_ctx *ctx, int q0, int q1, int zval, int nval, double
 int pnamp_kernels_batched(pnamp_ctx *ctx, int npairs, const int32_t *q0, const int32_t *q1,
                           int zval, int nval, double *zres, double *kern);
 
+/* pnamp_kernels_batched in the PLAIN form without any device allocation.
+ * No counterpart in the reference: it exists for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results as
+ * pnamp_kernels_batched with the context in PNAMP_FORM_PLAIN (the form set
+ * by pnamp_set_form is ignored).  Every intermediate (pair lists, ZRES,
+ * status flags, projected kernels) lives in one static device array of
+ * PNAMP_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; pairs
+ * are processed in chunks that fit it.  Returns PNAMP_E_SIZE if a single
+ * pair needs more (per pair: (8 + 4 n_euler) + 16 n_euler PNAMP_NOBS +
+ * 16 PNAMP_NOBS (jmax+1)(2jmax+1)^2 + 4 bytes, about 0.45 MB for the pf
+ * workload's 2000 Euler points and J <= 8).  All calls on one GPU share the
+ * array: calls that overlap in time on the same GPU (from several threads
+ * or contexts) race and must be serialised by the caller. */
+#define PNAMP_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+int pnamp_kernels_batched_noalloc(pnamp_ctx *ctx, int npairs, const int32_t *q0, const int32_t *q1, int zval,
+                                  int nval, double *zres, double *kern);
+
 /* Norm and energy of every gauge point: samples [npairs][NE][nphi][nphi][2]
  * complex = (norm, E + E0) per (pair, Euler point, phi_p, phi_n), norm as
  * summed into ZRES slot 1 before the 1/nphi^2 (PNAMP_Kernel3D.f90:294-324).

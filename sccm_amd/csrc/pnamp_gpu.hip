@@ -30,6 +30,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2120,9 +2121,43 @@ __global__ void k_matmul(const double *A, const double *B, double *C, int m, int
 
 }  // namespace
 
+// Device memory of pnamp_kernels_batched_noalloc: every intermediate of that
+// entry point (pair lists, ZRES, status flags, projected kernels) lives here.
+// Each GPU that loads the library holds its own copy until the process ends;
+// host code reaches it only through the per-device address from
+// hipGetSymbolAddress (noalloc_scratch_ptr), never through the name.
+static __device__ __align__(256) unsigned char scratch[PNAMP_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+
+// Address of this device's scratch array; looked up once per device (with
+// that device current).  nullptr and a message on failure.
+void *noalloc_scratch_ptr(int device, std::string &err) {
+  if (device < 0 || device >= MAX_DEVICES) {
+    err = "device index beyond the scratch table";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return nullptr;
+    }
+    g_scratch_ptr[device] = ptr;
+  }
+  return g_scratch_ptr[device];
+}
+}  // namespace
+
 // ====================================================================== host
 struct pnamp_ctx {
   int device = 0;
+  void *scratch = nullptr;  // this device's scratch array (pnamp_kernels_batched_noalloc)
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   float ms_points = 0.f, ms_total = 0.f;
@@ -2357,6 +2392,8 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
   if (inter->nme < 0 || inter->nme > (1 << 30)) return bail(PNAMP_E_ARG, "bad interaction size");
   c->device = device;
   HIPCHK(c, hipSetDevice(device));
+  c->scratch = noalloc_scratch_ptr(device, c->err);
+  if (!c->scratch) return bail(PNAMP_E_DEVICE, c->err);
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto &e : c->ev) HIPCHK(c, hipEventCreate(&e));
   double fact[40];
@@ -2707,6 +2744,28 @@ static int ensure_pairs(pnamp_ctx *c, int np) {
   return 0;
 }
 
+// Arguments of k_point for the pairs q0/q1 (device), writing zres/status.
+static PointParams point_params(const pnamp_ctx *c, int zval, int nval, const int *q0, const int *q1,
+                                double2 *zres, int *status) {
+  PointParams pp;
+  pp.n = c->n; pp.nalp = c->nalp; pp.nbet = c->nbet; pp.ngam = c->ngam; pp.nphi = c->nphi;
+  pp.zval = zval; pp.nval = nval; pp.npack = c->npack; pp.NE = c->NE; pp.nme = c->nme;
+  pp.cscale = c->cscale; pp.vac_scale = c->vac_scale; pp.snpfaf = c->snpfaf; pp.e0 = c->e0;
+  pp.alp = c->d_alp; pp.bet = c->d_bet; pp.gam = c->d_gam;
+  pp.U = c->d_U; pp.V = c->d_V; pp.Z = c->d_Z; pp.det = c->d_det;
+  pp.rtab = c->d_rtab; pp.pk_a = c->d_pk_a; pp.pk_b = c->d_pk_b;
+  pp.blk_off = c->d_blk_off; pp.blk_sz = c->d_blk_sz; pp.blk_base = c->d_blk_base; pp.twom = c->d_twom;
+  pp.q0 = q0; pp.q1 = q1; pp.vlist = c->d_vlist; pp.t1b = c->d_t1b; pp.jpack = c->d_jpack;
+  pp.zres = zres; pp.status = status;
+  return pp;
+}
+
+static ProjParams proj_params(const pnamp_ctx *c, const double2 *zres, const int *status, double2 *kern,
+                              int *pair_status) {
+  return ProjParams{c->nalp, c->nbet, c->ngam, c->NE, c->jmax, c->fold, c->npj, c->d_alp, c->d_walp, c->d_wbet,
+                    c->d_gam, c->d_wgam, c->d_ptab, zres, status, kern, pair_status};
+}
+
 // The batched kernels; with samples (device, [npairs][NE][nphi][nphi][2]) the
 // factored forms also write the norm and energy of every gauge pair, and run
 // their launches directly instead of through the captured graph.
@@ -2723,16 +2782,7 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipMemcpyAsync(c->d_q0, q0, sizeof(int) * npairs, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_q1, q1, sizeof(int) * npairs, hipMemcpyHostToDevice, c->stream));
-  PointParams pp;
-  pp.n = c->n; pp.nalp = c->nalp; pp.nbet = c->nbet; pp.ngam = c->ngam; pp.nphi = c->nphi;
-  pp.zval = zval; pp.nval = nval; pp.npack = c->npack; pp.NE = c->NE; pp.nme = c->nme;
-  pp.cscale = c->cscale; pp.vac_scale = c->vac_scale; pp.snpfaf = c->snpfaf; pp.e0 = c->e0;
-  pp.alp = c->d_alp; pp.bet = c->d_bet; pp.gam = c->d_gam;
-  pp.U = c->d_U; pp.V = c->d_V; pp.Z = c->d_Z; pp.det = c->d_det;
-  pp.rtab = c->d_rtab; pp.pk_a = c->d_pk_a; pp.pk_b = c->d_pk_b;
-  pp.blk_off = c->d_blk_off; pp.blk_sz = c->d_blk_sz; pp.blk_base = c->d_blk_base; pp.twom = c->d_twom;
-  pp.q0 = c->d_q0; pp.q1 = c->d_q1; pp.vlist = c->d_vlist; pp.t1b = c->d_t1b; pp.jpack = c->d_jpack;
-  pp.zres = c->d_zres; pp.status = c->d_status;
+  const PointParams pp = point_params(c, zval, nval, c->d_q0, c->d_q1, c->d_zres, c->d_status);
   if (c->form != PNAMP_FORM_PLAIN) {
     if (!c->separable)
       return fail(c, PNAMP_E_ARG, "factored and mixed forms need states without proton-neutron mixing");
@@ -2879,8 +2929,7 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   }
-  ProjParams pr{c->nalp, c->nbet, c->ngam, c->NE, c->jmax, c->fold, c->npj, c->d_alp, c->d_walp, c->d_wbet,
-                c->d_gam, c->d_wgam, c->d_ptab, c->d_zres, c->d_status, c->d_kern, c->d_pair_status};
+  const ProjParams pr = proj_params(c, c->d_zres, c->d_status, c->d_kern, c->d_pair_status);
   HIPCHK(c, launch_project(c, npairs, pr));
   HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
   const size_t kper = (size_t)NOBS * (c->jmax + 1) * (2 * c->jmax + 1) * (2 * c->jmax + 1);
@@ -2930,8 +2979,7 @@ extern "C" int pnamp_project_zres(pnamp_ctx *c, int npairs, const double *zres, 
   HIPCHK(c, hipMemcpyAsync(c->d_zres, zres, sizeof(double2) * (size_t)npairs * c->NE * NOBS, hipMemcpyHostToDevice,
                            c->stream));
   HIPCHK(c, hipMemsetAsync(c->d_status, 0, sizeof(int) * (size_t)npairs * c->NE, c->stream));
-  ProjParams pr{c->nalp, c->nbet, c->ngam, c->NE, c->jmax, c->fold, c->npj, c->d_alp, c->d_walp, c->d_wbet,
-                c->d_gam, c->d_wgam, c->d_ptab, c->d_zres, c->d_status, c->d_kern, c->d_pair_status};
+  const ProjParams pr = proj_params(c, c->d_zres, c->d_status, c->d_kern, c->d_pair_status);
   HIPCHK(c, launch_project(c, npairs, pr));
   const size_t kper = (size_t)NOBS * (c->jmax + 1) * (2 * c->jmax + 1) * (2 * c->jmax + 1);
   HIPCHK(c, hipMemcpyAsync(kern, c->d_kern, sizeof(double2) * kper * npairs, hipMemcpyDeviceToHost, c->stream));
@@ -2942,6 +2990,57 @@ extern "C" int pnamp_project_zres(pnamp_ctx *c, int npairs, const double *zres, 
 extern "C" int pnamp_kernel_pair(pnamp_ctx *c, int q0, int q1, int zval, int nval, double *zres, double *kern) {
   const int32_t a = q0, b = q1;
   return pnamp_kernels_batched(c, 1, &a, &b, zval, nval, zres, kern);
+}
+
+extern "C" int pnamp_kernels_batched_noalloc(pnamp_ctx *c, int npairs, const int32_t *q0, const int32_t *q1,
+                                             int zval, int nval, double *zres, double *kern) {
+  if (!c || npairs <= 0 || !q0 || !q1) return PNAMP_E_ARG;
+  if (!c->nstates) return fail(c, PNAMP_E_STATE, "no states set");
+  if (!c->scratch) return fail(c, PNAMP_E_STATE, "no scratch address for this device");
+  for (int i = 0; i < npairs; ++i)
+    if (q0[i] < 0 || q1[i] < 0 || q0[i] >= c->nstates || q1[i] >= c->nstates)
+      return fail(c, PNAMP_E_ARG, "pair index out of range");
+  // scratch regions per chunk of pairs, each 256-byte aligned: q0, q1, ZRES,
+  // per-Euler status, projected kernels, per-pair status
+  const size_t kper = (size_t)NOBS * (c->jmax + 1) * (2 * c->jmax + 1) * (2 * c->jmax + 1);
+  const size_t per_pair = 2 * sizeof(int) + sizeof(double2) * (size_t)c->NE * NOBS + sizeof(int) * (size_t)c->NE +
+                          sizeof(double2) * kper + sizeof(int);
+  const size_t cap = PNAMP_NOALLOC_SCRATCH_BYTES - 6 * 256;
+  if (per_pair > cap)
+    return fail(c, PNAMP_E_SIZE, "one pair needs " + std::to_string(per_pair) + " bytes of scratch; the static array holds " +
+                                     std::to_string((size_t)PNAMP_NOALLOC_SCRATCH_BYTES));
+  const int chunk = (int)std::min<size_t>({cap / per_pair, (size_t)npairs, (size_t)65535});
+  HIPCHK(c, hipSetDevice(c->device));
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  char *base = (char *)c->scratch;
+  int *sq0 = (int *)base;
+  int *sq1 = (int *)(base + al(sizeof(int) * chunk));
+  double2 *szr = (double2 *)((char *)sq1 + al(sizeof(int) * chunk));
+  int *sst = (int *)((char *)szr + al(sizeof(double2) * (size_t)chunk * c->NE * NOBS));
+  double2 *sk = (double2 *)((char *)sst + al(sizeof(int) * (size_t)chunk * c->NE));
+  int *spst = (int *)((char *)sk + al(sizeof(double2) * kper * chunk));
+  std::vector<int> pst(chunk);
+  for (int p0 = 0; p0 < npairs; p0 += chunk) {
+    const int np = std::min(chunk, npairs - p0);
+    HIPCHK(c, hipMemcpyAsync(sq0, q0 + p0, sizeof(int) * np, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(sq1, q1 + p0, sizeof(int) * np, hipMemcpyHostToDevice, c->stream));
+    // k_point writes every ZRES and status entry of the chunk and k_project
+    // every kernel and pair-status entry: nothing left by an earlier call is read
+    const PointParams pp = point_params(c, zval, nval, sq0, sq1, szr, sst);
+    hipLaunchKernelGGL(k_point, dim3(c->NE, np), dim3(NT), 0, c->stream, pp);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, launch_project(c, np, proj_params(c, szr, sst, sk, spst)));
+    if (kern)
+      HIPCHK(c, hipMemcpyAsync(kern + 2 * kper * p0, sk, sizeof(double2) * kper * np, hipMemcpyDeviceToHost, c->stream));
+    if (zres)
+      HIPCHK(c, hipMemcpyAsync(zres + 2 * (size_t)c->NE * NOBS * p0, szr, sizeof(double2) * (size_t)np * c->NE * NOBS,
+                               hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(pst.data(), spst, sizeof(int) * np, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < np; ++i)
+      if (pst[i]) return fail(c, pst[i], "singular T (ROKAPPAphiomega) in pair " + std::to_string(p0 + i));
+  }
+  return PNAMP_OK;
 }
 
 extern "C" int pnamp_hw_solve(pnamp_ctx *c, int dim, const double *N, const double *H, double eps, int nos,

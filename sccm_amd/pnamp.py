@@ -70,6 +70,8 @@ def load_library(path: str = LIB_PATH):
     L.pnamp_kernel_pair.restype = C.c_int
     L.pnamp_kernels_batched.argtypes = [vp, C.c_int, _i32p, _i32p, C.c_int, C.c_int, _f64p, _f64p]
     L.pnamp_kernels_batched.restype = C.c_int
+    L.pnamp_kernels_batched_noalloc.argtypes = [vp, C.c_int, _i32p, _i32p, C.c_int, C.c_int, _f64p, _f64p]
+    L.pnamp_kernels_batched_noalloc.restype = C.c_int
     L.pnamp_mesh_samples.argtypes = [vp, C.c_int, _i32p, _i32p, C.c_int, C.c_int, _f64p]
     L.pnamp_mesh_samples.restype = C.c_int
     L.pnamp_project_zres.argtypes = [vp, C.c_int, _f64p, _f64p]
@@ -185,17 +187,19 @@ class PNAMP:
                                               _p(zres) if zres is not None else None, _p(kern)))
         return (zres, kern) if want_zres else kern
 
-    def kernels_batched(self, pairs, zval: int, nval: int, want_zres: bool = False, want_kern: bool = True):
+    def kernels_batched(self, pairs, zval: int, nval: int, want_zres: bool = False, want_kern: bool = True,
+                        noalloc: bool = False):
+        """Kernels of many pairs.  ``noalloc`` selects the entry point that
+        allocates no device memory (plain form, static scratch array)."""
         pairs = np.asarray(pairs, dtype=np.int32).reshape(-1, 2)
         q0 = np.ascontiguousarray(pairs[:, 0])
         q1 = np.ascontiguousarray(pairs[:, 1])
         npairs = len(pairs)
         kern = np.zeros((npairs,) + self._kshape(), dtype=np.complex128) if want_kern else None
         zres = np.zeros((npairs, self.n_euler, NOBS), dtype=np.complex128) if want_zres else None
-        self._check(self._L.pnamp_kernels_batched(self._h, npairs, _p(q0, C.c_int32), _p(q1, C.c_int32),
-                                                  int(zval), int(nval),
-                                                  _p(zres) if zres is not None else None,
-                                                  _p(kern) if kern is not None else None))
+        fn = self._L.pnamp_kernels_batched_noalloc if noalloc else self._L.pnamp_kernels_batched
+        self._check(fn(self._h, npairs, _p(q0, C.c_int32), _p(q1, C.c_int32), int(zval), int(nval),
+                       _p(zres) if zres is not None else None, _p(kern) if kern is not None else None))
         return (zres, kern) if want_zres else kern
 
     def mesh_samples(self, pairs, zval: int, nval: int) -> np.ndarray:
