@@ -73,43 +73,81 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(w, threads: int):
+    """The CPU oracle (this project's FP64 restatement of the reference call
+    sequence, OpenMP over Euler points) on one full pair of the pf GCM
+    (162,000 mesh points) plus one beta node of a strided sample of pairs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from sccm_amd.mesh import Mesh
 
     m = w.mesh
-    nb = 4  # the first four beta nodes x all alpha, gamma x all gauge points
-    sub = Mesh(m.alp, m.walp, m.bet[:nb], m.wbet[:nb], m.gam, m.wgam, m.nphi, m.jmax)
-    o = oracle.Oracle(w.basis, w.inter, sub, w.zval, w.nval, nthreads=threads)
+    o = oracle.Oracle(w.basis, w.inter, m, w.zval, w.nval, nthreads=threads)
     q0, q1 = w.pairs[3]
     t = time.perf_counter()
     o.kernel_pair(w.U[q0], w.V[q0], w.U[q1], w.V[q1])
-    dt = time.perf_counter() - t
-    pts = sub.n_points
-    return dict(value=pts / dt, unit="mesh-pts/s", cores=threads, kind="port",
-                sample=f"pair ({q0},{q1}) of the pf GCM, {sub.n_euler} Euler points ({nb} of 20 beta nodes) "
-                       f"x {sub.n_gauge} gauge = {pts} mesh points, {dt:.2f} s, OpenMP over Euler points")
+    dt_full = time.perf_counter() - t
+    pts = m.n_points
+    strided = list(range(10, len(w.pairs), 50))
+    dt_s, pts_s = 0.0, 0
+    for k, ip in enumerate(strided):
+        ib = (7 * k) % m.bet.size
+        sub = Mesh(m.alp, m.walp, m.bet[ib:ib + 1], m.wbet[ib:ib + 1], m.gam, m.wgam, m.nphi, m.jmax)
+        os_ = oracle.Oracle(w.basis, w.inter, sub, w.zval, w.nval, nthreads=threads)
+        p0, p1 = w.pairs[ip]
+        t = time.perf_counter()
+        os_.kernel_pair(w.U[p0], w.V[p0], w.U[p1], w.V[p1])
+        dt_s += time.perf_counter() - t
+        pts_s += sub.n_points
+    return dict(value=(pts + pts_s) / (dt_full + dt_s), unit="mesh-pts/s", cores=threads, kind="port",
+                cpu_model=cpu_model(), full_pair_mesh_pts_per_s=pts / dt_full,
+                sample=f"pair ({q0},{q1}) of the pf GCM in full ({pts} mesh points, {dt_full:.2f} s) plus one beta "
+                       f"node of pairs {strided} ({pts_s} mesh points, {dt_s:.2f} s); OpenMP over Euler points on "
+                       f"{threads} threads; the 34.02 M-point GCM would take "
+                       f"{34.02e6 * (dt_full + dt_s) / (pts + pts_s) / 60:.1f} min")
 
 
 def gcm_compare(ctx, w, kern_a, kern_b, nlevels: int):
-    """Hill-Wheeler energies of two kernel sets: the largest |dE| over the
-    lowest ``nlevels`` levels of every J, the levels of ``kern_b`` and the
-    per-level |dE| (MeV)."""
+    """Hill-Wheeler levels of two kernel sets (a: chosen form, b: plain FP64).
+
+    Levels are those of PreSolve_HWG's natural-state selection
+    (``Codes/GCM/GCM_HWG.f90:2-100``) on the plain kernels: per J and level the
+    number of natural states (NOS) of least variance, where the level sits on
+    a plateau in NOS; the chosen form is solved at the same NOS.  Returns the
+    largest |dE| over those levels (the metric's condition), a per-J report
+    (norm spectrum, NOS, plateau, energies, |dE|) and, for comparison, the
+    largest |dE| of HNDIAG's levels with every natural state above the norm
+    cutoff, which the unconverged Euler mesh leaves unstable."""
     import numpy as np
 
-    from sccm_amd.driver import gcm_energies
-    ea = gcm_energies(ctx, kern_a, w.pairs, len(w.U), w.mesh.jmax, w.hw_eps, nlevels=nlevels)
-    eb = gcm_energies(ctx, kern_b, w.pairs, len(w.U), w.mesh.jmax, w.hw_eps, nlevels=nlevels)
-    worst, levels, dlev = 0.0, {}, {}
-    for J in ea:
-        k = min(len(ea[J]), len(eb[J]))
-        if k:
-            d = np.abs(ea[J][:k] - eb[J][:k])
-            worst = max(worst, float(d.max()))
-            levels[str(J)] = [round(float(x), 6) for x in eb[J][:k]]
-            dlev[str(J)] = [float(x) for x in d]
-    return worst, levels, dlev
+    from sccm_amd.driver import gcm_energies, gcm_levels, levels_at_nos
+    lv = gcm_levels(ctx, kern_b, w.pairs, len(w.U), w.mesh.jmax, w.hw_eps, kmax=nlevels)
+    ea = levels_at_nos(ctx, kern_a, w.pairs, len(w.U), w.mesh.jmax, lv, w.hw_eps)
+    worst, report = 0.0, {}
+    for J, l in lv.items():
+        d = [abs(x - y) if (x is not None and y is not None) else None for x, y in zip(ea[J], l["e"])]
+        worst = max([worst] + [x for x in d if x is not None])
+        report[str(J)] = {"dim": l["dim"], "natural_states_above_cutoff": l["m"],
+                          "most_negative_norm_eigenvalue_rel": l["en_min_rel"],
+                          "nos": l["nos"], "plateau_keV": [None if x is None else round(1e3 * x, 3) for x in l["plateau"]],
+                          "E_plain_MeV": [round(x, 6) for x in l["e"]],
+                          "abs_dE_keV": [None if x is None else float(1e3 * x) for x in d],
+                          "E_all_above_cutoff_MeV": [round(x, 6) for x in l["e_eps"]]}
+    e1 = gcm_energies(ctx, kern_a, w.pairs, len(w.U), w.mesh.jmax, w.hw_eps, nlevels=nlevels)
+    e2 = gcm_energies(ctx, kern_b, w.pairs, len(w.U), w.mesh.jmax, w.hw_eps, nlevels=nlevels)
+    worst_eps = max([0.0] + [float(np.abs(e1[J][:k] - e2[J][:k]).max()) for J in e1
+                             for k in [min(len(e1[J]), len(e2[J]))] if k])
+    return worst, report, worst_eps
 
 
 def kernel_errors(kern_a, kern_b, jmax: int):
@@ -205,6 +243,18 @@ def main():
         fl = plain_flops_per_point(w.n, w.inter.nme) * pts_step
         kname, tform = "k_point", "plain_fp64"
     achieved = fl / (k_avg * 1e-3) / 1e12
+    # the reference call sequence's count (147 n^3 + 190 L per mesh point) at
+    # the same time: what the step's work costs done the reference's way
+    plain_equiv = plain_flops_per_point(w.n, w.inter.nme) * pts_step / (k_avg * 1e-3) / 1e12
+    form_text = {
+        FORM_FACTORED: "factored FP64: per-species Pfaffians and densities, exact algebraic refactoring of the "
+                       "reference sequence, FP64 throughout, batched launches replayed from a captured hipGraph; "
+                       "no MFMA contraction, no surrogate",
+        FORM_MIXED_FP32: "factored, V contraction as FP32 GEMMs on MFMA, everything else FP64, hipGraph",
+        FORM_MIXED_BF16: "factored, V contraction as split-BF16 GEMMs on MFMA (FP32 accumulation), everything "
+                         "else FP64, hipGraph",
+        FORM_PLAIN: "plain FP64 drop-in: the reference call sequence per mesh point",
+    }[form]
     out = {
         "metric": "mesh-pts/s: mixed+batched+surrogate vs plain FP64 GPU at |dE_GCM|<1 keV",
         "value": value,
@@ -218,6 +268,7 @@ def main():
         "vs_baseline": None,
         "dtype": {FORM_MIXED_FP32: "f64+f32(mfma)", FORM_MIXED_BF16: "f64+bf16x3(mfma)"}.get(form, "f64"),
         "data": "synthetic (seeded pf-space interaction and HFB states)",
+        "form_measured": form_text,
         "config": {
             "workload": "pf_gcm: pf space n=40, Z,N=2,4, 20 HFB states / 210 pairs, "
                         "alpha=gamma 10 x beta 20 Euler x 9x9 gauge = 162000 mesh points per pair, J<=8",
@@ -234,7 +285,10 @@ def main():
             "traffic": traffic_per_launch(tform, P),
             "kernel": kname,
             "algorithmic_flops_per_launch": fl,
+            "flops_count": ("the measured form's own algorithm" if form != FORM_PLAIN
+                            else "the reference call sequence, 147 n^3 + 190 L per mesh point"),
             "kernel_ms_per_launch": k_avg,
+            "reference_sequence_equivalent_tflops": plain_equiv,
         },
     }
     if form != FORM_PLAIN and not args.no_plain:
@@ -251,15 +305,16 @@ def main():
             ctx.set_form(form)
             kern_chosen = ctx.kernels_batched(w.pairs, w.zval, w.nval)
         rel = float(np.abs(kern_chosen - kern_plain).max() / np.abs(kern_plain).max())
-        dE, levels, dlev = gcm_compare(ctx, w, kern_chosen, kern_plain, args.levels)
-        out["gcm"] = {"max_abs_dE_keV": 1e3 * dE, "levels_compared_per_J": args.levels,
+        dE, report, dE_eps = gcm_compare(ctx, w, kern_chosen, kern_plain, args.levels)
+        out["gcm"] = {"max_abs_dE_keV": 1e3 * dE,
+                      "levels": "lowest %d per J at PreSolve_HWG's NOS of least variance (plain kernels)" % args.levels,
                       "hw_norm_cutoff": w.hw_eps, "max_rel_kernel_diff": rel,
-                      "energies_plain_MeV": levels,
-                      "abs_dE_keV_per_level": {J: [round(1e3 * x, 9) for x in v] for J, v in dlev.items()},
+                      "per_J": report,
+                      "max_abs_dE_keV_all_natural_states_above_cutoff": 1e3 * dE_eps,
                       "kernel_rel_err_per_J": kernel_errors(kern_chosen, kern_plain, w.mesh.jmax)}
     if not args.no_cpu:
-        threads = min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 16)
-        log(f"bench: CPU oracle sample on {threads} threads")
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        log(f"bench: CPU oracle sample on {threads} threads ({cpu_model()})")
         out["cpu_baseline"] = cpu_baseline(w, threads)
     ctx.close()
     print(json.dumps(out), flush=True)

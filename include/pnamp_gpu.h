@@ -178,6 +178,19 @@ int pnamp_project_zres(pnamp_ctx *ctx, int npairs, const double *zres, double *k
 int pnamp_hw_solve(pnamp_ctx *ctx, int dim, const double *N, const double *H, double eps, int nos,
                    double *EN, double *E, double *F, int *nkept);
 
+/* Natural-state selection of PreSolve_HWG (Codes/GCM/GCM_HWG.f90:2-100) for
+ * one J block: with M the number of norm eigenvalues kept by HNDIAG's cutoff
+ * (EN(i)/EN(1) >= eps, no NOS limit), HNDIAG is solved for every number of
+ * natural states NOS = 1..M (on the device, all NOS at once); per level
+ * k < kmax its energy and the variance sum_i [sum_j (H_ij - E_k N_ij) f_j]^2
+ * of its amplitudes f (mean_variance, :205-254).  Outputs: EN[dim] norm
+ * eigenvalues (descending), *m = M, E and var [M][kmax] (row NOS - 1; zero
+ * for k >= NOS), and per level the NOS of least variance nos_best[kmax]
+ * (strictly below the reference's starting value 100, else 0) with its
+ * energy e_best[kmax].  Returns 1 (IFL) if M = 0. */
+int pnamp_hw_presolve(pnamp_ctx *ctx, int dim, const double *N, const double *H, double eps, int kmax, double *EN,
+                      int *m, double *E, double *var, int *nos_best, double *e_best);
+
 /* Device time of the last kernel call, from HIP events on the context
  * stream: the mesh-point kernel and the whole call (ms). */
 int pnamp_last_timing(pnamp_ctx *ctx, float *ms_points, float *ms_total);

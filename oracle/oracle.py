@@ -174,3 +174,49 @@ def hndiag(NN, HH, eps: float, nos: int = 0):
                           C.c_int(nos), _p(EN), C.byref(m), _p(E), _p(F))
     M = m.value
     return dict(ifl=ifl, m=M, EN=EN, E=E[:M], F=np.array(F[:, :M]))
+
+
+def presolve_hwg(NN, HH, eps: float, kmax: int = 10, literal: bool = True):
+    """PreSolve_HWG (Codes/GCM/GCM_HWG.f90:2-100) with mean_variance (:205-254).
+
+    For NOS = 1..M (M = norm eigenvalues kept by HNDIAG's cutoff) the levels
+    E_k of HNDIAG with NOS natural states and the variance
+    q2 = sum_i [sum_j (H_ij - E_k N_ij) f_j]^2 of their amplitudes f = FF(:, k),
+    k < min(kmax, NOS); per level the NOS of least variance, starting from
+    the reference's 100 and updating on a strictly smaller |q2|.  NOS > M
+    repeat NOS = M and never win.  ``literal``: one or_hndiag call per NOS, as
+    the reference loops; otherwise the same quantities from one numpy
+    eigendecomposition of N and the nested leading blocks of R^T H R (for
+    large dimensions)."""
+    dim = NN.shape[0]
+    r0 = hndiag(NN, HH, eps)
+    M = r0["m"]
+    E = np.zeros((max(M, 0), kmax))
+    var = np.zeros((max(M, 0), kmax))
+    best = np.full(kmax, 100.0)
+    nos_best = np.zeros(kmax, dtype=int)
+    e_best = np.zeros(kmax)
+    if M <= 0:
+        return dict(m=0, EN=r0["EN"], E=E, var=var, nos=nos_best, e_best=e_best)
+    if not literal:
+        w, v = np.linalg.eigh(-NN)
+        order = np.argsort(w, kind="stable")
+        R = v[:, order[:M]] / np.sqrt(-w[order[:M]])
+        G = R.T @ HH @ R
+    for nos in range(1, M + 1):
+        if literal:
+            r = hndiag(NN, HH, eps, nos)
+            ek, F = r["E"], r["F"]
+        else:
+            ek, g = np.linalg.eigh(G[:nos, :nos])
+            F = R[:, :nos] @ g
+        for k in range(min(kmax, nos)):
+            z = (HH - ek[k] * NN) @ F[:, k]
+            q2 = float(z @ z)
+            E[nos - 1, k] = ek[k]
+            var[nos - 1, k] = q2
+            if abs(best[k]) > abs(q2):
+                best[k] = q2
+                nos_best[k] = nos
+                e_best[k] = ek[k]
+    return dict(m=M, EN=r0["EN"], E=E, var=var, nos=nos_best, e_best=e_best)

@@ -1995,16 +1995,21 @@ __global__ __launch_bounds__(256) void k_project_sep(ProjParams p) {
 }
 
 // ------------------------------------------------------------------ Hill-Wheeler
-// Cyclic Jacobi on a real symmetric matrix in global memory, one workgroup,
-// round-robin pairing so that dim/2 rotations run in parallel per step.
-// Eigenvectors accumulate in Vv.  Ends after a sweep whose off-diagonal norm
-// is below 1e-30 of the total, or after 60 sweeps.
-__global__ __launch_bounds__(256) void k_jacobi(double *A, double *Vv, int dim, int *nsweeps) {
+// Cyclic Jacobi on a real symmetric dim x dim matrix A (global memory,
+// column-major) by the calling workgroup, round-robin pairing so that dim/2
+// rotations run in parallel per step; eigenvectors accumulate in Vv.  Ends
+// after a sweep whose off-diagonal norm is below 1e-30 of the total, or after
+// 60 sweeps; returns the sweep count.  Scratch in LDS: cs, sn, pp, qq of 512
+// entries (dim <= 1022), red of blockDim.x entries.
+struct JacobiLds {
+  double cs[512], sn[512];
+  int pp[512], qq[512];
+  double red[256];
+  int done;
+};
+
+__device__ int block_jacobi(double *A, double *Vv, int dim, JacobiLds &L) {
   const int tid = threadIdx.x, nt = blockDim.x;
-  __shared__ double cs[512], sn[512];
-  __shared__ int pp[512], qq[512];
-  __shared__ double red[256];
-  __shared__ int done;
   const int m = dim + (dim & 1);  // even number of slots, slot dim is a dummy
   for (int e = tid; e < dim * dim; e += nt) Vv[e] = ((e % dim) == (e / dim)) ? 1.0 : 0.0;
   __syncthreads();
@@ -2016,29 +2021,28 @@ __global__ __launch_bounds__(256) void k_jacobi(double *A, double *Vv, int dim, 
       tot += x;
       if ((e % dim) != (e / dim)) off += x;
     }
-    red[tid] = off;
+    L.red[tid] = off;
     __syncthreads();
     if (tid == 0) {
       double o = 0.0;
-      for (int i = 0; i < nt; ++i) o += red[i];
-      red[0] = o;
+      for (int i = 0; i < nt; ++i) o += L.red[i];
+      L.red[0] = o;
     }
     __syncthreads();
-    const double offs = red[0];
+    const double offs = L.red[0];
     __syncthreads();
-    red[tid] = tot;
+    L.red[tid] = tot;
     __syncthreads();
     if (tid == 0) {
       double o = 0.0;
-      for (int i = 0; i < nt; ++i) o += red[i];
-      done = (offs <= 1e-30 * o || offs == 0.0);
+      for (int i = 0; i < nt; ++i) o += L.red[i];
+      L.done = (offs <= 1e-30 * o || offs == 0.0);
     }
     __syncthreads();
-    if (done) break;
+    if (L.done) break;
     for (int step = 0; step < m - 1; ++step) {
-      // round-robin pairs: slot 0 fixed, others rotate
+      // circle method: slot 0 fixed, slots 1..m-1 on a circle of odd length
       for (int k = tid; k < m / 2; k += nt) {
-        // circle method: slot 0 fixed, slots 1..m-1 on a circle of odd length
         int a, b;
         if (k == 0) {
           a = 0;
@@ -2058,14 +2062,14 @@ __global__ __launch_bounds__(256) void k_jacobi(double *A, double *Vv, int dim, 
             s = t * c;
           }
         }
-        pp[k] = P; qq[k] = Q; cs[k] = c; sn[k] = s;
+        L.pp[k] = P; L.qq[k] = Q; L.cs[k] = c; L.sn[k] = s;
       }
       __syncthreads();
       // columns: A = A J, V = V J
       for (int e = tid; e < (m / 2) * dim; e += nt) {
-        const int k = e / dim, r = e % dim, P = pp[k], Q = qq[k];
+        const int k = e / dim, r = e % dim, P = L.pp[k], Q = L.qq[k];
         if (Q >= dim) continue;
-        const double c = cs[k], s = sn[k];
+        const double c = L.cs[k], s = L.sn[k];
         const double ap = A[r + P * dim], aq = A[r + Q * dim];
         A[r + P * dim] = c * ap - s * aq;
         A[r + Q * dim] = s * ap + c * aq;
@@ -2076,9 +2080,9 @@ __global__ __launch_bounds__(256) void k_jacobi(double *A, double *Vv, int dim, 
       __syncthreads();
       // rows: A = J^T A
       for (int e = tid; e < (m / 2) * dim; e += nt) {
-        const int k = e / dim, r = e % dim, P = pp[k], Q = qq[k];
+        const int k = e / dim, r = e % dim, P = L.pp[k], Q = L.qq[k];
         if (Q >= dim) continue;
-        const double c = cs[k], s = sn[k];
+        const double c = L.cs[k], s = L.sn[k];
         const double ap = A[P + r * dim], aq = A[Q + r * dim];
         A[P + r * dim] = c * ap - s * aq;
         A[Q + r * dim] = s * ap + c * aq;
@@ -2086,7 +2090,83 @@ __global__ __launch_bounds__(256) void k_jacobi(double *A, double *Vv, int dim, 
       __syncthreads();
     }
   }
-  if (tid == 0) *nsweeps = sweep;
+  return sweep;
+}
+
+__global__ __launch_bounds__(256) void k_jacobi(double *A, double *Vv, int dim, int *nsweeps) {
+  __shared__ JacobiLds L;
+  const int sw = block_jacobi(A, Vv, dim, L);
+  if (threadIdx.x == 0) *nsweeps = sw;
+}
+
+// Natural-state scan of PreSolve_HWG (Codes/GCM/GCM_HWG.f90:2-100): block b
+// solves HNDIAG restricted to the first nos = b + 1 natural states.  The
+// columns of R (dim x M, R(:, k) = D(:, k) / sqrt(EN_k), EN descending) are
+// nested, so the reduced Hamiltonian of nos states is the leading nos x nos
+// block of G = R^T H R (M x M).  Per level k < min(kmax, nos): the energy
+// E_k (ascending), the amplitudes f = R(:, :nos) g_k (FF of HNDIAG) and the
+// variance q2 = sum_i [sum_j (H_ij - E_k N_ij) f_j]^2 of mean_variance
+// (:205-254).  work holds nos^2 + nos^2 doubles per block at offset
+// 2 sum_{i <= b} i^2; E and var are [M][kmax].
+struct NosParams {
+  int dim, M, kmax;
+  const double *N, *H, *R, *G;
+  double *work, *E, *var;
+};
+
+__global__ __launch_bounds__(256) void k_nos_scan(NosParams p) {
+  __shared__ JacobiLds L;
+  __shared__ int ord[1024];
+  __shared__ double f[1024], z[1024];
+  const int b = blockIdx.x, nos = b + 1, tid = threadIdx.x, nt = blockDim.x, dim = p.dim;
+  const size_t woff = 2 * ((size_t)b * (b + 1) * (2 * b + 1) / 6);
+  double *A = p.work + woff, *Vv = A + (size_t)nos * nos;
+  for (int e = tid; e < nos * nos; e += nt) A[e] = p.G[(e % nos) + (size_t)(e / nos) * p.M];
+  __syncthreads();
+  block_jacobi(A, Vv, nos, L);
+  // ascending order of the eigenvalues (ties by index)
+  for (int i = tid; i < nos; i += nt) {
+    const double ei = A[i + (size_t)i * nos];
+    int r = 0;
+    for (int j = 0; j < nos; ++j) {
+      const double ej = A[j + (size_t)j * nos];
+      r += (ej < ei || (ej == ei && j < i)) ? 1 : 0;
+    }
+    ord[r] = i;
+  }
+  __syncthreads();
+  const int kk = min(p.kmax, nos);
+  for (int k = 0; k < p.kmax; ++k) {
+    double *Eo = p.E + (size_t)b * p.kmax + k, *vo = p.var + (size_t)b * p.kmax + k;
+    if (k >= kk) {
+      if (tid == 0) { *Eo = 0.0; *vo = 0.0; }
+      continue;
+    }
+    const int col = ord[k];
+    const double ek = A[col + (size_t)col * nos];
+    for (int i = tid; i < dim; i += nt) {
+      double s = 0.0;
+      for (int l = 0; l < nos; ++l) s = fma(p.R[i + (size_t)l * dim], Vv[l + (size_t)col * nos], s);
+      f[i] = s;
+    }
+    __syncthreads();
+    double q = 0.0;
+    for (int i = tid; i < dim; i += nt) {
+      double s = 0.0;
+      for (int j = 0; j < dim; ++j) s = fma(p.H[i + (size_t)j * dim] - ek * p.N[i + (size_t)j * dim], f[j], s);
+      z[i] = s;
+      q = fma(s, s, q);
+    }
+    L.red[tid] = q;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int i = 0; i < nt; ++i) t += L.red[i];
+      *Eo = ek;
+      *vo = t;
+    }
+    __syncthreads();
+  }
 }
 
 // W = R^T H R for R (dim x M) and symmetric H (dim x dim): one thread per (j, k)
@@ -3043,82 +3123,146 @@ extern "C" int pnamp_kernels_batched_noalloc(pnamp_ctx *c, int npairs, const int
   return PNAMP_OK;
 }
 
-extern "C" int pnamp_hw_solve(pnamp_ctx *c, int dim, const double *N, const double *H, double eps, int nos,
-                              double *EN, double *E, double *F, int *nkept) {
-  if (!c || dim <= 0 || !N || !H || !EN || !E || !nkept) return PNAMP_E_ARG;
-  if (dim > 1022) return fail(c, PNAMP_E_SIZE, "HW dimension above 1022");
-  HIPCHK(c, hipSetDevice(c->device));
+// Device buffers of one Hill-Wheeler solve, freed on scope exit.
+struct HwBufs {
+  std::vector<void *> p;
+  ~HwBufs() {
+    for (void *x : p)
+      if (x) (void)hipFree(x);
+  }
+  template <class T>
+  bool alloc(T **q, size_t cnt) {
+    const hipError_t e = hipMalloc((void **)q, (cnt ? cnt : 1) * sizeof(T));
+    p.push_back(*q);
+    return e == hipSuccess;
+  }
+};
+
+// The first half of HNDIAG (Codes/GCM/GCM_HNDiag.f90:25-77): diagonalise -N
+// (EN descending), keep the M leading natural states with EN(i)/EN(1) >= eps
+// (the first failure ends the count), at most nos of them, R = D / sqrt(EN)
+// (dim x M), G = R^T H R (M x M).  Leaves N, H, R, G on the device.
+static int hw_prepare(pnamp_ctx *c, HwBufs &B, int dim, const double *N, const double *H, double eps, int nos,
+                      double *EN, int *Mout, double **dN, double **dH, double **dR, double **dG) {
   const size_t dd = (size_t)dim * dim;
-  double *dA = nullptr, *dV = nullptr, *dH = nullptr, *dR = nullptr, *dW = nullptr, *dG = nullptr, *dw = nullptr,
-         *dF = nullptr;
+  double *dA = nullptr, *dV = nullptr, *dD = nullptr, *dw = nullptr;
   int *dsw = nullptr;
-  struct Guard {
-    std::vector<void *> p;
-    ~Guard() {
-      for (void *x : p)
-        if (x) (void)hipFree(x);
-    }
-  } guard;
-  auto al = [&](double **p, size_t cnt) -> int {
-    hipError_t e = hipMalloc((void **)p, cnt * sizeof(double));
-    guard.p.push_back(*p);
-    return e == hipSuccess ? 0 : PNAMP_E_DEVICE;
-  };
-  if (al(&dA, dd) || al(&dV, dd) || al(&dH, dd) || al(&dR, dd) || al(&dW, dd) || al(&dG, dd) || al(&dw, dim) ||
-      al(&dF, dd))
-    return fail(c, PNAMP_E_DEVICE, "hipMalloc failed in pnamp_hw_solve");
-  HIPCHK(c, hipMalloc((void **)&dsw, sizeof(int)));
-  guard.p.push_back(dsw);
-  // HNDIAG: diagonalise -N
+  if (!B.alloc(&dA, dd) || !B.alloc(&dV, dd) || !B.alloc(dN, dd) || !B.alloc(dH, dd) || !B.alloc(&dsw, 1))
+    return fail(c, PNAMP_E_DEVICE, "hipMalloc failed in the Hill-Wheeler solve");
   std::vector<double> negN(dd);
   for (size_t i = 0; i < dd; ++i) negN[i] = -N[i];
   HIPCHK(c, hipMemcpyAsync(dA, negN.data(), sizeof(double) * dd, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(dH, H, sizeof(double) * dd, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(*dN, N, sizeof(double) * dd, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(*dH, H, sizeof(double) * dd, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(256), 0, c->stream, dA, dV, dim, dsw);
   HIPCHK(c, hipGetLastError());
   std::vector<double> A(dd), Vv(dd);
   HIPCHK(c, hipMemcpyAsync(A.data(), dA, sizeof(double) * dd, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(Vv.data(), dV, sizeof(double) * dd, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  // order: ascending eigenvalues of -N == descending EN
+  // ascending eigenvalues of -N == descending EN
   std::vector<int> ord(dim);
   for (int i = 0; i < dim; ++i) ord[i] = i;
-  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return A[a + (size_t)a * dim] < A[b + (size_t)b * dim]; });
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return A[x + (size_t)x * dim] < A[y + (size_t)y * dim]; });
   for (int i = 0; i < dim; ++i) EN[i] = -A[ord[i] + (size_t)ord[i] * dim];
   int M = 0;
   while (M < dim && EN[M] / EN[0] >= eps) ++M;
   if (nos > 0 && nos < M) M = nos;
-  *nkept = M;
-  if (M <= 0) return 1;
+  *Mout = M;
+  if (M <= 0) return PNAMP_OK;
   std::vector<double> D((size_t)dim * M), w(M);
   for (int k = 0; k < M; ++k) {
     w[k] = EN[k];
     for (int i = 0; i < dim; ++i) D[i + (size_t)k * dim] = Vv[i + (size_t)ord[k] * dim];
   }
-  HIPCHK(c, hipMemcpyAsync(dR, D.data(), sizeof(double) * D.size(), hipMemcpyHostToDevice, c->stream));
+  if (!B.alloc(&dD, (size_t)dim * M) || !B.alloc(&dw, M) || !B.alloc(dR, (size_t)dim * M) || !B.alloc(dG, (size_t)M * M))
+    return fail(c, PNAMP_E_DEVICE, "hipMalloc failed in the Hill-Wheeler solve");
+  HIPCHK(c, hipMemcpyAsync(dD, D.data(), sizeof(double) * D.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(dw, w.data(), sizeof(double) * M, hipMemcpyHostToDevice, c->stream));
-  // RR = DD / sqrt(EN)  (in place into dW), GG = RR^T H RR
-  hipLaunchKernelGGL(k_scale_cols, dim3((dim * M + 255) / 256), dim3(256), 0, c->stream, dR, dw, dW, dim, M);
-  hipLaunchKernelGGL(k_congruence, dim3((M * M + 255) / 256), dim3(256), 0, c->stream, dH, dW, dG, dim, M);
-  hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(256), 0, c->stream, dG, dA, M, dsw);
+  hipLaunchKernelGGL(k_scale_cols, dim3((dim * M + 255) / 256), dim3(256), 0, c->stream, dD, dw, *dR, dim, M);
+  hipLaunchKernelGGL(k_congruence, dim3((M * M + 255) / 256), dim3(256), 0, c->stream, *dH, *dR, *dG, dim, M);
+  HIPCHK(c, hipGetLastError());
+  return PNAMP_OK;
+}
+
+extern "C" int pnamp_hw_solve(pnamp_ctx *c, int dim, const double *N, const double *H, double eps, int nos,
+                              double *EN, double *E, double *F, int *nkept) {
+  if (!c || dim <= 0 || !N || !H || !EN || !E || !nkept) return PNAMP_E_ARG;
+  if (dim > 1022) return fail(c, PNAMP_E_SIZE, "HW dimension above 1022");
+  HIPCHK(c, hipSetDevice(c->device));
+  HwBufs B;
+  double *dN = nullptr, *dH = nullptr, *dR = nullptr, *dG = nullptr, *dV = nullptr, *dF = nullptr;
+  int *dsw = nullptr;
+  int M = 0;
+  const int r = hw_prepare(c, B, dim, N, H, eps, nos, EN, &M, &dN, &dH, &dR, &dG);
+  if (r) return r;
+  *nkept = M;
+  if (M <= 0) return 1;
+  if (!B.alloc(&dV, (size_t)M * M) || !B.alloc(&dsw, 1) || (F && !B.alloc(&dF, (size_t)dim * M)))
+    return fail(c, PNAMP_E_DEVICE, "hipMalloc failed in pnamp_hw_solve");
+  hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(256), 0, c->stream, dG, dV, M, dsw);
   HIPCHK(c, hipGetLastError());
   std::vector<double> G((size_t)M * M), gv((size_t)M * M);
   HIPCHK(c, hipMemcpyAsync(G.data(), dG, sizeof(double) * G.size(), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(gv.data(), dA, sizeof(double) * gv.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(gv.data(), dV, sizeof(double) * gv.size(), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   std::vector<int> o2(M);
   for (int i = 0; i < M; ++i) o2[i] = i;
-  std::stable_sort(o2.begin(), o2.end(), [&](int a, int b) { return G[a + (size_t)a * M] < G[b + (size_t)b * M]; });
+  std::stable_sort(o2.begin(), o2.end(), [&](int x, int y) { return G[x + (size_t)x * M] < G[y + (size_t)y * M]; });
   for (int k = 0; k < M; ++k) E[k] = G[o2[k] + (size_t)o2[k] * M];
   if (F) {
+    // FF = RR GG (MAB in HNDIAG), eigenvectors in ascending order of E
     std::vector<double> gs((size_t)M * M);
     for (int k = 0; k < M; ++k)
       for (int l = 0; l < M; ++l) gs[l + (size_t)k * M] = gv[l + (size_t)o2[k] * M];
-    HIPCHK(c, hipMemcpyAsync(dG, gs.data(), sizeof(double) * gs.size(), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_matmul, dim3((dim * M + 255) / 256), dim3(256), 0, c->stream, dW, dG, dF, dim, M, M);
+    HIPCHK(c, hipMemcpyAsync(dV, gs.data(), sizeof(double) * gs.size(), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_matmul, dim3((dim * M + 255) / 256), dim3(256), 0, c->stream, dR, dV, dF, dim, M, M);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(F, dF, sizeof(double) * (size_t)dim * M, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return PNAMP_OK;
+}
+
+extern "C" int pnamp_hw_presolve(pnamp_ctx *c, int dim, const double *N, const double *H, double eps, int kmax,
+                                 double *EN, int *m, double *E, double *var, int *nos_best, double *e_best) {
+  if (!c || dim <= 0 || !N || !H || !EN || !m || !E || !var || !nos_best || !e_best || kmax <= 0)
+    return PNAMP_E_ARG;
+  if (dim > 1022) return fail(c, PNAMP_E_SIZE, "HW dimension above 1022");
+  HIPCHK(c, hipSetDevice(c->device));
+  HwBufs B;
+  double *dN = nullptr, *dH = nullptr, *dR = nullptr, *dG = nullptr, *dwork = nullptr, *dE = nullptr, *dvar = nullptr;
+  int M = 0;
+  const int r = hw_prepare(c, B, dim, N, H, eps, 0, EN, &M, &dN, &dH, &dR, &dG);
+  if (r) return r;
+  *m = M;
+  for (int k = 0; k < kmax; ++k) {
+    nos_best[k] = 0;
+    e_best[k] = 0.0;
+  }
+  if (M <= 0) return 1;
+  const size_t wsz = 2 * ((size_t)M * (M + 1) * (2 * M + 1) / 6);
+  if (!B.alloc(&dwork, wsz) || !B.alloc(&dE, (size_t)M * kmax) || !B.alloc(&dvar, (size_t)M * kmax))
+    return fail(c, PNAMP_E_DEVICE, "hipMalloc failed in pnamp_hw_presolve");
+  NosParams np{dim, M, kmax, dN, dH, dR, dG, dwork, dE, dvar};
+  hipLaunchKernelGGL(k_nos_scan, dim3(M), dim3(256), 0, c->stream, np);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(E, dE, sizeof(double) * (size_t)M * kmax, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(var, dvar, sizeof(double) * (size_t)M * kmax, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // PreSolve_HWG's choice (Codes/GCM/GCM_HWG.f90:60-79): per level the NOS of
+  // least variance, starting from the reference's initial value 100 and
+  // updating on a strictly smaller variance; NOS > M repeat NOS = M
+  for (int k = 0; k < kmax; ++k) {
+    double best = 100.0;
+    for (int b = k; b < M; ++b) {
+      const double q2 = var[(size_t)b * kmax + k];
+      if (std::fabs(best) > std::fabs(q2)) {
+        best = q2;
+        nos_best[k] = b + 1;
+        e_best[k] = E[(size_t)b * kmax + k];
+      }
+    }
   }
   return PNAMP_OK;
 }

@@ -9,6 +9,10 @@ the Hill-Wheeler (GCM) assembly, and the case file of the Fortran host.
   hermiticity, as the reference's symmetric-pair mode ``ICons = 0`` implies,
   ``Codes/PNAMP/PNAMP_Main.f90:61-66``); ``gcm_energies`` solves them with the
   library's HNDIAG (``Codes/GCM/GCM_HNDiag.f90``).
+* ``gcm_levels`` adds the natural-state selection of ``PreSolve_HWG``
+  (``Codes/GCM/GCM_HWG.f90:2-100``): per J and level the number of natural
+  states of least variance, the level there, the plateau around it and the
+  norm spectrum; ``levels_at_nos`` evaluates another kernel set at those NOS.
 * ``write_case`` / ``read_elem`` serve the Fortran host ``fortran/pnamp_driver``.
 """
 from __future__ import annotations
@@ -61,6 +65,53 @@ def gcm_energies(ctx, kern_pairs, pairs, nstates: int, jmax: int, eps: float = 1
             continue
         r = ctx.hw_solve(N, H, eps, want_f=False)
         out[J] = r["E"][:nlevels] if r["ifl"] == 0 else np.zeros(0)
+    return out
+
+
+def gcm_levels(ctx, kern_pairs, pairs, nstates: int, jmax: int, eps: float = 1e-6, kmax: int = 3,
+               jlist=None) -> dict:
+    """Per J: the norm spectrum (natural states kept at ``eps``, the most
+    negative EN(i)/EN(1), a sign of an unconverged projection), the levels of
+    HNDIAG with every natural state above eps, and PreSolve_HWG's choice: per
+    level k < kmax the NOS of least variance, the energy there and the largest
+    change of that energy for NOS - 1 and NOS + 1 (the plateau)."""
+    out = {}
+    for J in (range(jmax + 1) if jlist is None else jlist):
+        N, H, _, _ = hw_matrices(kern_pairs, pairs, nstates, J, jmax)
+        if np.abs(N).max() < 1e-12:
+            continue
+        r = ctx.hw_presolve(N, H, eps, kmax)
+        if r["ifl"]:
+            continue
+        M, Et = r["m"], r["E"]
+        plateau = []
+        for k in range(kmax):
+            n = int(r["nos"][k])
+            if n <= 0:
+                plateau.append(None)
+                continue
+            nb = [Et[i - 1, k] for i in (n - 1, n + 1) if k + 1 <= i <= M]
+            plateau.append(float(max(abs(x - Et[n - 1, k]) for x in nb)) if nb else 0.0)
+        out[J] = dict(dim=N.shape[0], m=M, en_min_rel=float(r["EN"].min() / r["EN"][0]),
+                      e_eps=[float(x) for x in Et[M - 1, :min(kmax, M)]],
+                      nos=[int(x) for x in r["nos"]], e=[float(x) for x in r["e_best"]], plateau=plateau)
+    return out
+
+
+def levels_at_nos(ctx, kern_pairs, pairs, nstates: int, jmax: int, levels: dict, eps: float = 1e-6) -> dict:
+    """Energies of ``kern_pairs`` at the NOS that ``levels`` (gcm_levels of
+    another kernel set) chose per J and level, by HNDIAG with that NOS."""
+    out = {}
+    for J, lv in levels.items():
+        N, H, _, _ = hw_matrices(kern_pairs, pairs, nstates, J, jmax)
+        e = []
+        for k, n in enumerate(lv["nos"]):
+            if n <= 0:
+                e.append(None)
+                continue
+            r = ctx.hw_solve(N, H, eps, nos=n, want_f=False)
+            e.append(float(r["E"][k]) if r["ifl"] == 0 and r["m"] > k else None)
+        out[J] = e
     return out
 
 

@@ -8,6 +8,15 @@ loop order of ``Kernel4Triaxial`` (``Codes/PNAMP/PNAMP_Kernel3D.f90:200-217``):
 iang = (ib * nalp + ia) * ngam + ig.
 Gauge angles are Fomenko points phi_k = 2 pi k / nphi, k = 0..nphi-1
 (``PNAMP_Kernel3D.f90:262-276``).
+
+``fold_mesh`` is the reduced mesh of the literal 3D mirror fold
+(``PNAMP_Kernel3D.f90:415-526``): alpha in [0, pi], beta in [0, pi/2],
+gamma in [0, 2 pi], the ranges the reference's run card states for the
+library ``Gaussmesh`` it links (``Codes/exec/PNAMP.sh:26-28``; that library
+file is not shipped).  For real D2-symmetric states the fold on this mesh
+equals 2 / (d0_K1 d0_K2) times the plain projection over the full mesh
+(d0_K = 2 for K = 0, else 1): an overall factor and a diagonal rescaling of
+the K >= 0 basis, which leave the Hill-Wheeler energies unchanged.
 """
 from __future__ import annotations
 
@@ -63,3 +72,13 @@ def make_mesh(nalp: int, nbet: int, ngam: int, nphi: int, jmax: int, fold: int =
     if ngam == 1:
         gam, wgam = np.zeros(1), np.ones(1)
     return Mesh(alp, walp, bet, wbet, gam, wgam, nphi, jmax, fold, norm_method, cscale, iscale)
+
+
+def fold_mesh(nalp: int, nbet: int, ngam: int, nphi: int, jmax: int, cscale: float = 1.0,
+              iscale: int = 0) -> Mesh:
+    """Gauss-Legendre alpha on [0, pi], cos(beta) on [0, 1], gamma on
+    [0, 2 pi], for the literal fold (fold = 1)."""
+    alp, walp = gauleg(0.0, np.pi, nalp)
+    x, wbet = gauleg(0.0, 1.0, nbet)
+    gam, wgam = gauleg(0.0, 2 * np.pi, ngam)
+    return Mesh(alp, walp, np.arccos(x), wbet, gam, wgam, nphi, jmax, 1, 0, cscale, iscale)

@@ -79,6 +79,9 @@ def load_library(path: str = LIB_PATH):
     L.pnamp_hw_solve.argtypes = [vp, C.c_int, _f64p, _f64p, C.c_double, C.c_int, _f64p, _f64p, _f64p,
                                  C.POINTER(C.c_int)]
     L.pnamp_hw_solve.restype = C.c_int
+    L.pnamp_hw_presolve.argtypes = [vp, C.c_int, _f64p, _f64p, C.c_double, C.c_int, _f64p, C.POINTER(C.c_int),
+                                    _f64p, _f64p, _i32p, _f64p]
+    L.pnamp_hw_presolve.restype = C.c_int
     L.pnamp_last_timing.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.pnamp_last_timing.restype = C.c_int
     L.pnamp_n_euler.argtypes = [vp]
@@ -240,3 +243,22 @@ class PNAMP:
             self._check(rc)
         M = m.value
         return dict(ifl=rc, m=M, EN=EN, E=E[:M], F=None if F is None else np.array(F[:, :M]))
+
+    def hw_presolve(self, N: np.ndarray, H: np.ndarray, eps: float, kmax: int = 10) -> dict:
+        """PreSolve_HWG natural-state selection of one J block: levels and
+        variances for every NOS, and per level the NOS of least variance."""
+        dim = N.shape[0]
+        Nf = np.asfortranarray(N, dtype=np.float64)
+        Hf = np.asfortranarray(H, dtype=np.float64)
+        EN = np.zeros(dim)
+        E = np.zeros((dim, kmax))
+        var = np.zeros((dim, kmax))
+        nos = np.zeros(kmax, dtype=np.int32)
+        eb = np.zeros(kmax)
+        m = C.c_int()
+        rc = self._L.pnamp_hw_presolve(self._h, dim, _p(Nf), _p(Hf), float(eps), int(kmax), _p(EN), C.byref(m),
+                                       _p(E), _p(var), _p(nos, C.c_int32), _p(eb))
+        if rc not in (0, 1):
+            self._check(rc)
+        M = m.value
+        return dict(ifl=rc, m=M, EN=EN, E=E[:M], var=var[:M], nos=nos, e_best=eb)

@@ -68,6 +68,37 @@ def _skew_block(b: Basis, rng: np.random.Generator, sigma: float) -> np.ndarray:
     return X
 
 
+def ry_pi(b: Basis) -> np.ndarray:
+    """Single-particle matrix of exp(-i pi J_y) in the m basis: inside each
+    orbit m -> -m with the phase (-1)^(j - m) (a signed permutation; the
+    phase convention is immaterial where it enters twice, as in P X P^T)."""
+    n = b.n
+    P = np.zeros((n, n))
+    for k in range(n):
+        for l in range(n):
+            if b.t[l] == b.t[k] and b.orbit[l] == b.orbit[k] and b.twom[l] == -b.twom[k]:
+                P[l, k] = -1.0 if ((int(b.twoj[k]) - int(b.twom[k])) // 2) % 2 else 1.0
+    return P
+
+
+def d2_symmetrize(b: Basis, X: np.ndarray, Y: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Projects a Bogoliubov generator onto its D2-invariant part: invariant
+    under exp(-i pi J_z) (X_ij only for m_i - m_j even, Y_ij only for
+    m_i + m_j even) and under exp(-i pi J_y) (X = P X P^T, Y = P Y P^T).
+    With a real generator and a Slater reference of time-reversed pairs
+    the state is real and D2 symmetric: the symmetries the reference's 3D
+    mirror fold of the projection sums relies on
+    (Codes/PNAMP/PNAMP_Kernel3D.f90:415-526), and the reason its GCM keeps
+    only even K >= 0 (Codes/GCM/GCM_HWG.f90:166-170)."""
+    P = ry_pi(b)
+    X = 0.5 * (X + P @ X @ P.T)
+    Y = 0.5 * (Y + P @ Y @ P.T)
+    tm = b.twom.astype(int)
+    X = X * ((((tm[:, None] - tm[None, :]) // 2) % 2) == 0)
+    Y = Y * ((((tm[:, None] + tm[None, :]) // 2) % 2) == 0)
+    return X, Y
+
+
 def bogoliubov_from_generator(b: Basis, X: np.ndarray, Y: np.ndarray,
                               occ: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     n = b.n
@@ -83,7 +114,7 @@ def bogoliubov_from_generator(b: Basis, X: np.ndarray, Y: np.ndarray,
 
 def synthetic_state(b: Basis, zval: int, nval: int, seed: int, sigma: float = 0.3,
                     sigma_pair: float = 0.05, beta2: float = 0.0, gamma_deg: float = 0.0, offset_seed: int | None = None,
-                    offset_scale: float = 1.0, noise: float = 0.0) -> tuple[np.ndarray, np.ndarray]:
+                    offset_scale: float = 1.0, noise: float = 0.0, d2: bool = False) -> tuple[np.ndarray, np.ndarray]:
     """One seeded state.  With ``offset_seed`` set, the generator is
     G(seed) + beta2 (cos(gamma) Gc + sin(gamma) Gs) * offset_scale, where Gc, Gs
     are fixed seeded generators: the GCM configurations of one workload share
@@ -102,6 +133,8 @@ def synthetic_state(b: Basis, zval: int, nval: int, seed: int, sigma: float = 0.
         rn = np.random.default_rng(seed + 7919)
         X = X + _skew_block(b, rn, noise)
         Y = Y + _skew_block(b, rn, noise * sigma_pair / sigma)
+    if d2:
+        X, Y = d2_symmetrize(b, X, Y)
     occ = slater_occupations(b, zval, nval)
     return bogoliubov_from_generator(b, X, Y, occ)
 
