@@ -155,6 +155,35 @@ int pnamp_kernels_batched(pnamp_ctx *ctx, int npairs, const int32_t *q0, const i
 int pnamp_kernels_batched_noalloc(pnamp_ctx *ctx, int npairs, const int32_t *q0, const int32_t *q1, int zval,
                                   int nval, double *zres, double *kern);
 
+/* E2 observables (iE2 = 1 of the reference run card,
+ * Codes/PNAMP/PNAMP_Kernel3D.f90:331-360,527-710): the one-body quadrupole
+ * matrices Q_{2mu}, [5][n*n] real column-major for mu = -2..2 (cME1B%Q2_2t ..
+ * Q22t built by SingleParticleME, Codes/PNAMP/PNAMP_ME1B.f90:138-216).
+ * Needed before pnamp_kernels_e2. */
+int pnamp_set_q2(pnamp_ctx *ctx, const double *q2);
+
+/* Orbits per species (the one-body density's dimension). */
+int pnamp_norb(pnamp_ctx *ctx);
+
+/* pnamp_kernels_batched with the E2 and one-body-density observables, in
+ * the plain, factored and mixed forms (plain fold only: PNAMP_E_ARG with the
+ * literal fold).  Besides zres and kern (layouts of pnamp_kernels_batched):
+ *  ze2   [npairs][n_euler][2][5] complex: per Euler point the norm-weighted
+ *        gauge sums of tr_s(Q_{2mu} rho) for species s (protons, neutrons),
+ *        the ZRES slots 9-13 of Kernel4Triaxial (ZQ2, PNAMP_ME1B.f90:52-75);
+ *  qred  [npairs][2][5][jmax+1][2jmax+5][2jmax+1] complex: reduced matrix
+ *        elements <J1 K1 q0 || Q2_s || J2 K2 q1> at (s, J1 - J2 + 2, J2,
+ *        K1 + jmax + 2, K2 + jmax), J1 - J2 = -2..2 (Kernel%qp / qpred,
+ *        PNAMP_Kernel3D.f90:590-710): the Wigner-Eckart reduced element,
+ *        <Phi0|P^{J1}_{K1M1} Q_{2mu} P^{J2}_{M2K2}|Phi1> =
+ *        (-1)^(J1-M1) (J1 2 J2; -M1 mu M2) <J1 K1 || Q2 || J2 K2>;
+ *  rho1b [npairs][2][norb][norb] complex: the J = 0 projected one-body
+ *        transition density ZRho1BJ0 (ZRho1B_Angles / ZRho1B_Integral,
+ *        Codes/PNAMP/PNAMP_Rho1B.f90:2-62), zero for orbits of different l, j.
+ * Any output may be NULL. */
+int pnamp_kernels_e2(pnamp_ctx *ctx, int npairs, const int32_t *q0, const int32_t *q1, int zval, int nval,
+                     double *zres, double *kern, double *ze2, double *qred, double *rho1b);
+
 /* Norm and energy of every gauge point: samples [npairs][NE][nphi][nphi][2]
  * complex = (norm, E + E0) per (pair, Euler point, phi_p, phi_n), norm as
  * summed into ZRES slot 1 before the 1/nphi^2 (PNAMP_Kernel3D.f90:294-324).

@@ -33,6 +33,7 @@ class OrCfg(C.Structure):
         ("nphi", C.c_int), ("jmax", C.c_int), ("fold", C.c_int), ("norm_method", C.c_int),
         ("cscale", C.c_double), ("iscale", C.c_int),
         ("zval", C.c_int), ("nval", C.c_int), ("nthreads", C.c_int), ("energy_method", C.c_int),
+        ("q2", _f64p),
     ]
 
 
@@ -58,6 +59,10 @@ def lib():
         L.or_point.restype = C.c_int
         L.or_zbareover.restype = C.c_int
         L.or_hndiag.restype = C.c_int
+        L.or_threej.restype = C.c_double
+        L.or_threej.argtypes = [C.c_int] * 6
+        L.or_norb.restype = C.c_int
+        L.or_kernel_pair_e2.restype = C.c_int
         _lib = L
     return _lib
 
@@ -70,9 +75,11 @@ class Oracle:
     """Holds one (basis, interaction, mesh, Z, N) configuration; keeps the
     numpy buffers alive for the C side."""
 
-    def __init__(self, basis, inter, mesh, zval: int, nval: int, nthreads: int = 0, energy_method: int = 0):
+    def __init__(self, basis, inter, mesh, zval: int, nval: int, nthreads: int = 0, energy_method: int = 0,
+                 q2=None):
         """energy_method 0: the list sum of HFB_Energy_COMPLEX (Method II, the
-        reference's path); 1: the field form (Method I)."""
+        reference's path); 1: the field form (Method I).  q2: the one-body
+        Q_{2mu} matrices [5, n, n] (mu = -2..2) for the E2 observables."""
         self._keep = []
         k = self._keep
 
@@ -96,11 +103,39 @@ class Oracle:
             mesh.alp.size, mesh.bet.size, mesh.gam.size,
             f64(mesh.alp), f64(mesh.walp), f64(mesh.bet), f64(mesh.wbet), f64(mesh.gam), f64(mesh.wgam),
             mesh.nphi, mesh.jmax, mesh.fold, mesh.norm_method, float(mesh.cscale), int(mesh.iscale),
-            int(zval), int(nval), int(nthreads), int(energy_method))
+            int(zval), int(nval), int(nthreads), int(energy_method),
+            None if q2 is None else _p(self._q2(q2)))
+
+    def _q2(self, q2):
+        a = np.ascontiguousarray(np.transpose(np.asarray(q2, dtype=np.float64), (0, 2, 1)))  # column-major
+        self._keep.append(a)
+        return a
 
     @property
     def n(self):
         return self.basis.n
+
+    @property
+    def norb(self):
+        return lib().or_norb(C.byref(self.cfg))
+
+    def kernel_pair_e2(self, U0, V0, U1, V1):
+        """(zres, kern, ze2 [NE, 2, 5], zrho [NE, 2, norb, norb],
+        qred [2, 5, jmax+1, 2jmax+5, 2jmax+1], rho1b [2, norb, norb])."""
+        m = self.mesh
+        NE, J2m, jm, nb = m.n_euler, 2 * m.jmax + 1, m.jmax, self.norb
+        zres = np.zeros((NE, NOBS), dtype=np.complex128)
+        kern = np.zeros((NOBS, jm + 1, J2m, J2m), dtype=np.complex128)
+        ze2 = np.zeros((NE, 2, 5), dtype=np.complex128)
+        zrho = np.zeros((NE, 2, nb, nb), dtype=np.complex128)
+        qred = np.zeros((2, 5, jm + 1, J2m + 4, J2m), dtype=np.complex128)
+        rho1b = np.zeros((2, nb, nb), dtype=np.complex128)
+        args = [np.asfortranarray(x, dtype=np.float64) for x in (U0, V0, U1, V1)]
+        info = lib().or_kernel_pair_e2(C.byref(self.cfg), *[_p(a) for a in args], _p(zres), _p(kern), _p(ze2),
+                                       _p(zrho), _p(qred), _p(rho1b))
+        if info:
+            raise RuntimeError(f"oracle kernel_pair_e2 failed: info={info}")
+        return zres, kern, ze2, zrho, qred, rho1b
 
     def kernel_pair(self, U0, V0, U1, V1):
         """Returns (zres [NE, 5], kern [5, jmax+1, 2jmax+1, 2jmax+1]) complex."""
@@ -143,6 +178,11 @@ class Oracle:
 
 def wigner_d(twoj: int, twom1: int, twom2: int, beta: float) -> float:
     return lib().or_wigner_d(twoj, twom1, twom2, beta)
+
+
+def threej(tj1, tj2, tj3, tm1, tm2, tm3) -> float:
+    """Wigner 3j symbol, arguments doubled."""
+    return lib().or_threej(tj1, tj2, tj3, tm1, tm2, tm3)
 
 
 def pfaffian(S: np.ndarray) -> complex:

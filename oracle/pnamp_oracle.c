@@ -239,6 +239,34 @@ double or_wigner_d(int tj, int tm1, int tm2, double beta) {
   return pre * sum;
 }
 
+/* Wigner 3j symbol (j1 j2 j3; m1 m2 m3), arguments doubled, Racah formula
+ * (the reference's wignei / Wigner_3j, Codes/GCM/VAPHFB_ThreeJ.f90). */
+double or_threej(int tj1, int tj2, int tj3, int tm1, int tm2, int tm3) {
+  init_fact();
+  if (tm1 + tm2 + tm3 != 0) return 0.0;
+  if (abs(tm1) > tj1 || abs(tm2) > tj2 || abs(tm3) > tj3) return 0.0;
+  if (((tj1 + tm1) & 1) || ((tj2 + tm2) & 1) || ((tj3 + tm3) & 1)) return 0.0;
+  if (tj3 > tj1 + tj2 || tj3 < abs(tj1 - tj2) || ((tj1 + tj2 + tj3) & 1)) return 0.0;
+  int a = (tj1 + tj2 - tj3) / 2, b = (tj1 - tj2 + tj3) / 2, cc = (-tj1 + tj2 + tj3) / 2;
+  double tri = fact_tab[a] * fact_tab[b] * fact_tab[cc] / fact_tab[(tj1 + tj2 + tj3) / 2 + 1];
+  double pre = sqrt(tri * fact_tab[(tj1 + tm1) / 2] * fact_tab[(tj1 - tm1) / 2] * fact_tab[(tj2 + tm2) / 2] *
+                    fact_tab[(tj2 - tm2) / 2] * fact_tab[(tj3 + tm3) / 2] * fact_tab[(tj3 - tm3) / 2]);
+  int kmin = 0, kmax = a;
+  int t1 = (tj3 - tj2 + tm1) / 2, t2 = (tj3 - tj1 - tm2) / 2;
+  if (-t1 > kmin) kmin = -t1;
+  if (-t2 > kmin) kmin = -t2;
+  if ((tj1 - tm1) / 2 < kmax) kmax = (tj1 - tm1) / 2;
+  if ((tj2 + tm2) / 2 < kmax) kmax = (tj2 + tm2) / 2;
+  double sum = 0.0;
+  for (int k = kmin; k <= kmax; ++k) {
+    double den = fact_tab[k] * fact_tab[a - k] * fact_tab[(tj1 - tm1) / 2 - k] * fact_tab[(tj2 + tm2) / 2 - k] *
+                 fact_tab[t1 + k] * fact_tab[t2 + k];
+    sum += (k & 1) ? -1.0 / den : 1.0 / den;
+  }
+  int ph = (tj1 - tj2 - tm3) / 2;
+  return ((ph & 1) ? -1.0 : 1.0) * pre * sum;
+}
+
 /* ---------------------------------------------------------------- reference routines */
 
 typedef struct {
@@ -261,6 +289,10 @@ typedef struct {
    * field form HFB_FIELD_COMPLEX + HFB_ENER_comp (Method I, the commented
    * alternative at PNAMP_Kernel3D.f90:309-310) */
   int energy_method;
+  /* E2 (iE2 = 1): the one-body Q_{2mu} matrices, [5][n*n] real column-major
+   * for mu = -2..2 (cME1B%Q2_2t .. Q22t of SingleParticleME,
+   * Codes/PNAMP/PNAMP_ME1B.f90:138-216), or NULL */
+  const double *q2;
 } or_cfg;
 
 /* ROTATION_MAT (Codes/PNAMP/PNAMP_Proj.f90:83-136). */
@@ -581,9 +613,51 @@ zc or_j2(const or_cfg *c, const zc *J, const zc *J2, const zc *ro, const zc *k10
 
 /* One Euler point: the Fomenko gauge loop of Kernel4Triaxial
  * (PNAMP_Kernel3D.f90:245-387), returns ZRES(iang, :) for the five observables. */
+/* Orbits of one species half (basis order, contiguous runs of equal n, l, j;
+ * both halves have the same orbits): first m-state, 2j, l.  Returns norb. */
+static int species_orbits(const or_cfg *c, int *first, int *tj, int *l) {
+  int h = c->n / 2, norb = 0;
+  for (int i = 0; i < h;) {
+    first[norb] = i;
+    tj[norb] = c->twoj[i];
+    l[norb] = c->l[i];
+    i += c->twoj[i] + 1;
+    ++norb;
+  }
+  return norb;
+}
+
+/* Per gauge point, E2 and one-body-density observables (ZQ2,
+ * Codes/PNAMP/PNAMP_ME1B.f90:52-75, and ZRho1B_Angles,
+ * Codes/PNAMP/PNAMP_Rho1B.f90:2-36):
+ *   q[s][mu] = sum_{i in s, j} Q_mu(i, j) rho(j, i)      (trace of Q_mu rho over species s)
+ *   d[s][a][b] = sum_m rho(k_a(m), k_b(m))              (orbits a, b of species s with equal l, j)
+ * q: [2][5], d: [2][norb][norb]. */
+static void e2_point(const or_cfg *c, const zc *ro, zc *q, zc *d) {
+  int n = c->n, h = n / 2;
+  for (int sp = 0; sp < 2; ++sp)
+    for (int mu = 0; mu < 5; ++mu) {
+      zc acc = 0;
+      const double *Q = c->q2 + (size_t)mu * n * n;
+      for (int i = sp * h; i < sp * h + h; ++i)
+        for (int j = 0; j < n; ++j) acc += Q[IX(i, j, n)] * ro[IX(j, i, n)];
+      q[sp * 5 + mu] = acc;
+    }
+  int first[64], tj[64], l[64];
+  int norb = species_orbits(c, first, tj, l);
+  for (int sp = 0; sp < 2; ++sp)
+    for (int a = 0; a < norb; ++a)
+      for (int b = 0; b < norb; ++b) {
+        zc acc = 0;
+        if (tj[a] == tj[b] && l[a] == l[b])
+          for (int m = 0; m <= tj[a]; ++m) acc += ro[IX(sp * h + first[a] + m, sp * h + first[b] + m, n)];
+        d[((size_t)sp * norb + a) * norb + b] = acc;
+      }
+}
+
 static int euler_point(const or_cfg *c, double alp, double bet, double gam, const double *U0,
                        const double *V0, const double *U1, const double *V1, const zc *Zc0,
-                       double det0, double det1, zc *zres) {
+                       double det0, double det1, zc *zres, zc *ze2, zc *zrho) {
   int n = c->n;
   size_t nn = (size_t)n * n;
   zc *R = malloc(sizeof(zc) * nn), *Ub = malloc(sizeof(zc) * nn), *Vb = malloc(sizeof(zc) * nn),
@@ -597,6 +671,10 @@ static int euler_point(const or_cfg *c, double alp, double bet, double gam, cons
   zc sum[NOBS] = {0};
   int nf = c->nphi;
   int info = 0;
+  int first[64], tjo[64], lo[64];
+  int norb = species_orbits(c, first, tjo, lo);
+  zc qs[10], *ds = malloc(sizeof(zc) * 2 * norb * norb);
+  zc sq[10] = {0}, *sd = calloc(2 * norb * norb, sizeof(zc));
   for (int lp = 0; lp < nf; ++lp) {
     double phip = (nf == 1) ? 0.0 : (2 * M_PI / nf) * lp;
     zc zpp = cexp(-I * phip * c->zval);
@@ -625,9 +703,20 @@ static int euler_point(const or_cfg *c, double alp, double bet, double gam, cons
       sum[3] += znorm * ek;
       sum[4] += znorm * (eh + c->e0);
       sum[5] += znorm * or_j2(c, J, J2, ro, k10, k01);
+      if (ze2 && c->q2) {
+        e2_point(c, ro, qs, ds);
+        for (int k = 0; k < 10; ++k) sq[k] += znorm * qs[k];
+        for (int k = 0; k < 2 * norb * norb; ++k) sd[k] += znorm * ds[k];
+      }
     }
   }
   for (int io = 0; io < NOBS; ++io) zres[io] = sum[io] / (double)(nf * nf);
+  if (ze2 && c->q2) {
+    for (int k = 0; k < 10; ++k) ze2[k] = sq[k] / (double)(nf * nf);
+    for (int k = 0; k < 2 * norb * norb; ++k) zrho[k] = sd[k] / (double)(nf * nf);
+  }
+  free(ds);
+  free(sd);
   free(R); free(Ub); free(Vb); free(A); free(ro); free(k10); free(k01); free(J); free(J2);
   return info;
 }
@@ -705,10 +794,124 @@ int or_kernel_pair(const or_cfg *c, const double *U0, const double *V0, const do
     int ig = iang % c->ngam, ia = (iang / c->ngam) % c->nalp, ib = iang / (c->ngam * c->nalp);
     double bet = (c->nbet == 1) ? 0.0 : c->bet[ib];
     err |= euler_point(c, c->alp[ia], bet, c->gam[ig], U0, V0, U1, V1, Zc0, det0, det1,
-                       zres + (size_t)iang * NOBS);
+                       zres + (size_t)iang * NOBS, NULL, NULL);
   }
   if (kern) or_project(c, zres, kern);
   if (!zres_out) free(zres);
+  free(Zc0); free(Zc1);
+  return err ? 1 : 0;
+}
+
+/* Reduced E2 matrix elements and the J = 0 one-body transition density of
+ * one pair (the iE2 = 1 part of Kernel4Triaxial).
+ *
+ * ze2 [NE][2][5]: per Euler point, norm-weighted gauge sums of tr_s(Q_{2mu} rho)
+ * (ZRES slots 9-13 of the reference, both species); zrho [NE][2][norb][norb]
+ * the same for the orbit densities (ZOBDJ0).
+ *
+ * qred [2][5][jmax+1][2jmax+5][2jmax+1] (species, J1 - J2 + 2, J2, K1 + jmax + 2,
+ * K2 + jmax), the reduced matrix elements <J1 K1 q0 || Q2 || J2 K2 q1> of
+ * PNAMP_Kernel3D.f90:590-710:
+ *   wJ(J2) (2 J1 + 1) (-1)^(J1 - K1) sum_mu (J1 2 J2; -K1 mu K1-mu)
+ *     sum_O w D^{J2*}_{K1-mu, K2}(O) ze2_mu(O),   |K1 - mu| <= J2,
+ * with D^{J*}_{K'K} = e^{i K' alp} d^J_{K'K}(bet) e^{i K gam} and wJ as in
+ * or_project (plain fold).  By 3j orthogonality this is the Wigner-Eckart
+ * reduced element: <Phi0| P^{J1}_{K1M1} Q_{2mu} P^{J2}_{M2K2} |Phi1> =
+ * (-1)^(J1-M1) (J1 2 J2; -M1 mu M2) <J1 K1 || Q2 || J2 K2>.
+ * rho1b [2][norb][norb]: ZRho1B_Integral (Codes/PNAMP/PNAMP_Rho1B.f90:39-62),
+ * wJ(0) sum_O w zrho(O) / sqrt(2 j_a + 1) (the 1/sqrt(2j+1) of
+ * ZRho1B_Angles, :29-30). */
+static void or_project_e2(const or_cfg *c, const zc *ze2, const zc *zrho, zc *qred, zc *rho1b) {
+  int na = c->nalp, nb = c->nbet, ng = c->ngam, jm = c->jmax;
+  int K1d = 2 * jm + 5, K2d = 2 * jm + 1;
+  int onept = (na * ng == 1);
+  memset(qred, 0, sizeof(zc) * 2 * 5 * (jm + 1) * K1d * K2d);
+  for (int J2 = 0; J2 <= jm; ++J2) {
+    double wJ = onept ? (2.0 * J2 + 1.0) / 2.0 : (2.0 * J2 + 1.0) / (8.0 * M_PI * M_PI);
+    for (int dJ = -2; dJ <= 2; ++dJ) {
+      int J1 = J2 + dJ;
+      if (J1 < 0) continue;
+      for (int K1 = -J1; K1 <= J1; ++K1)
+        for (int K2 = -J2; K2 <= J2; ++K2) {
+          zc acc[2] = {0, 0};
+          for (int mu = -2; mu <= 2; ++mu) {
+            int Kp = K1 - mu;
+            if (Kp < -J2 || Kp > J2) continue;
+            double tj = or_threej(2 * J1, 4, 2 * J2, -2 * K1, 2 * mu, 2 * Kp);
+            if (tj == 0.0) continue;
+            for (int ib = 0; ib < nb; ++ib) {
+              double bet = (nb == 1) ? 0.0 : c->bet[ib], wb = (nb == 1) ? 1.0 : c->wbet[ib];
+              double dp = or_wigner_d(2 * J2, 2 * Kp, 2 * K2, bet);
+              for (int ia = 0; ia < na; ++ia)
+                for (int ig = 0; ig < ng; ++ig) {
+                  int i1 = (ib * na + ia) * ng + ig;
+                  zc D = cexp(I * Kp * c->alp[ia]) * dp * cexp(I * K2 * c->gam[ig]);
+                  double w = c->walp[ia] * wb * c->wgam[ig];
+                  for (int sp = 0; sp < 2; ++sp) acc[sp] += tj * w * D * ze2[(size_t)i1 * 10 + sp * 5 + (mu + 2)];
+                }
+            }
+          }
+          double ph = ((J1 - K1) & 1) ? -1.0 : 1.0;
+          for (int sp = 0; sp < 2; ++sp)
+            qred[((((size_t)sp * 5 + (dJ + 2)) * (jm + 1) + J2) * K1d + (K1 + jm + 2)) * K2d + (K2 + jm)] =
+                wJ * (2 * J1 + 1) * ph * acc[sp];
+        }
+    }
+  }
+  int first[64], tj[64], lo[64];
+  int norb = species_orbits(c, first, tj, lo);
+  double w0 = onept ? 0.5 : 1.0 / (8.0 * M_PI * M_PI);
+  for (int k = 0; k < 2 * norb * norb; ++k) {
+    zc acc = 0;
+    for (int ib = 0; ib < nb; ++ib)
+      for (int ia = 0; ia < na; ++ia)
+        for (int ig = 0; ig < ng; ++ig) {
+          double wb = (nb == 1) ? 1.0 : c->wbet[ib];
+          acc += c->walp[ia] * wb * c->wgam[ig] * zrho[(size_t)((ib * na + ia) * ng + ig) * 2 * norb * norb + k];
+        }
+    int a = (k / norb) % norb;
+    rho1b[k] = w0 * acc / sqrt(tj[a] + 1.0);
+  }
+}
+
+int or_norb(const or_cfg *c) {
+  int first[64], tj[64], l[64];
+  return species_orbits(c, first, tj, l);
+}
+
+/* Kernel4Triaxial with iE2 = 1 for one pair: the outputs of or_kernel_pair
+ * plus ze2, zrho, qred, rho1b (layouts at or_project_e2; any may be NULL
+ * except that qred / rho1b need c->q2). */
+int or_kernel_pair_e2(const or_cfg *c, const double *U0, const double *V0, const double *U1, const double *V1,
+                      zc *zres_out, zc *kern, zc *ze2_out, zc *zrho_out, zc *qred, zc *rho1b) {
+  if (!c->q2) return -1;
+  int n = c->n;
+  int NE = c->nalp * c->nbet * c->ngam;
+  int norb = or_norb(c);
+  zc *Zc0 = malloc(sizeof(zc) * n * n), *Zc1 = malloc(sizeof(zc) * n * n);
+  double det0 = 1, det1 = 1;
+  int info = or_zbareover(n, U0, V0, Zc0, &det0);
+  if (!info) info = or_zbareover(n, U1, V1, Zc1, &det1);
+  if (info) { free(Zc0); free(Zc1); return info; }
+  zc *zres = zres_out ? zres_out : malloc(sizeof(zc) * NE * NOBS);
+  zc *ze2 = ze2_out ? ze2_out : malloc(sizeof(zc) * NE * 10);
+  zc *zrho = zrho_out ? zrho_out : malloc(sizeof(zc) * NE * 2 * norb * norb);
+  int err = 0;
+#ifdef _OPENMP
+  int nt = c->nthreads > 0 ? c->nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt) reduction(| : err)
+#endif
+  for (int iang = 0; iang < NE; ++iang) {
+    int ig = iang % c->ngam, ia = (iang / c->ngam) % c->nalp, ib = iang / (c->ngam * c->nalp);
+    double bet = (c->nbet == 1) ? 0.0 : c->bet[ib];
+    err |= euler_point(c, c->alp[ia], bet, c->gam[ig], U0, V0, U1, V1, Zc0, det0, det1,
+                       zres + (size_t)iang * NOBS, ze2 + (size_t)iang * 10, zrho + (size_t)iang * 2 * norb * norb);
+  }
+  if (kern) or_project(c, zres, kern);
+  if (qred && rho1b) or_project_e2(c, ze2, zrho, qred, rho1b);
+  if (!zres_out) free(zres);
+  if (!ze2_out) free(ze2);
+  if (!zrho_out) free(zrho);
   free(Zc0); free(Zc1);
   return err ? 1 : 0;
 }

@@ -498,9 +498,18 @@ struct PointParams {
   const double2 *jpack;             // [6][npack]: J_x, J_y, J_z, J_x^2, J_y^2, J_z^2 blocks
   double2 *zres;                    // [npairs][NE][NOBS]
   int *status;                      // [npairs][NE]
+  // E2 observables (k_point<true>): Q_{2mu} [5][n*n] column-major; orbit
+  // pairs [nob2] (species, first m-state of a, of b, size; size 0 where
+  // l or j differ); outputs [npairs][NE][10] and [npairs][NE][nob2]
+  const double *q2;
+  const int4 *orbp;
+  int nob2;
+  double2 *ze2, *zrho;
 };
 
+constexpr int NOB2_MAX = 2 * 8 * 8;   // orbit pairs of both species (<= 8 orbits per species)
 
+template <bool WITH_E2>
 __global__ __launch_bounds__(NT) void k_point(PointParams p) {
   __shared__ double2 sm[5 * NMAX * NMAX];
   __shared__ double2 sR[RPACK_MAX];
@@ -513,6 +522,10 @@ __global__ __launch_bounds__(NT) void k_point(PointParams p) {
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int iang = blockIdx.x, ip = blockIdx.y;
+  double2 accq[WITH_E2 ? 10 : 1];  // tid 0: gauge sums of znorm tr_s(Q_mu rho)
+  double2 accd = cz();        // tid < nob2: gauge sum of znorm times one orbit-pair trace
+  if (WITH_E2)
+    for (int k = 0; k < 10; ++k) accq[k] = cz();
   const int n = p.n, h = n >> 1, nn = n * n, N2 = 2 * n;
   const int ig = iang % p.ngam, ia = (iang / p.ngam) % p.nalp, ib = iang / (p.ngam * p.nalp);
   const double alp = p.alp[ia], bet = p.bet[ib], gam = p.gam[ig];
@@ -729,6 +742,43 @@ __global__ __launch_bounds__(NT) void k_point(PointParams p) {
       acc[4] = cadd(acc[4], cmul(znorm, cadd(cadd(E2, EK), make_double2(p.e0, 0.0))));
       acc[5] = cadd(acc[5], cmul(znorm, J2));
     }
+    if (WITH_E2) {
+      double *rede2 = (double *)prow;  // NWAVE x 20 doubles; prow is free after the Gauss-Jordan
+      // tr_s(Q_mu rho) = sum_{i, j in s} Q_mu(i, j) rho(j, i) (ZQ2, PNAMP_ME1B.f90:52-75)
+      double q[20];
+      for (int k = 0; k < 20; ++k) q[k] = 0.0;
+      for (int e = tid; e < 2 * h * h; e += NT) {
+        const int sp = e / (h * h), r = e - sp * h * h;
+        const int i = sp * h + r % h, j = sp * h + r / h;
+        const double2 rji = ro[j + i * n];
+        for (int mu = 0; mu < 5; ++mu) {
+          const double qv = p.q2[(size_t)mu * nn + i + (size_t)j * n];
+          q[sp * 10 + 2 * mu] = fma(qv, rji.x, q[sp * 10 + 2 * mu]);
+          q[sp * 10 + 2 * mu + 1] = fma(qv, rji.y, q[sp * 10 + 2 * mu + 1]);
+        }
+      }
+      for (int k = 0; k < 20; ++k)
+        for (int off = 32; off > 0; off >>= 1) q[k] += __shfl_xor(q[k], off);
+      if (lane == 0)
+        for (int k = 0; k < 20; ++k) rede2[wave * 20 + k] = q[k];
+      // orbit-pair traces sum_m rho(k_a(m), k_b(m)) (ZRho1B_Angles, PNAMP_Rho1B.f90:13-33)
+      if (tid < p.nob2) {
+        const int4 op = p.orbp[tid];
+        double2 t = cz();
+        for (int m = 0; m < op.w; ++m) t = cadd(t, ro[(op.y + m) + (op.z + m) * n]);
+        accd = cadd(accd, cmul(znorm, t));
+      }
+      __syncthreads();
+      if (tid == 0)
+        for (int k = 0; k < 10; ++k) {
+          double re = 0.0, im = 0.0;
+          for (int w = 0; w < NWAVE; ++w) {
+            re += rede2[w * 20 + 2 * k];
+            im += rede2[w * 20 + 2 * k + 1];
+          }
+          accq[k] = cadd(accq[k], cmul(znorm, make_double2(re, im)));
+        }
+    }
     __syncthreads();
   }
   // the singular-T flag is per thread; combine through LDS
@@ -743,7 +793,11 @@ __global__ __launch_bounds__(NT) void k_point(PointParams p) {
     double2 *out = p.zres + ((size_t)ip * p.NE + iang) * NOBS;
     for (int o = 0; o < NOBS; ++o) out[o] = cscale(inv, acc[o]);
     p.status[(size_t)ip * p.NE + iang] = st;
+    if (WITH_E2)
+      for (int k = 0; k < 10; ++k) p.ze2[((size_t)ip * p.NE + iang) * 10 + k] = cscale(inv, accq[k]);
   }
+  if (WITH_E2 && tid < p.nob2)
+    p.zrho[((size_t)ip * p.NE + iang) * p.nob2 + tid] = cscale(1.0 / (double)(nphi * nphi), accd);
 }
 
 // ------------------------------------------------------------------ factored form
@@ -1601,6 +1655,141 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
   }
 }
 
+// E2 observables of the factored form, one workgroup per (pair, Euler point),
+// launched after k_fac_combine: the species densities rho_s(phi) that
+// k_fac_phi wrote give, per species and gauge angle, tr_s(Q_mu rho_s(phi))
+// (ZQ2, Codes/PNAMP/PNAMP_ME1B.f90:52-75) and the orbit-pair traces
+// sum_m rho(k_a(m), k_b(m)) (ZRho1B_Angles, Codes/PNAMP/PNAMP_Rho1B.f90:13-33).
+// A proton quantity depends on phi_p only, so its gauge sum weighs it with the
+// norm summed over phi_n (and a neutron one with the norm summed over phi_p).
+struct FacE2Params {
+  int n, h, NE, nphi, zval, nval, nob2;
+  double vac_scale, snpfaf, sgn_h;
+  const double *det;
+  const int *q0, *q1;
+  const double2 *out, *rho;          // k_fac_phi outputs
+  const double *q2;                  // [5][n*n] column-major
+  const int4 *orbp;                  // [nob2]: species, first m-state of a, of b (global), size
+  double2 *ze2, *zrho;               // [npairs][NE][10], [npairs][NE][nob2]
+};
+
+__global__ __launch_bounds__(256) void k_fac_e2(FacE2Params p) {
+  __shared__ double2 zn[NPHI_MAX * NPHI_MAX];
+  __shared__ double2 wsum[2][NPHI_MAX];        // norm summed over phi_n (per phi_p), over phi_p (per phi_n)
+  __shared__ double2 tq[2][NPHI_MAX][5];
+  __shared__ double2 td[NPHI_MAX][NOB2_MAX];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int iang = blockIdx.x, ip = blockIdx.y;
+  const int h = p.h, hh = h * h, n = p.n, np = p.nphi;
+  const size_t base = ((size_t)ip * p.NE + iang) * 2;
+  const double vac = p.snpfaf * p.sgn_h * (p.det[p.q0[ip]] * p.vac_scale) * (p.det[p.q1[ip]] * p.vac_scale);
+  for (int t = tid; t < np * np; t += blockDim.x) {
+    const int fp = t / np, fn = t % np;
+    const double phip = (np == 1) ? 0.0 : (2.0 * M_PI / np) * fp;
+    const double phin = (np == 1) ? 0.0 : (2.0 * M_PI / np) * fn;
+    const double2 a = p.out[(base * np + fp) * FAC_OUT], b = p.out[((base + 1) * np + fn) * FAC_OUT];
+    zn[t] = cmul(cscale(vac, cmul(a, b)), cexpi(-phip * p.zval - phin * p.nval));
+  }
+  // one wave per (species, phi): tr_s(Q_mu rho_s(phi)) for the five mu
+  for (int u = wave; u < 2 * np; u += blockDim.x / 64) {
+    const int sp = u / np, f = u % np;
+    const double2 *r = p.rho + ((base + sp) * np + f) * hh;
+    double q[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int e = lane; e < hh; e += 64) {
+      const int a = e % h, b = e / h;
+      const double2 rba = r[b + a * h];  // rho(b, a), column-major h x h
+      const size_t qi = (size_t)(sp * h + a) + (size_t)(sp * h + b) * n;
+      for (int mu = 0; mu < 5; ++mu) {
+        const double qv = p.q2[(size_t)mu * n * n + qi];
+        q[2 * mu] = fma(qv, rba.x, q[2 * mu]);
+        q[2 * mu + 1] = fma(qv, rba.y, q[2 * mu + 1]);
+      }
+    }
+    for (int k = 0; k < 10; ++k) q[k] = wave_sum_dpp(q[k]);
+    if (lane < 5) tq[sp][f][lane] = make_double2(q[2 * lane], q[2 * lane + 1]);
+  }
+  for (int t = tid; t < np * p.nob2; t += blockDim.x) {
+    const int f = t / p.nob2, k = t % p.nob2;
+    const int4 op = p.orbp[k];
+    const double2 *r = p.rho + ((base + op.x) * np + f) * hh;
+    const int fa = op.y - op.x * h, fb = op.z - op.x * h;
+    double2 acc = cz();
+    for (int m = 0; m < op.w; ++m) acc = cadd(acc, r[(fa + m) + (fb + m) * h]);
+    td[f][k] = acc;
+  }
+  __syncthreads();
+  if (tid < 2 * np) {
+    const int sp = tid / np, f = tid % np;
+    double2 acc = cz();
+    for (int g = 0; g < np; ++g) acc = cadd(acc, sp == 0 ? zn[f * np + g] : zn[g * np + f]);
+    wsum[sp][f] = acc;
+  }
+  __syncthreads();
+  const double inv = 1.0 / (double)(np * np);
+  if (tid < 10) {
+    const int sp = tid / 5, mu = tid % 5;
+    double2 acc = cz();
+    for (int f = 0; f < np; ++f) acc = cadd(acc, cmul(wsum[sp][f], tq[sp][f][mu]));
+    p.ze2[((size_t)ip * p.NE + iang) * 10 + tid] = cscale(inv, acc);
+  }
+  for (int k = tid; k < p.nob2; k += blockDim.x) {
+    const int sp = p.orbp[k].x;
+    double2 acc = cz();
+    for (int f = 0; f < np; ++f) acc = cadd(acc, cmul(wsum[sp][f], td[f][k]));
+    p.zrho[((size_t)ip * p.NE + iang) * p.nob2 + k] = cscale(inv, acc);
+  }
+}
+
+// Reduced E2 matrix elements from the projected E2 observables
+// I_mu(J2, K', K2) = <Phi0| Q_{2mu} P^{J2}_{K' K2} |Phi1> (the plain
+// projection of ze2): qred(s, J1 - J2, J2, K1, K2) = sum_mu T(J1 - J2, J2, K1, mu)
+// I_{s,mu}(J2, K1 - mu, K2), T = (2 J1 + 1) (-1)^(J1 - K1) (J1 2 J2; -K1 mu K1 - mu)
+// tabled on the host (the sums of Codes/PNAMP/PNAMP_Kernel3D.f90:590-710).
+// One workgroup per pair; rho1b: wJ(0) sum_O w zrho(O) / sqrt(2 j_a + 1)
+// (ZRho1B_Integral, Codes/PNAMP/PNAMP_Rho1B.f90:39-62).
+struct E2RedParams {
+  int jmax, NE, nob2, nalp, ngam;
+  const double2 *iproj;              // [npairs][10][jmax+1][2jmax+1][2jmax+1]
+  const double *t3j;                 // [5][jmax+1][2jmax+5][5]
+  double2 *qred;                     // [npairs][2][5][jmax+1][2jmax+5][2jmax+1]
+  const double2 *zrho;               // [npairs][NE][nob2]
+  const double *walp, *wbet, *wgam;
+  const double *orbscale;            // [nob2]: wJ(0) / sqrt(2 j_a + 1)
+  double2 *rho1b;                    // [npairs][nob2]
+};
+
+__global__ __launch_bounds__(256) void k_e2_reduce(E2RedParams p) {
+  const int ip = blockIdx.x, tid = threadIdx.x;
+  const int jm = p.jmax, J2m = 2 * jm + 1, K1d = 2 * jm + 5;
+  const size_t per_io = (size_t)(jm + 1) * J2m * J2m;
+  const double2 *I = p.iproj + (size_t)ip * 10 * per_io;
+  const int nq = 2 * 5 * (jm + 1) * K1d * J2m;
+  for (int o = tid; o < nq; o += blockDim.x) {
+    const int K2 = o % J2m - jm, K1 = (o / J2m) % K1d - jm - 2, J2 = (o / (J2m * K1d)) % (jm + 1);
+    const int dJ = (o / (J2m * K1d * (jm + 1))) % 5 - 2, sp = o / (J2m * K1d * (jm + 1) * 5);
+    const int J1 = J2 + dJ;
+    double2 acc = cz();
+    if (J1 >= 0 && K1 >= -J1 && K1 <= J1 && K2 >= -J2 && K2 <= J2) {
+      for (int mu = -2; mu <= 2; ++mu) {
+        const int Kp = K1 - mu;
+        if (Kp < -J2 || Kp > J2) continue;
+        const double t = p.t3j[(((size_t)(dJ + 2) * (jm + 1) + J2) * K1d + (K1 + jm + 2)) * 5 + (mu + 2)];
+        const double2 v = I[(size_t)(sp * 5 + mu + 2) * per_io + ((size_t)J2 * J2m + (Kp + jm)) * J2m + (K2 + jm)];
+        acc = cadd(acc, cscale(t, v));
+      }
+    }
+    p.qred[(size_t)ip * nq + o] = acc;
+  }
+  for (int k = tid; k < p.nob2; k += blockDim.x) {
+    double2 acc = cz();
+    for (int i = 0; i < p.NE; ++i) {
+      const int ig = i % p.ngam, ia = (i / p.ngam) % p.nalp, ib = i / (p.ngam * p.nalp);
+      acc = cadd(acc, cscale(p.walp[ia] * p.wbet[ib] * p.wgam[ig], p.zrho[((size_t)ip * p.NE + i) * p.nob2 + k]));
+    }
+    p.rho1b[(size_t)ip * p.nob2 + k] = cscale(p.orbscale[k], acc);
+  }
+}
+
 // ------------------------------------------------------------------ V contraction on MFMA (mixed forms)
 // The quadratic forms of the V list as dense h^2 x h^2 matrices applied to
 // all density columns of a chunk at once, as in HFB_FIELD_COMPLEX
@@ -1834,6 +2023,7 @@ static bool launch_combine_np(int np, bool gmix, dim3 grid, size_t lds, hipStrea
 // ------------------------------------------------------------------ projection
 struct ProjParams {
   int nalp, nbet, ngam, NE, jmax, fold, npj;
+  int nobs;                  // observables per Euler point in zres (and per pair in kern)
   const double *alp, *walp, *wbet, *gam, *wgam;
   const double *ptab;        // [nbet][2][npj]
   const double2 *zres;       // [npairs][NE][NOBS]
@@ -1853,8 +2043,8 @@ __global__ __launch_bounds__(256) void k_project(ProjParams p) {
   for (int i = tid; i < J2m * p.nalp; i += blockDim.x) ea[i] = cexpi((i / p.nalp - p.jmax) * p.alp[i % p.nalp]);
   for (int i = tid; i < J2m * p.ngam; i += blockDim.x) eg[i] = cexpi((i / p.ngam - p.jmax) * p.gam[i % p.ngam]);
   __syncthreads();
-  const double2 *z = p.zres + (size_t)ip * p.NE * NOBS;
-  double2 *kout = p.kern + ((size_t)ip * NOBS + io) * (p.jmax + 1) * J2m * J2m;
+  const double2 *z = p.zres + (size_t)ip * p.NE * p.nobs;
+  double2 *kout = p.kern + ((size_t)ip * p.nobs + io) * (p.jmax + 1) * J2m * J2m;
   const bool onept = (p.nalp * p.ngam == 1);
   const int ntot = (p.jmax + 1) * J2m * J2m;
   for (int o = tid; o < ntot; o += blockDim.x) {
@@ -1880,7 +2070,7 @@ __global__ __launch_bounds__(256) void k_project(ProjParams p) {
             const double2 zap = ea[(K1 + p.jmax) * p.nalp + ia], zgp = eg[(K2 + p.jmax) * p.ngam + ig];
             if (p.fold == PNAMP_FOLD_PLAIN) {
               const double2 D = cscale(dp * w * wJ, cmul(zap, zgp));
-              cfma(acc, D, z[(size_t)i1 * NOBS + io]);
+              cfma(acc, D, z[(size_t)i1 * p.nobs + io]);
             } else {
               const int i2 = (ib * p.nalp + ia) * p.ngam + (p.ngam - 1 - ig);
               const int i3 = (ib * p.nalp + (p.nalp - 1 - ia)) * p.ngam + ig;
@@ -1888,10 +2078,10 @@ __global__ __launch_bounds__(256) void k_project(ProjParams p) {
               const double2 zam = cconj(zap);
               const double2 za1 = cscale(dp, cmul(zap, zgp)), za2 = cscale(dp, cmul(zam, zgp));
               const double2 za3 = cscale(dm, cmul(zap, zgp)), za4 = cscale(dm, cmul(zam, zgp));
-              double2 s = cmul(za1, z[(size_t)i1 * NOBS + io]);
-              s = cadd(s, cmul(za2, cconj(z[(size_t)i2 * NOBS + io])));
-              s = cadd(s, cmul(za3, cconj(z[(size_t)i3 * NOBS + io])));
-              s = cadd(s, cmul(za4, z[(size_t)i4 * NOBS + io]));
+              double2 s = cmul(za1, z[(size_t)i1 * p.nobs + io]);
+              s = cadd(s, cmul(za2, cconj(z[(size_t)i2 * p.nobs + io])));
+              s = cadd(s, cmul(za3, cconj(z[(size_t)i3 * p.nobs + io])));
+              s = cadd(s, cmul(za4, z[(size_t)i4 * p.nobs + io]));
               acc = cadd(acc, cscale(wJ * w / d0, s));
             }
           }
@@ -1931,15 +2121,15 @@ __global__ __launch_bounds__(256) void k_project_sep(ProjParams p) {
           *zag = zg + (size_t)na * J2m;
   for (int i = tid; i < J2m * na; i += blockDim.x) ea[i] = cexpi((i / na - p.jmax) * p.alp[i % na]);
   for (int i = tid; i < J2m * ng; i += blockDim.x) eg[i] = cexpi((i / ng - p.jmax) * p.gam[i % ng]);
-  const double2 *z = p.zres + (size_t)ip * p.NE * NOBS;
-  double2 *kout = p.kern + ((size_t)ip * NOBS + io) * (p.jmax + 1) * J2m * J2m;
+  const double2 *z = p.zres + (size_t)ip * p.NE * p.nobs;
+  double2 *kout = p.kern + ((size_t)ip * p.nobs + io) * (p.jmax + 1) * J2m * J2m;
   const int ntot = (p.jmax + 1) * J2m * J2m;
   double2 acc[PROJ_MAXO];
 #pragma unroll
   for (int m = 0; m < PROJ_MAXO; ++m) acc[m] = cz();
   for (int ib = 0; ib < p.nbet; ++ib) {
     __syncthreads();
-    for (int i = tid; i < na * ng; i += blockDim.x) zb[i] = z[((size_t)ib * na * ng + i) * NOBS + io];
+    for (int i = tid; i < na * ng; i += blockDim.x) zb[i] = z[((size_t)ib * na * ng + i) * p.nobs + io];
     __syncthreads();
     for (int i = tid; i < na * J2m; i += blockDim.x) {
       const int a = i / J2m, k2 = i % J2m;
@@ -2266,6 +2456,15 @@ struct pnamp_ctx {
   double2 *d_zres = nullptr, *d_kern = nullptr;
   int *d_status = nullptr, *d_pair_status = nullptr;
   // factored form
+  // E2 observables (pnamp_set_q2): Q_{2mu}, orbit pairs, 3j table and the
+  // per-call outputs (ZRES of the ten E2 observables and the orbit traces,
+  // their projections, reduced elements, J = 0 one-body density)
+  std::vector<int> twoj_h, l_h, nq_h;
+  double *d_q2 = nullptr, *d_t3j = nullptr, *d_orbscale = nullptr;
+  int4 *d_orbp = nullptr;
+  int nob2 = 0, norb = 0;
+  double2 *d_ze2 = nullptr, *d_zrho = nullptr, *d_iproj = nullptr, *d_qred = nullptr, *d_rho1b = nullptr;
+  int e2_cap = 0;
   int form = PNAMP_FORM_PLAIN;
   bool separable = false;
   int nme_s[3] = {0, 0, 0};
@@ -2484,6 +2683,9 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
   c->h = h;
   c->npack = (int)pa.size();
   c->twom_h.assign(basis->twom, basis->twom + h);
+  c->twoj_h.assign(basis->twoj, basis->twoj + h);
+  c->l_h.assign(basis->l, basis->l + h);
+  c->nq_h.assign(basis->nq, basis->nq + h);
   c->nalp = mesh->nalp;
   c->nbet = mesh->nbet;
   c->ngam = mesh->ngam;
@@ -2800,7 +3002,7 @@ static hipError_t launch_project(pnamp_ctx *c, int npairs, const ProjParams &pr)
       const hipError_t e = hipFuncSetAttribute((const void *)k_project_sep, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_project_sep, dim3(npairs, NOBS), dim3(256), lds, c->stream, pr);
+    hipLaunchKernelGGL(k_project_sep, dim3(npairs, pr.nobs), dim3(256), lds, c->stream, pr);
     return hipGetLastError();
   }
   const size_t lds = sizeof(double2) * (2 * (size_t)c->jmax + 1) * (c->nalp + c->ngam);
@@ -2808,7 +3010,7 @@ static hipError_t launch_project(pnamp_ctx *c, int npairs, const ProjParams &pr)
     const hipError_t e = hipFuncSetAttribute((const void *)k_project, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_project, dim3(npairs, NOBS), dim3(256), lds, c->stream, pr);
+  hipLaunchKernelGGL(k_project, dim3(npairs, pr.nobs), dim3(256), lds, c->stream, pr);
   return hipGetLastError();
 }
 
@@ -2837,22 +3039,48 @@ static PointParams point_params(const pnamp_ctx *c, int zval, int nval, const in
   pp.blk_off = c->d_blk_off; pp.blk_sz = c->d_blk_sz; pp.blk_base = c->d_blk_base; pp.twom = c->d_twom;
   pp.q0 = q0; pp.q1 = q1; pp.vlist = c->d_vlist; pp.t1b = c->d_t1b; pp.jpack = c->d_jpack;
   pp.zres = zres; pp.status = status;
+  pp.q2 = nullptr; pp.orbp = nullptr; pp.nob2 = 0; pp.ze2 = nullptr; pp.zrho = nullptr;
   return pp;
 }
 
 static ProjParams proj_params(const pnamp_ctx *c, const double2 *zres, const int *status, double2 *kern,
                               int *pair_status) {
-  return ProjParams{c->nalp, c->nbet, c->ngam, c->NE, c->jmax, c->fold, c->npj, c->d_alp, c->d_walp, c->d_wbet,
-                    c->d_gam, c->d_wgam, c->d_ptab, zres, status, kern, pair_status};
+  return ProjParams{c->nalp, c->nbet, c->ngam, c->NE, c->jmax, c->fold, c->npj, NOBS, c->d_alp, c->d_walp,
+                    c->d_wbet, c->d_gam, c->d_wgam, c->d_ptab, zres, status, kern, pair_status};
 }
 
 // The batched kernels; with samples (device, [npairs][NE][nphi][nphi][2]) the
 // factored forms also write the norm and energy of every gauge pair, and run
 // their launches directly instead of through the captured graph.
+struct E2Out {
+  double *ze2, *qred, *rho1b;  // host outputs (any may be NULL)
+};
+
+static int ensure_e2(pnamp_ctx *c, int np) {
+  if (np <= c->e2_cap) return 0;
+  const size_t J2m = 2 * (size_t)c->jmax + 1, per_io = (size_t)(c->jmax + 1) * J2m * J2m;
+  int r = 0;
+  if ((r = dalloc(c, &c->d_ze2, (size_t)np * c->NE * 10)) || (r = dalloc(c, &c->d_zrho, (size_t)np * c->NE * c->nob2)) ||
+      (r = dalloc(c, &c->d_iproj, (size_t)np * 10 * per_io)) ||
+      (r = dalloc(c, &c->d_qred, (size_t)np * 2 * 5 * (c->jmax + 1) * (J2m + 4) * J2m)) ||
+      (r = dalloc(c, &c->d_rho1b, (size_t)np * c->nob2)))
+    return r;
+  c->e2_cap = np;
+  return 0;
+}
+
 static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32_t *q1, int zval, int nval,
-                        double *zres, double *kern, double2 *samples) {
+                        double *zres, double *kern, double2 *samples, const E2Out *e2 = nullptr) {
   if (!c || npairs <= 0 || !q0 || !q1) return PNAMP_E_ARG;
   if (!c->nstates) return fail(c, PNAMP_E_STATE, "no states set");
+  if (e2) {
+    if (!c->d_q2) return fail(c, PNAMP_E_STATE, "E2 kernels need the Q2 matrices (pnamp_set_q2)");
+    if (c->fold != PNAMP_FOLD_PLAIN)
+      return fail(c, PNAMP_E_ARG, "E2 kernels are projected with the plain fold only");
+    if (npairs > 65535) return fail(c, PNAMP_E_SIZE, "at most 65535 pairs per call");
+    int r2 = ensure_e2(c, npairs);
+    if (r2) return r2;
+  }
   for (int i = 0; i < npairs; ++i)
     if (q0[i] < 0 || q1[i] < 0 || q0[i] >= c->nstates || q1[i] >= c->nstates)
       return fail(c, PNAMP_E_ARG, "pair index out of range");
@@ -2862,7 +3090,10 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipMemcpyAsync(c->d_q0, q0, sizeof(int) * npairs, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_q1, q1, sizeof(int) * npairs, hipMemcpyHostToDevice, c->stream));
-  const PointParams pp = point_params(c, zval, nval, c->d_q0, c->d_q1, c->d_zres, c->d_status);
+  PointParams pp = point_params(c, zval, nval, c->d_q0, c->d_q1, c->d_zres, c->d_status);
+  if (e2) {
+    pp.q2 = c->d_q2; pp.orbp = c->d_orbp; pp.nob2 = c->nob2; pp.ze2 = c->d_ze2; pp.zrho = c->d_zrho;
+  }
   if (c->form != PNAMP_FORM_PLAIN) {
     if (!c->separable)
       return fail(c, PNAMP_E_ARG, "factored and mixed forms need states without proton-neutron mixing");
@@ -2958,7 +3189,11 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
     if (c->fac_graph && (c->g_np != npairs || c->g_z != zval || c->g_n != nval || c->g_form != c->form))
       drop_graph(c);
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    const bool direct = samples != nullptr;
+    const bool direct = samples != nullptr || e2 != nullptr;
+    FacE2Params ep;
+    ep.n = c->n; ep.h = h; ep.NE = c->NE; ep.nphi = c->nphi; ep.zval = zval; ep.nval = nval; ep.nob2 = c->nob2;
+    ep.vac_scale = c->vac_scale; ep.snpfaf = c->snpfaf; ep.sgn_h = (h & 1) ? -1.0 : 1.0; ep.det = c->d_det;
+    ep.out = c->d_fout; ep.rho = c->d_frho; ep.q2 = c->d_q2; ep.orbp = c->d_orbp;
     if (direct || !c->fac_graph) {
       // the launches of all chunks: captured once into a graph that later
       // calls of the same shape and form replay (the pair lists are copied
@@ -2983,6 +3218,12 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
           launch_vgemm(h, mix, dim3((ncols + 127) / 128, NVPROB), c->stream, vp);
         }
         launch_combine_np(c->nphi, mix != MIX_FP64, dim3(c->NE, np), comb_lds, c->stream, cp);
+        if (e2) {
+          ep.q0 = c->d_q0 + p0; ep.q1 = c->d_q1 + p0;
+          ep.ze2 = c->d_ze2 + (size_t)p0 * c->NE * 10;
+          ep.zrho = c->d_zrho + (size_t)p0 * c->NE * c->nob2;
+          hipLaunchKernelGGL(k_fac_e2, dim3(c->NE, np), dim3(256), 0, c->stream, ep);
+        }
       }
     }
     if (!direct && !c->fac_graph) {
@@ -3005,12 +3246,33 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   } else {
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    hipLaunchKernelGGL(k_point, dim3(c->NE, npairs), dim3(NT), 0, c->stream, pp);
+    if (e2) hipLaunchKernelGGL(k_point<true>, dim3(c->NE, npairs), dim3(NT), 0, c->stream, pp);
+    else hipLaunchKernelGGL(k_point<false>, dim3(c->NE, npairs), dim3(NT), 0, c->stream, pp);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   }
   const ProjParams pr = proj_params(c, c->d_zres, c->d_status, c->d_kern, c->d_pair_status);
   HIPCHK(c, launch_project(c, npairs, pr));
+  if (e2) {
+    // the ten E2 observables through the plain projection, then the 3j sums
+    ProjParams pe = proj_params(c, c->d_ze2, c->d_status, c->d_iproj, c->d_pair_status);
+    pe.nobs = 10;
+    HIPCHK(c, launch_project(c, npairs, pe));
+    E2RedParams rp{c->jmax, c->NE, c->nob2, c->nalp, c->ngam, c->d_iproj, c->d_t3j, c->d_qred, c->d_zrho,
+                   c->d_walp, c->d_wbet, c->d_wgam, c->d_orbscale, c->d_rho1b};
+    hipLaunchKernelGGL(k_e2_reduce, dim3(npairs), dim3(256), 0, c->stream, rp);
+    HIPCHK(c, hipGetLastError());
+    const size_t J2m = 2 * (size_t)c->jmax + 1;
+    if (e2->ze2)
+      HIPCHK(c, hipMemcpyAsync(e2->ze2, c->d_ze2, sizeof(double2) * (size_t)npairs * c->NE * 10, hipMemcpyDeviceToHost,
+                               c->stream));
+    if (e2->qred)
+      HIPCHK(c, hipMemcpyAsync(e2->qred, c->d_qred, sizeof(double2) * (size_t)npairs * 10 * (c->jmax + 1) * (J2m + 4) * J2m,
+                               hipMemcpyDeviceToHost, c->stream));
+    if (e2->rho1b)
+      HIPCHK(c, hipMemcpyAsync(e2->rho1b, c->d_rho1b, sizeof(double2) * (size_t)npairs * c->nob2, hipMemcpyDeviceToHost,
+                               c->stream));
+  }
   HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
   const size_t kper = (size_t)NOBS * (c->jmax + 1) * (2 * c->jmax + 1) * (2 * c->jmax + 1);
   if (kern)  // k_project writes every slot of the dense layout, zeros where |K| > J
@@ -3032,6 +3294,82 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
 extern "C" int pnamp_kernels_batched(pnamp_ctx *c, int npairs, const int32_t *q0, const int32_t *q1, int zval,
                                      int nval, double *zres, double *kern) {
   return kernels_impl(c, npairs, q0, q1, zval, nval, zres, kern, nullptr);
+}
+
+// (j1 j2 j3; m1 m2 m3), arguments doubled, Racah formula
+static double threej_host(int tj1, int tj2, int tj3, int tm1, int tm2, int tm3) {
+  if (tm1 + tm2 + tm3 != 0 || std::abs(tm1) > tj1 || std::abs(tm2) > tj2 || std::abs(tm3) > tj3) return 0.0;
+  if (((tj1 + tm1) & 1) || ((tj2 + tm2) & 1) || ((tj3 + tm3) & 1)) return 0.0;
+  if (tj3 > tj1 + tj2 || tj3 < std::abs(tj1 - tj2) || ((tj1 + tj2 + tj3) & 1)) return 0.0;
+  auto f = [](int k) { return std::tgamma(k + 1.0); };
+  const int a = (tj1 + tj2 - tj3) / 2, b = (tj1 - tj2 + tj3) / 2, cc = (-tj1 + tj2 + tj3) / 2;
+  const double pre = std::sqrt(f(a) * f(b) * f(cc) / f((tj1 + tj2 + tj3) / 2 + 1) * f((tj1 + tm1) / 2) *
+                               f((tj1 - tm1) / 2) * f((tj2 + tm2) / 2) * f((tj2 - tm2) / 2) * f((tj3 + tm3) / 2) *
+                               f((tj3 - tm3) / 2));
+  const int t1 = (tj3 - tj2 + tm1) / 2, t2 = (tj3 - tj1 - tm2) / 2;
+  const int kmin = std::max({0, -t1, -t2}), kmax = std::min({a, (tj1 - tm1) / 2, (tj2 + tm2) / 2});
+  double sum = 0.0;
+  for (int k = kmin; k <= kmax; ++k) {
+    const double den = f(k) * f(a - k) * f((tj1 - tm1) / 2 - k) * f((tj2 + tm2) / 2 - k) * f(t1 + k) * f(t2 + k);
+    sum += (k & 1) ? -1.0 / den : 1.0 / den;
+  }
+  return (((tj1 - tj2 - tm3) / 2) & 1 ? -1.0 : 1.0) * pre * sum;
+}
+
+extern "C" int pnamp_set_q2(pnamp_ctx *c, const double *q2) {
+  if (!c || !q2) return PNAMP_E_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int n = c->n, h = c->h, jm = c->jmax;
+  // orbits of one species half: contiguous runs m = j .. -j
+  std::vector<int> first;
+  for (int i = 0; i < h; i += c->twoj_h[i] + 1) first.push_back(i);
+  const int norb = (int)first.size();
+  if (2 * norb * norb > NOB2_MAX) return fail(c, PNAMP_E_SIZE, "more than 8 orbits per species");
+  std::vector<int4> op;
+  std::vector<double> osc;
+  const double w0 = (c->nalp * c->ngam == 1) ? 0.5 : 1.0 / (8.0 * M_PI * M_PI);
+  for (int sp = 0; sp < 2; ++sp)
+    for (int a = 0; a < norb; ++a)
+      for (int b = 0; b < norb; ++b) {
+        const int fa = first[a], fb = first[b];
+        const bool same = c->twoj_h[fa] == c->twoj_h[fb] && c->l_h[fa] == c->l_h[fb];
+        op.push_back(make_int4(sp, sp * h + fa, sp * h + fb, same ? c->twoj_h[fa] + 1 : 0));
+        osc.push_back(w0 / std::sqrt(c->twoj_h[fa] + 1.0));
+      }
+  // T(J1 - J2, J2, K1, mu) = (2 J1 + 1) (-1)^(J1 - K1) (J1 2 J2; -K1 mu K1 - mu)
+  const int K1d = 2 * jm + 5;
+  std::vector<double> t3((size_t)5 * (jm + 1) * K1d * 5, 0.0);
+  for (int dJ = -2; dJ <= 2; ++dJ)
+    for (int J2 = 0; J2 <= jm; ++J2) {
+      const int J1 = J2 + dJ;
+      if (J1 < 0) continue;
+      for (int K1 = -J1; K1 <= J1; ++K1)
+        for (int mu = -2; mu <= 2; ++mu)
+          t3[(((size_t)(dJ + 2) * (jm + 1) + J2) * K1d + (K1 + jm + 2)) * 5 + (mu + 2)] =
+              (2.0 * J1 + 1.0) * (((J1 - K1) & 1) ? -1.0 : 1.0) *
+              threej_host(2 * J1, 4, 2 * J2, -2 * K1, 2 * mu, 2 * (K1 - mu));
+    }
+  int r = 0;
+  if ((r = dupload(c, &c->d_q2, q2, (size_t)5 * n * n)) || (r = dupload(c, &c->d_orbp, op.data(), op.size())) ||
+      (r = dupload(c, &c->d_orbscale, osc.data(), osc.size())) || (r = dupload(c, &c->d_t3j, t3.data(), t3.size())))
+    return r;
+  c->norb = norb;
+  c->nob2 = (int)op.size();
+  c->e2_cap = 0;
+  return PNAMP_OK;
+}
+
+extern "C" int pnamp_norb(pnamp_ctx *c) {
+  if (!c) return PNAMP_E_ARG;
+  int norb = 0;
+  for (int i = 0; i < c->h; i += c->twoj_h[i] + 1) ++norb;
+  return norb;
+}
+
+extern "C" int pnamp_kernels_e2(pnamp_ctx *c, int npairs, const int32_t *q0, const int32_t *q1, int zval, int nval,
+                                double *zres, double *kern, double *ze2, double *qred, double *rho1b) {
+  const E2Out e2{ze2, qred, rho1b};
+  return kernels_impl(c, npairs, q0, q1, zval, nval, zres, kern, nullptr, &e2);
 }
 
 extern "C" int pnamp_mesh_samples(pnamp_ctx *c, int npairs, const int32_t *q0, const int32_t *q1, int zval, int nval,
@@ -3107,7 +3445,7 @@ extern "C" int pnamp_kernels_batched_noalloc(pnamp_ctx *c, int npairs, const int
     // k_point writes every ZRES and status entry of the chunk and k_project
     // every kernel and pair-status entry: nothing left by an earlier call is read
     const PointParams pp = point_params(c, zval, nval, sq0, sq1, szr, sst);
-    hipLaunchKernelGGL(k_point, dim3(c->NE, np), dim3(NT), 0, c->stream, pp);
+    hipLaunchKernelGGL(k_point<false>, dim3(c->NE, np), dim3(NT), 0, c->stream, pp);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, launch_project(c, np, proj_params(c, szr, sst, sk, spst)));
     if (kern)
@@ -3300,6 +3638,9 @@ extern "C" void pnamp_destroy(pnamp_ctx *c) {
     for (void *b : mb)
       if (b) (void)hipFree(b);
   }
+  void *e2b[] = {c->d_q2, c->d_t3j, c->d_orbscale, c->d_orbp, c->d_ze2, c->d_zrho, c->d_iproj, c->d_qred, c->d_rho1b};
+  for (void *b : e2b)
+    if (b) (void)hipFree(b);
   if (c->d_a01) (void)hipFree(c->d_a01);
   if (c->d_gpn) (void)hipFree(c->d_gpn);
   for (auto e : c->ev)

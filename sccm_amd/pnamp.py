@@ -72,6 +72,12 @@ def load_library(path: str = LIB_PATH):
     L.pnamp_kernels_batched.restype = C.c_int
     L.pnamp_kernels_batched_noalloc.argtypes = [vp, C.c_int, _i32p, _i32p, C.c_int, C.c_int, _f64p, _f64p]
     L.pnamp_kernels_batched_noalloc.restype = C.c_int
+    L.pnamp_set_q2.argtypes = [vp, _f64p]
+    L.pnamp_set_q2.restype = C.c_int
+    L.pnamp_norb.argtypes = [vp]
+    L.pnamp_norb.restype = C.c_int
+    L.pnamp_kernels_e2.argtypes = [vp, C.c_int, _i32p, _i32p, C.c_int, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.pnamp_kernels_e2.restype = C.c_int
     L.pnamp_mesh_samples.argtypes = [vp, C.c_int, _i32p, _i32p, C.c_int, C.c_int, _f64p]
     L.pnamp_mesh_samples.restype = C.c_int
     L.pnamp_project_zres.argtypes = [vp, C.c_int, _f64p, _f64p]
@@ -204,6 +210,31 @@ class PNAMP:
         self._check(fn(self._h, npairs, _p(q0, C.c_int32), _p(q1, C.c_int32), int(zval), int(nval),
                        _p(zres) if zres is not None else None, _p(kern) if kern is not None else None))
         return (zres, kern) if want_zres else kern
+
+    def set_q2(self, q2: np.ndarray):
+        """Q_{2mu} one-body matrices [5, n, n] (mu = -2..2) for the E2 kernels."""
+        a = np.ascontiguousarray(np.transpose(np.asarray(q2, dtype=np.float64), (0, 2, 1)))  # column-major
+        self._check(self._L.pnamp_set_q2(self._h, _p(a)))
+
+    @property
+    def norb(self) -> int:
+        return self._L.pnamp_norb(self._h)
+
+    def kernels_e2(self, pairs, zval: int, nval: int):
+        """(zres, kern, ze2 [P, NE, 2, 5], qred [P, 2, 5, jmax+1, 2jmax+5, 2jmax+1],
+        rho1b [P, 2, norb, norb]) of many pairs (see pnamp_kernels_e2)."""
+        pairs = np.asarray(pairs, dtype=np.int32).reshape(-1, 2)
+        q0 = np.ascontiguousarray(pairs[:, 0])
+        q1 = np.ascontiguousarray(pairs[:, 1])
+        P, jm, nb = len(pairs), self.mesh.jmax, self.norb
+        kern = np.zeros((P,) + self._kshape(), dtype=np.complex128)
+        zres = np.zeros((P, self.n_euler, NOBS), dtype=np.complex128)
+        ze2 = np.zeros((P, self.n_euler, 2, 5), dtype=np.complex128)
+        qred = np.zeros((P, 2, 5, jm + 1, 2 * jm + 5, 2 * jm + 1), dtype=np.complex128)
+        rho1b = np.zeros((P, 2, nb, nb), dtype=np.complex128)
+        self._check(self._L.pnamp_kernels_e2(self._h, P, _p(q0, C.c_int32), _p(q1, C.c_int32), int(zval), int(nval),
+                                             _p(zres), _p(kern), _p(ze2), _p(qred), _p(rho1b)))
+        return zres, kern, ze2, qred, rho1b
 
     def mesh_samples(self, pairs, zval: int, nval: int) -> np.ndarray:
         """Norm and energy of every gauge point: [npairs, NE, nphi, nphi, 2]
