@@ -4,7 +4,8 @@
 ! module arrays that Kernel4Triaxial reads (tnljm, H, AMP, PNP, Input) are
 ! handed over once through pnamp_create, the HFB states through
 ! pnamp_set_states, and each pair through pnamp_kernel_pair (or all pairs at
-! once through pnamp_kernels_batched).
+! once through pnamp_kernels_batched, with the E2 kernels through
+! pnamp_kernels_e2).
 module pnamp_gpu
   use iso_c_binding
   implicit none
@@ -13,6 +14,7 @@ module pnamp_gpu
   public :: pnamp_create, pnamp_set_states, pnamp_kernel_pair, pnamp_kernels_batched
   public :: pnamp_hw_solve, pnamp_last_timing, pnamp_last_error_f, pnamp_destroy
   public :: pnamp_set_form, pnamp_states_separable, pnamp_project_zres
+  public :: pnamp_kernels_batched_noalloc, pnamp_set_q2, pnamp_norb, pnamp_kernels_e2, pnamp_hw_presolve
   public :: PNAMP_NOBS, PNAMP_FORM_PLAIN, PNAMP_FORM_FACTORED, PNAMP_FORM_MIXED_FP32, PNAMP_FORM_MIXED_BF16
 
   integer(c_int), parameter :: PNAMP_NOBS = 6
@@ -102,6 +104,52 @@ module pnamp_gpu
       real(c_double), intent(out) :: en(*), e(*)
       type(c_ptr), value :: f
       integer(c_int), intent(out) :: nkept
+    end function
+
+    integer(c_int) function pnamp_kernels_batched_noalloc(ctx, npairs, q0, q1, zval, nval, zres, kern) &
+        bind(C, name="pnamp_kernels_batched_noalloc")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npairs
+      integer(c_int32_t), intent(in) :: q0(*), q1(*)
+      integer(c_int), value :: zval, nval
+      type(c_ptr), value :: zres, kern
+    end function
+
+    integer(c_int) function pnamp_set_q2(ctx, q2) bind(C, name="pnamp_set_q2")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: q2(*)
+    end function
+
+    integer(c_int) function pnamp_norb(ctx) bind(C, name="pnamp_norb")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function pnamp_kernels_e2(ctx, npairs, q0, q1, zval, nval, zres, kern, ze2, qred, rho1b) &
+        bind(C, name="pnamp_kernels_e2")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npairs
+      integer(c_int32_t), intent(in) :: q0(*), q1(*)
+      integer(c_int), value :: zval, nval
+      type(c_ptr), value :: zres, kern, ze2, qred, rho1b
+    end function
+
+    integer(c_int) function pnamp_hw_presolve(ctx, dim, nmat, hmat, eps, kmax, en, m, e, var, nos_best, e_best) &
+        bind(C, name="pnamp_hw_presolve")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: dim
+      real(c_double), intent(in) :: nmat(*), hmat(*)
+      real(c_double), value :: eps
+      integer(c_int), value :: kmax
+      real(c_double), intent(out) :: en(*)
+      integer(c_int), intent(out) :: m
+      real(c_double), intent(out) :: e(*), var(*)
+      integer(c_int), intent(out) :: nos_best(*)
+      real(c_double), intent(out) :: e_best(*)
     end function
 
     integer(c_int) function pnamp_last_timing(ctx, ms_points, ms_total) bind(C, name="pnamp_last_timing")

@@ -115,12 +115,16 @@ def levels_at_nos(ctx, kern_pairs, pairs, nstates: int, jmax: int, levels: dict,
     return out
 
 
-def write_case(path, basis, inter, mesh, U, V, pairs, coords, zval, nval, ncore=(0, 0)):
-    """Unformatted stream input of fortran/pnamp_driver."""
+def write_case(path, basis, inter, mesh, U, V, pairs, coords, zval, nval, ncore=(0, 0), q2=None,
+               echarge=(1.0, 0.0)):
+    """Unformatted stream input of fortran/pnamp_driver; with ``q2`` (Q_{2mu}
+    matrices [5, n, n]) and the effective charges (e_p, e_n) the host also
+    computes the E2 kernels."""
     n = basis.n
     nst = U.shape[0]
     hdr = np.array([n, inter.nme, mesh.alp.size, mesh.bet.size, mesh.gam.size, mesh.nphi, mesh.jmax,
-                    mesh.fold, mesh.iscale, nst, len(pairs), zval, nval, ncore[0], ncore[1]], dtype=np.int32)
+                    mesh.fold, mesh.iscale, nst, len(pairs), zval, nval, ncore[0], ncore[1],
+                    0 if q2 is None else 1], dtype=np.int32)
     with open(path, "wb") as f:
         f.write(hdr.tobytes())
         f.write(np.array([mesh.cscale, inter.e0], dtype=np.float64).tobytes())
@@ -134,26 +138,61 @@ def write_case(path, basis, inter, mesh, U, V, pairs, coords, zval, nval, ncore=
         f.write(np.ascontiguousarray(np.transpose(U, (0, 2, 1)), dtype=np.float64).tobytes())
         f.write(np.ascontiguousarray(np.transpose(V, (0, 2, 1)), dtype=np.float64).tobytes())
         f.write(np.ascontiguousarray(coords, dtype=np.float64).tobytes())
+        if q2 is not None:
+            f.write(np.asarray(echarge, dtype=np.float64).tobytes())
+            f.write(np.ascontiguousarray(np.transpose(np.asarray(q2, dtype=np.float64), (0, 2, 1))).tobytes())
 
 
 def read_elem(path: str) -> dict:
-    """post_general file -> {J: {(K1, K2): (N, H, nn, zz)}}."""
+    """post_general file (``Codes/PNAMP/PNAMP_IO.f90:39-94``) -> {J: {"coords": (b1, g1, b2, g2),
+    "k": {(K1, K2): (N, H, nn, zz, J2, qpred0, qcpred0, q0p, q0pc)},
+    "dJ1": {(K1, K2): (qpred1, qcpred1)}, "dJ2": {(K1, K2): (qpred2, qcpred2)}}}.
+
+    Follows the record structure line by line: (10i5) J; (8e15.8) the
+    deformations; per K2, K1 in [-k2max, k2max] ((10i5) K1 K2 and five (8e15.8)
+    lines); then per K2 the K1 in [-k2max-1, k2max+1] and [-k2max-2, k2max+2]
+    lines of the dJ = 1, 2 reduced Q2 elements; k2max = J, or |J - 1| for odd J.
+    Raises ValueError on any line that breaks the layout."""
     lines = open(path).read().split("\n")
+    while lines and not lines[-1].strip():
+        lines.pop()
     out, i = {}, 0
 
-    def cpx(s, k):
-        return complex(float(s[30 * k:30 * k + 15]), float(s[30 * k + 15:30 * k + 30]))
+    def nums(s, count):
+        if len(s) != 15 * count:
+            raise ValueError(f"{path}:{i + 1}: expected {count} fields of 15 characters")
+        return [float(s[15 * k:15 * k + 15]) for k in range(count)]
 
-    while i < len(lines) and lines[i].strip():
-        J = int(lines[i])
+    def cpx(s, count):
+        v = nums(s, 2 * count)
+        return [complex(v[2 * k], v[2 * k + 1]) for k in range(count)]
+
+    def ints(s, count):
+        if len(s) != 5 * count:
+            raise ValueError(f"{path}:{i + 1}: expected {count} fields of 5 characters")
+        return [int(s[5 * k:5 * k + 5]) for k in range(count)]
+
+    while i < len(lines):
+        J = ints(lines[i], 1)[0]
+        coords = tuple(nums(lines[i + 1], 4))
         i += 2
         k2max = J if J % 2 == 0 else abs(J - 1)
         blk = {}
-        for _ in range((2 * k2max + 1) ** 2):
-            k1, k2 = (int(x) for x in lines[i].split())
-            nh = lines[i + 1]
-            nz = lines[i + 2]
-            blk[(k1, k2)] = (cpx(nh, 0), cpx(nh, 1), cpx(nz, 0), cpx(nz, 1))
-            i += 6
-        out[J] = blk
+        for k2 in range(-k2max, k2max + 1):
+            for k1 in range(-k2max, k2max + 1):
+                if ints(lines[i], 2) != [k1, k2]:
+                    raise ValueError(f"{path}:{i + 1}: expected K1 K2 = {k1} {k2}")
+                n_h, nn_zz = cpx(lines[i + 1], 2), cpx(lines[i + 2], 2)
+                j2, q0, q0p = cpx(lines[i + 3], 1), cpx(lines[i + 4], 2), cpx(lines[i + 5], 2)
+                blk[(k1, k2)] = (n_h[0], n_h[1], nn_zz[0], nn_zz[1], j2[0], q0[0], q0[1], q0p[0], q0p[1])
+                i += 6
+        d1, d2 = {}, {}
+        for k2 in range(-k2max, k2max + 1):
+            for k1 in range(-k2max - 1, k2max + 2):
+                d1[(k1, k2)] = tuple(cpx(lines[i], 2))
+                i += 1
+            for k1 in range(-k2max - 2, k2max + 3):
+                d2[(k1, k2)] = tuple(cpx(lines[i], 2))
+                i += 1
+        out[J] = {"coords": coords, "k": blk, "dJ1": d1, "dJ2": d2}
     return out
