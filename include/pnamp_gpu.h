@@ -113,13 +113,15 @@ int pnamp_set_states(pnamp_ctx *ctx, int nstates, const double *U, const double 
 /* Selects the evaluation form (default PNAMP_FORM_PLAIN).  No counterpart in
  * the reference, which has one form; callers choosing FACTORED or a MIXED
  * form must set states for which pnamp_states_separable returns 1, and an
- * interaction list in the reference's order (a < b and c < d in every entry)
- * whose proton-neutron entries conserve M; otherwise the kernel calls of
- * those forms return PNAMP_E_ARG (the PLAIN form takes any list). */
+ * interaction list whose entries conserve the charge, have a != b and c != d
+ * (any order: each pair is brought to canonical order with the
+ * antisymmetry sign) and, for proton-neutron entries, conserve M; otherwise
+ * the kernel calls of those forms return PNAMP_E_ARG.  The PLAIN form takes
+ * any list. */
 int pnamp_set_form(pnamp_ctx *ctx, int form);
 
-/* 1 if every state set has no proton-neutron mixing (U, V block-diagonal by
- * species up to the order of quasiparticles), 0 otherwise. */
+/* 1 if no state set mixes protons and neutrons (Z = V U^{-1} has no
+ * proton-neutron block, whatever the quasiparticle basis of U, V), 0 otherwise. */
 int pnamp_states_separable(pnamp_ctx *ctx);
 
 /* Vacuum overlaps <0|Phi_q> of the states set (Det of zbareover). */
@@ -136,6 +138,14 @@ int pnamp_kernel_pair(pnamp_ctx *ctx, int q0, int q1, int zval, int nval, double
  * kern: [npairs][PNAMP_NOBS][jmax+1][2jmax+1][2jmax+1] complex or NULL. */
 int pnamp_kernels_batched(pnamp_ctx *ctx, int npairs, const int32_t *q0, const int32_t *q1,
                           int zval, int nval, double *zres, double *kern);
+
+/* pnamp_kernels_batched with the valence numbers given per pair (zval[i],
+ * nval[i]): one call over the pairs of several GCM instances of one model
+ * space (their states set together by one pnamp_set_states, pairs indexing
+ * into them).  No counterpart in the reference, which runs one nucleus at a
+ * time; it lets an uneven batch of instances share one launch sequence. */
+int pnamp_kernels_batched_zn(pnamp_ctx *ctx, int npairs, const int32_t *q0, const int32_t *q1, const int32_t *zval,
+                             const int32_t *nval, double *zres, double *kern);
 
 /* pnamp_kernels_batched in the PLAIN form without any device allocation.
  * No counterpart in the reference: it exists for callers that cannot

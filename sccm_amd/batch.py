@@ -8,10 +8,11 @@ gauge 9 per species, J <= 8.  Interaction per space with the sd recipe
 (J-coupled TBMEs N(0, 1.5) MeV, single-particle energies U[-5, 0] MeV).
 
 One device context per model space holds that space's basis, interaction
-and mesh; each instance sets its states and runs all its pairs in one
-batched call (chunked, graph-captured launches of the factored form).  The
-plain FP64 path is timed on a seeded sample of pairs of every instance and
-its whole-batch time extrapolated per space (it runs at ~0.6 M mesh-pts/s).
+and mesh and the states of all its instances (one pnamp_set_states); all pairs
+of all its instances run as one batched call with per-pair valence numbers
+(pnamp_kernels_batched_zn): one captured launch sequence per space covers
+every instance.  The plain FP64 path is timed on a seeded sample of pairs of
+every instance and its whole-batch time extrapolated per space.
 """
 from __future__ import annotations
 
@@ -61,51 +62,62 @@ def instance_states(inst: Instance, basis):
 
 
 def run(n: int = 64, seed: int = 4001, nalp: int = 10, nbet: int = 20, nphi: int = 9, jmax: int = 8,
-        form: str = "factored", plain_pairs: int = 2, log=print) -> dict:
+        form: str = "factored", plain_pairs: int = 2, log=print, repeats: int = 2) -> dict:
     from .pnamp import FORM_FACTORED, FORM_MIXED_BF16, FORM_MIXED_FP32, FORM_PLAIN, PNAMP
     fcode = {"factored": FORM_FACTORED, "mixed-fp32": FORM_MIXED_FP32, "mixed-bf16": FORM_MIXED_BF16}[form]
     insts = draw_instances(n, seed)
     mesh = make_mesh(nalp, nbet, nalp, nphi, jmax)
-    ctxs = {}
-    for sp in SPACES:
-        b = make_basis(sp)
-        ctxs[sp] = (b, PNAMP(b, synthetic_interaction(b, 1.5, *SEEDS[sp]), mesh))
     rng = np.random.default_rng(seed + 1)
-    tot_pts, t_chosen = 0, 0.0
-    per_space = {sp: {"instances": 0, "pairs": 0, "mesh_points": 0, "chosen_s": 0.0, "plain_s": 0.0,
-                      "plain_points": 0} for sp in SPACES}
-    worst_rel = 0.0
-    for k, inst in enumerate(insts):
-        b, ctx = ctxs[inst.space]
-        U, V, pairs = instance_states(inst, b)
-        ctx.set_states(U, V)
+    per_space = {}
+    tot_pts, t_chosen, t_plain_est, worst_rel = 0, 0.0, 0.0, 0.0
+    for sp in SPACES:
+        mine = [i for i in insts if i.space == sp]
+        if not mine:
+            continue
+        b = make_basis(sp)
+        ctx = PNAMP(b, synthetic_interaction(b, 1.5, *SEEDS[sp]), mesh)
+        Us, Vs, pairs, zv, nv, sample = [], [], [], [], [], []
+        off = 0
+        for inst in mine:
+            U, V, pr = instance_states(inst, b)
+            Us.append(U)
+            Vs.append(V)
+            first = len(pairs)
+            pairs.extend((p + off, q + off) for p, q in pr)
+            zv.extend([inst.zval] * len(pr))
+            nv.extend([inst.nval] * len(pr))
+            sample.extend(first + int(k) for k in rng.choice(len(pr), size=min(plain_pairs, len(pr)), replace=False))
+            off += inst.nconf
+        pairs = np.array(pairs, dtype=np.int32)
+        zv, nv = np.array(zv, dtype=np.int32), np.array(nv, dtype=np.int32)
+        ctx.set_states(np.concatenate(Us), np.concatenate(Vs))
         ctx.set_form(fcode)
-        t = time.perf_counter()
-        kern = ctx.kernels_batched(pairs, inst.zval, inst.nval)
-        dt = time.perf_counter() - t
+        kern = ctx.kernels_batched_zn(pairs, zv, nv)  # first call captures the launch graph
+        times = []
+        for _ in range(repeats):
+            t = time.perf_counter()
+            kern = ctx.kernels_batched_zn(pairs, zv, nv)
+            times.append(time.perf_counter() - t)
+        dt = min(times)
         pts = len(pairs) * mesh.n_points
-        s = per_space[inst.space]
-        s["instances"] += 1
-        s["pairs"] += len(pairs)
-        s["mesh_points"] += pts
-        s["chosen_s"] += dt
-        tot_pts += pts
-        t_chosen += dt
-        # plain FP64 on a seeded sample of this instance's pairs
-        sel = rng.choice(len(pairs), size=min(plain_pairs, len(pairs)), replace=False)
+        sample = np.array(sample)
         ctx.set_form(FORM_PLAIN)
         t = time.perf_counter()
-        kp = ctx.kernels_batched(pairs[sel], inst.zval, inst.nval)
-        s["plain_s"] += time.perf_counter() - t
-        s["plain_points"] += len(sel) * mesh.n_points
-        rel = float(np.abs(kern[sel] - kp).max() / np.abs(kp).max())
+        kp = ctx.kernels_batched_zn(pairs[sample], zv[sample], nv[sample])
+        tp = time.perf_counter() - t
+        rel = float(np.abs(kern[sample] - kp).max() / np.abs(kp).max())
         worst_rel = max(worst_rel, rel)
-        log(f"batch: {k + 1}/{n} {inst.space} Z={inst.zval} N={inst.nval} {inst.nconf} configs "
-            f"{len(pairs)} pairs {pts / dt / 1e6:.1f} M mesh-pts/s, rel diff vs plain {rel:.1e}")
-    t_plain_est = sum(s["plain_s"] / s["plain_points"] * s["mesh_points"] for s in per_space.values()
-                      if s["plain_points"])
-    for c in ctxs.values():
-        c[1].close()
+        plain_est = tp / (len(sample) * mesh.n_points) * pts
+        per_space[sp] = {"instances": len(mine), "states": off, "pairs": int(len(pairs)), "mesh_points": pts,
+                         "chosen_s": dt, "chosen_mesh_pts_per_s": pts / dt, "plain_sample_pairs": int(len(sample)),
+                         "plain_s_extrapolated": plain_est, "rel_kernel_diff_vs_plain": rel,
+                         "kernel_ms_device": ctx.timing()[0]}
+        tot_pts += pts
+        t_chosen += dt
+        t_plain_est += plain_est
+        log(f"batch: {sp}: {len(mine)} instances, {off} states, {len(pairs)} pairs in one call, "
+            f"{pts / dt / 1e6:.1f} M mesh-pts/s, rel diff vs plain {rel:.1e}")
+        ctx.close()
     return {
         "metric": "mesh-pts/s, uneven GCM batch (sd/pf/f5pg9)", "form": form,
         "instances": n, "mesh_points": tot_pts, "chosen_seconds": t_chosen,
@@ -113,6 +125,7 @@ def run(n: int = 64, seed: int = 4001, nalp: int = 10, nbet: int = 20, nphi: int
         "plain_fp64_mesh_pts_per_s_extrapolated": tot_pts / t_plain_est,
         "speedup_vs_plain": t_plain_est / t_chosen,
         "max_rel_kernel_diff_vs_plain_on_sampled_pairs": worst_rel,
+        "calls": "one set_states and one batched call per model space (per-pair valence numbers)",
         "per_space": per_space,
         "plain_sample": f"{plain_pairs} seeded pairs per instance, whole-batch plain time extrapolated per space",
     }

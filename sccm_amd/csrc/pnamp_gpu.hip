@@ -493,6 +493,7 @@ struct PointParams {
   const int *blk_off, *blk_sz, *blk_base;  // per local m-state
   const int *twom;                  // per local m-state (doubled m)
   const int *q0, *q1;               // [npairs]
+  const int *pz, *pn;               // valence numbers per pair, or NULL (zval, nval for all)
   const VEntry *vlist;
   const double *t1b;
   const double2 *jpack;             // [6][npack]: J_x, J_y, J_z, J_x^2, J_y^2, J_z^2 blocks
@@ -600,7 +601,7 @@ __global__ __launch_bounds__(NT) void k_point(PointParams p) {
     }
     __syncthreads();
     const double2 pf = block_pfaffian<NT>(S, N2, pfmap, redv, redk);
-    const double2 gphase = cexpi(-phip * p.zval - phin * p.nval);
+    const double2 gphase = cexpi(-phip * (p.pz ? p.pz[ip] : p.zval) - phin * (p.pn ? p.pn[ip] : p.nval));
     const double2 znorm = cmul(cscale(vac, pf), gphase);
     __syncthreads();
 
@@ -1487,6 +1488,7 @@ struct CombParams {
   double vac_scale, snpfaf, sgn_h, e0;
   const double *det;
   const int *q0, *q1;
+  const int *pz, *pn;               // valence numbers per pair, or NULL
   // W_pn (rows x = c + a h, columns y = d + b h) in dense blocks of equal
   // Delta m = m_c - m_a = -(m_d - m_b): the interaction conserves M
   const int2 *pn_rows;              // [h^2] by block order: (x, block)
@@ -1613,7 +1615,8 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
     const double phip = (NP == 1) ? 0.0 : (2.0 * M_PI / NP) * fp;
     const double phin = (NP == 1) ? 0.0 : (2.0 * M_PI / NP) * fn;
     const double2 *a = op + fp * FAC_OUT, *b = on + fn * FAC_OUT;
-    const double2 zn = cmul(cscale(vac, cmul(a[0], b[0])), cexpi(-phip * p.zval - phin * p.nval));
+    const double2 zn = cmul(cscale(vac, cmul(a[0], b[0])),
+                            cexpi(-phip * (p.pz ? p.pz[ip] : p.zval) - phin * (p.pn ? p.pn[ip] : p.nval)));
     const double2 ekin = cadd(a[2], b[2]);
     const double2 e2s = cadd(cadd(a[1], b[1]), cadd(a[8], b[8]));
     const double2 E = cadd(cadd(e2s, epn[fp * NP + fn]), cadd(ekin, make_double2(p.e0, 0.0)));
@@ -1667,6 +1670,7 @@ struct FacE2Params {
   double vac_scale, snpfaf, sgn_h;
   const double *det;
   const int *q0, *q1;
+  const int *pz, *pn;                // valence numbers per pair, or NULL
   const double2 *out, *rho;          // k_fac_phi outputs
   const double *q2;                  // [5][n*n] column-major
   const int4 *orbp;                  // [nob2]: species, first m-state of a, of b (global), size
@@ -1688,7 +1692,8 @@ __global__ __launch_bounds__(256) void k_fac_e2(FacE2Params p) {
     const double phip = (np == 1) ? 0.0 : (2.0 * M_PI / np) * fp;
     const double phin = (np == 1) ? 0.0 : (2.0 * M_PI / np) * fn;
     const double2 a = p.out[(base * np + fp) * FAC_OUT], b = p.out[((base + 1) * np + fn) * FAC_OUT];
-    zn[t] = cmul(cscale(vac, cmul(a, b)), cexpi(-phip * p.zval - phin * p.nval));
+    zn[t] = cmul(cscale(vac, cmul(a, b)),
+                 cexpi(-phip * (p.pz ? p.pz[ip] : p.zval) - phin * (p.pn ? p.pn[ip] : p.nval)));
   }
   // one wave per (species, phi): tr_s(Q_mu rho_s(phi)) for the five mu
   for (int u = wave; u < 2 * np; u += blockDim.x / 64) {
@@ -2452,7 +2457,7 @@ struct pnamp_ctx {
   double *d_jtab = nullptr;
   double *d_U = nullptr, *d_V = nullptr, *d_Z = nullptr, *d_det = nullptr;
   int *d_state_status = nullptr;
-  int *d_q0 = nullptr, *d_q1 = nullptr;
+  int *d_q0 = nullptr, *d_q1 = nullptr, *d_pz = nullptr, *d_pn = nullptr;
   double2 *d_zres = nullptr, *d_kern = nullptr;
   int *d_status = nullptr, *d_pair_status = nullptr;
   // factored form
@@ -2486,7 +2491,7 @@ struct pnamp_ctx {
   // captured launch sequence of the factored form, valid for (npairs, zval,
   // nval) and the current buffers; dropped whenever a buffer is reallocated
   hipGraphExec_t fac_graph = nullptr;
-  int g_np = -1, g_z = 0, g_n = 0, g_form = -1;
+  int g_np = -1, g_z = 0, g_n = 0, g_form = -1, g_zn = 0;
   // mixed forms: the dense V-contraction matrices (W_pp, W_nn, Q_pp, Q_nn,
   // W_pn; kp x kp row-major FP64 on the host), their fragment-ordered device
   // copies in the precision of va_mix, and the per-chunk GEMM operands
@@ -2760,10 +2765,29 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
   // pn (a, c proton; b, d neutron) with the (ab) <-> (cd) partner flag in pad
   {
     std::vector<VEntry> ls[3];
+    c->fac_list_ok = true;
     for (int64_t q = 0; q < inter->nme; ++q) {
-      const int a = inter->ka[q] - 1, b = inter->kb[q] - 1, cc = inter->kc[q] - 1, d = inter->kd[q] - 1;
-      const int ta = a >= h, tb = b >= h, tc = cc >= h, td = d >= h;
+      int a = inter->ka[q] - 1, b = inter->kb[q] - 1, cc = inter->kc[q] - 1, d = inter->kd[q] - 1;
+      int ta = a >= h, tb = b >= h, tc = cc >= h, td = d >= h;
       VEntry e{inter->me2b[q], 0u, 0u};
+      if (ta + tb != tc + td) {  // changes the charge: the plain form takes the list as it is
+        c->fac_list_ok = false;
+        c->fac_reason = "factored and mixed forms need list entries that conserve the charge";
+        continue;
+      }
+      // the same matrix element with each pair in canonical order (proton
+      // first, else the smaller index first): <ba|V|cd> = -<ab|V|cd>; the
+      // index sums a + b, c + d (the partner rule) are unchanged
+      if (ta > tb || (ta == tb && a > b)) {
+        std::swap(a, b);
+        std::swap(ta, tb);
+        e.me = -e.me;
+      }
+      if (tc > td || (tc == td && cc > d)) {
+        std::swap(cc, d);
+        std::swap(tc, td);
+        e.me = -e.me;
+      }
       if (!ta && !tb && !tc && !td) {
         e.idx = (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)cc << 16) | ((uint32_t)d << 24);
         ls[0].push_back(e);
@@ -2774,15 +2798,12 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
         e.idx = (uint32_t)a | ((uint32_t)(b - h) << 8) | ((uint32_t)cc << 16) | ((uint32_t)(d - h) << 24);
         e.pad = (a + b == cc + d) ? 0u : 1u;
         ls[2].push_back(e);
-      } else {
-        return bail(PNAMP_E_ARG, "interaction entry violates isospin-projection conservation");
       }
     }
     for (int k = 0; k < 3; ++k) c->nme_s[k] = (int)ls[k].size();
     // pp and nn lists for k_fac_phi: one term per entry, the (kl)(ij) partner
     // of an entry with a + b != c + d as an entry of its own, LDS byte
     // offsets precomputed (rho at ld h + 1, then the A-block at ld h)
-    c->fac_list_ok = true;
     for (int k = 0; k < 2; ++k) {
       std::vector<uint4> lo;
       std::vector<double> lm;
@@ -2795,9 +2816,9 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
       };
       for (const VEntry &e : ls[k]) {
         const int i = e.idx & 255, j = (e.idx >> 8) & 255, kk = (e.idx >> 16) & 255, l = e.idx >> 24;
-        if (!(i < j && kk < l)) {  // the A-block holds A01 and A10 for index pairs in increasing order only
+        if (!(i < j && kk < l)) {  // a = b or c = d: the A-block holds index pairs in increasing order only
           c->fac_list_ok = false;
-          c->fac_reason = "factored and mixed forms need list entries with a < b and c < d";
+          c->fac_reason = "factored and mixed forms need list entries with a != b and c != d";
           break;
         }
         term(i, j, kk, l, e.me);
@@ -2909,6 +2930,48 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
   return PNAMP_OK;
 }
 
+// Cyclic Jacobi for a small real symmetric matrix (host): A = Q diag(w) Q^T,
+// A and Q column-major n x n.
+static void host_jacobi(int n, std::vector<double> A, std::vector<double> &Q, std::vector<double> &w) {
+  Q.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) Q[i + (size_t)i * n] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const double x = A[i + (size_t)j * n] * A[i + (size_t)j * n];
+        tot += x;
+        if (i != j) off += x;
+      }
+    if (off <= 1e-32 * tot || off == 0.0) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int r = p + 1; r < n; ++r) {
+        const double apr = A[p + (size_t)r * n];
+        if (std::fabs(apr) < 1e-300) continue;
+        const double th = (A[r + (size_t)r * n] - A[p + (size_t)p * n]) / (2.0 * apr);
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+        for (int k = 0; k < n; ++k) {  // columns p, r
+          const double x = A[k + (size_t)p * n], y = A[k + (size_t)r * n];
+          A[k + (size_t)p * n] = cs * x - sn * y;
+          A[k + (size_t)r * n] = sn * x + cs * y;
+        }
+        for (int k = 0; k < n; ++k) {  // rows p, r
+          const double x = A[p + (size_t)k * n], y = A[r + (size_t)k * n];
+          A[p + (size_t)k * n] = cs * x - sn * y;
+          A[r + (size_t)k * n] = sn * x + cs * y;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double x = Q[k + (size_t)p * n], y = Q[k + (size_t)r * n];
+          Q[k + (size_t)p * n] = cs * x - sn * y;
+          Q[k + (size_t)r * n] = sn * x + cs * y;
+        }
+      }
+  }
+  w.resize(n);
+  for (int i = 0; i < n; ++i) w[i] = A[i + (size_t)i * n];
+}
+
 extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, const double *V) {
   if (!c || !U || !V || nstates <= 0) return PNAMP_E_ARG;
   const size_t nn = (size_t)c->n * c->n;
@@ -2930,34 +2993,56 @@ extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, cons
       c->nstates = 0;
       return fail(c, st[q], "U of state " + std::to_string(q) + " is singular (zbareover ZGETRF INFO)");
     }
-  // Species separability: every quasiparticle column of U and V has support
-  // in one species only, h columns per species.  Then the species blocks
-  // (columns reordered, protons first) are kept for the factored form.
+  // Species separability, decided on Z = V U^-1 (the state itself, whatever
+  // the quasiparticle basis): no proton-neutron block.  Then each species is
+  // the Bogoliubov block U_s = (1 + Z_s^T Z_s)^(-1/2), V_s = Z_s U_s, which
+  // has V_s U_s^-1 = Z_s and U_s^T U_s + V_s^T V_s = 1; the kernels depend
+  // only on Z_s (transition densities and overlaps are invariant under
+  // U -> U C, V -> V C), so the factored form may use these blocks.
   const int n = c->n, h = c->h;
+  std::vector<double> Zh(nn * nstates);
+  HIPCHK(c, hipMemcpy(Zh.data(), c->d_Z, sizeof(double) * Zh.size(), hipMemcpyDeviceToHost));
   std::vector<double> Uf((size_t)nstates * 2 * h * h), Vf((size_t)nstates * 2 * h * h);
   bool sep = true;
   for (int q = 0; q < nstates && sep; ++q) {
-    const double *Uq = U + (size_t)q * nn, *Vq = V + (size_t)q * nn;
-    double scale = 0.0;
-    for (size_t e = 0; e < nn; ++e) scale = std::max(scale, std::max(std::fabs(Uq[e]), std::fabs(Vq[e])));
-    const double tol = 1e-14 * scale;
-    int cnt[2] = {0, 0};
-    for (int k = 0; k < n && sep; ++k) {
-      bool inp = false, inn = false;
+    const double *Zq = Zh.data() + (size_t)q * nn;
+    double scale = 0.0, off = 0.0;
+    for (int j = 0; j < n; ++j)
       for (int i = 0; i < n; ++i) {
-        const bool nz = std::fabs(Uq[i + (size_t)k * n]) > tol || std::fabs(Vq[i + (size_t)k * n]) > tol;
-        if (nz) (i < h ? inp : inn) = true;
+        const double v = std::fabs(Zq[i + (size_t)j * n]);
+        scale = std::max(scale, v);
+        if ((i < h) != (j < h)) off = std::max(off, v);
       }
-      if (inp && inn) { sep = false; break; }
-      const int spc = inn ? 1 : 0;
-      if (cnt[spc] >= h) { sep = false; break; }
-      const int col = cnt[spc]++;
-      for (int i = 0; i < h; ++i) {
-        Uf[((size_t)q * 2 + spc) * h * h + i + (size_t)col * h] = Uq[(spc * h + i) + (size_t)k * n];
-        Vf[((size_t)q * 2 + spc) * h * h + i + (size_t)col * h] = Vq[(spc * h + i) + (size_t)k * n];
-      }
+    if (off > 1e-11 * std::max(scale, 1.0)) {  // rounding of U^-1 leaves ~1e-14 in a separable state
+      sep = false;
+      break;
     }
-    if (sep && (cnt[0] != h || cnt[1] != h)) sep = false;
+    for (int spc = 0; spc < 2; ++spc) {
+      // M = Z_s^T Z_s (symmetric, non-negative): M = Q diag(w) Q^T by Jacobi,
+      // U_s = Q diag((1 + w)^(-1/2)) Q^T
+      std::vector<double> M((size_t)h * h, 0.0), Q, w;
+      for (int j = 0; j < h; ++j)
+        for (int i = 0; i < h; ++i) {
+          double acc = 0.0;
+          for (int k = 0; k < h; ++k)
+            acc += Zq[(spc * h + k) + (size_t)(spc * h + i) * n] * Zq[(spc * h + k) + (size_t)(spc * h + j) * n];
+          M[i + (size_t)j * h] = acc;
+        }
+      host_jacobi(h, M, Q, w);
+      double *Us = Uf.data() + ((size_t)q * 2 + spc) * h * h, *Vs = Vf.data() + ((size_t)q * 2 + spc) * h * h;
+      for (int j = 0; j < h; ++j)
+        for (int i = 0; i < h; ++i) {
+          double acc = 0.0;
+          for (int k = 0; k < h; ++k) acc += Q[i + (size_t)k * h] * Q[j + (size_t)k * h] / std::sqrt(1.0 + w[k]);
+          Us[i + (size_t)j * h] = acc;
+        }
+      for (int j = 0; j < h; ++j)
+        for (int i = 0; i < h; ++i) {
+          double acc = 0.0;
+          for (int k = 0; k < h; ++k) acc += Zq[(spc * h + i) + (size_t)(spc * h + k) * n] * Us[k + (size_t)j * h];
+          Vs[i + (size_t)j * h] = acc;
+        }
+    }
   }
   c->separable = sep;
   if (sep) {
@@ -3018,7 +3103,8 @@ static int ensure_pairs(pnamp_ctx *c, int np) {
   if (np <= c->pair_cap) return 0;
   int r = 0;
   const size_t ko = (size_t)NOBS * (c->jmax + 1) * (2 * c->jmax + 1) * (2 * c->jmax + 1);
-  if ((r = dalloc(c, &c->d_q0, np)) || (r = dalloc(c, &c->d_q1, np)) ||
+  if ((r = dalloc(c, &c->d_q0, np)) || (r = dalloc(c, &c->d_q1, np)) || (r = dalloc(c, &c->d_pz, np)) ||
+      (r = dalloc(c, &c->d_pn, np)) ||
       (r = dalloc(c, &c->d_zres, (size_t)np * c->NE * NOBS)) || (r = dalloc(c, &c->d_kern, (size_t)np * ko)) ||
       (r = dalloc(c, &c->d_status, (size_t)np * c->NE)) || (r = dalloc(c, &c->d_pair_status, (size_t)np)))
     return r;
@@ -3040,6 +3126,7 @@ static PointParams point_params(const pnamp_ctx *c, int zval, int nval, const in
   pp.q0 = q0; pp.q1 = q1; pp.vlist = c->d_vlist; pp.t1b = c->d_t1b; pp.jpack = c->d_jpack;
   pp.zres = zres; pp.status = status;
   pp.q2 = nullptr; pp.orbp = nullptr; pp.nob2 = 0; pp.ze2 = nullptr; pp.zrho = nullptr;
+  pp.pz = nullptr; pp.pn = nullptr;
   return pp;
 }
 
@@ -3070,8 +3157,11 @@ static int ensure_e2(pnamp_ctx *c, int np) {
 }
 
 static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32_t *q1, int zval, int nval,
-                        double *zres, double *kern, double2 *samples, const E2Out *e2 = nullptr) {
+                        double *zres, double *kern, double2 *samples, const E2Out *e2 = nullptr,
+                        const int32_t *zv = nullptr, const int32_t *nv = nullptr) {
   if (!c || npairs <= 0 || !q0 || !q1) return PNAMP_E_ARG;
+  const bool per_pair = zv != nullptr;
+  if (per_pair && !nv) return PNAMP_E_ARG;
   if (!c->nstates) return fail(c, PNAMP_E_STATE, "no states set");
   if (e2) {
     if (!c->d_q2) return fail(c, PNAMP_E_STATE, "E2 kernels need the Q2 matrices (pnamp_set_q2)");
@@ -3090,7 +3180,15 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipMemcpyAsync(c->d_q0, q0, sizeof(int) * npairs, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_q1, q1, sizeof(int) * npairs, hipMemcpyHostToDevice, c->stream));
+  if (per_pair) {
+    HIPCHK(c, hipMemcpyAsync(c->d_pz, zv, sizeof(int) * npairs, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_pn, nv, sizeof(int) * npairs, hipMemcpyHostToDevice, c->stream));
+  }
   PointParams pp = point_params(c, zval, nval, c->d_q0, c->d_q1, c->d_zres, c->d_status);
+  if (per_pair) {
+    pp.pz = c->d_pz;
+    pp.pn = c->d_pn;
+  }
   if (e2) {
     pp.q2 = c->d_q2; pp.orbp = c->d_orbp; pp.nob2 = c->nob2; pp.ze2 = c->d_ze2; pp.zrho = c->d_zrho;
   }
@@ -3186,7 +3284,8 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
     cp.pn_w = c->d_pn_w; cp.pn_ncol = c->pn_ncol;
     cp.out = c->d_fout; cp.rho = c->d_frho; cp.fstatus = c->d_fstatus; cp.gpn = (const float2 *)c->d_gpn;
     cp.samples = nullptr;
-    if (c->fac_graph && (c->g_np != npairs || c->g_z != zval || c->g_n != nval || c->g_form != c->form))
+    if (c->fac_graph && (c->g_np != npairs || c->g_form != c->form || c->g_zn != (int)per_pair ||
+                         (!per_pair && (c->g_z != zval || c->g_n != nval))))
       drop_graph(c);
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     const bool direct = samples != nullptr || e2 != nullptr;
@@ -3205,6 +3304,8 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
         sp.q0 = c->d_q0 + p0; sp.q1 = c->d_q1 + p0;
         fp.q0 = c->d_q0 + p0; fp.q1 = c->d_q1 + p0;
         cp.q0 = c->d_q0 + p0; cp.q1 = c->d_q1 + p0;
+        cp.pz = per_pair ? c->d_pz + p0 : nullptr;
+        cp.pn = per_pair ? c->d_pn + p0 : nullptr;
         cp.zres = c->d_zres + (size_t)p0 * c->NE * NOBS;
         cp.status = c->d_status + (size_t)p0 * c->NE;
         if (direct) cp.samples = samples + (size_t)p0 * c->NE * c->nphi * c->nphi * 2;
@@ -3220,6 +3321,8 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
         launch_combine_np(c->nphi, mix != MIX_FP64, dim3(c->NE, np), comb_lds, c->stream, cp);
         if (e2) {
           ep.q0 = c->d_q0 + p0; ep.q1 = c->d_q1 + p0;
+          ep.pz = per_pair ? c->d_pz + p0 : nullptr;
+          ep.pn = per_pair ? c->d_pn + p0 : nullptr;
           ep.ze2 = c->d_ze2 + (size_t)p0 * c->NE * 10;
           ep.zrho = c->d_zrho + (size_t)p0 * c->NE * c->nob2;
           hipLaunchKernelGGL(k_fac_e2, dim3(c->NE, np), dim3(256), 0, c->stream, ep);
@@ -3240,6 +3343,7 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
       c->g_z = zval;
       c->g_n = nval;
       c->g_form = c->form;
+      c->g_zn = per_pair ? 1 : 0;
     }
     if (!direct) HIPCHK(c, hipGraphLaunch(c->fac_graph, c->stream));
     HIPCHK(c, hipGetLastError());
@@ -3370,6 +3474,12 @@ extern "C" int pnamp_kernels_e2(pnamp_ctx *c, int npairs, const int32_t *q0, con
                                 double *zres, double *kern, double *ze2, double *qred, double *rho1b) {
   const E2Out e2{ze2, qred, rho1b};
   return kernels_impl(c, npairs, q0, q1, zval, nval, zres, kern, nullptr, &e2);
+}
+
+extern "C" int pnamp_kernels_batched_zn(pnamp_ctx *c, int npairs, const int32_t *q0, const int32_t *q1,
+                                        const int32_t *zval, const int32_t *nval, double *zres, double *kern) {
+  if (!zval || !nval) return PNAMP_E_ARG;
+  return kernels_impl(c, npairs, q0, q1, 0, 0, zres, kern, nullptr, nullptr, zval, nval);
 }
 
 extern "C" int pnamp_mesh_samples(pnamp_ctx *c, int npairs, const int32_t *q0, const int32_t *q1, int zval, int nval,
@@ -3622,7 +3732,7 @@ extern "C" void pnamp_destroy(pnamp_ctx *c) {
   void *bufs[] = {c->d_alp, c->d_walp, c->d_bet, c->d_wbet, c->d_gam, c->d_wgam, c->d_rtab, c->d_ptab,
                   c->d_pk_a, c->d_pk_b, c->d_blk_off, c->d_blk_sz, c->d_blk_base, c->d_twoj, c->d_twom,
                   c->d_vlist, c->d_t1b, c->d_jtab, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status, c->d_q0,
-                  c->d_q1, c->d_zres, c->d_kern, c->d_status, c->d_pair_status,
+                  c->d_q1, c->d_pz, c->d_pn, c->d_zres, c->d_kern, c->d_status, c->d_pair_status,
                   c->d_lofs[0], c->d_lofs[1], c->d_lme[0], c->d_lme[1], c->d_Uf, c->d_Vf,
                   c->d_UfT, c->d_VfT, c->d_mtab, c->d_fout, c->d_frho,
                   c->d_sbuf, c->d_fstatus, c->d_jpack, c->d_pn_rows,

@@ -70,6 +70,8 @@ def load_library(path: str = LIB_PATH):
     L.pnamp_kernel_pair.restype = C.c_int
     L.pnamp_kernels_batched.argtypes = [vp, C.c_int, _i32p, _i32p, C.c_int, C.c_int, _f64p, _f64p]
     L.pnamp_kernels_batched.restype = C.c_int
+    L.pnamp_kernels_batched_zn.argtypes = [vp, C.c_int, _i32p, _i32p, _i32p, _i32p, _f64p, _f64p]
+    L.pnamp_kernels_batched_zn.restype = C.c_int
     L.pnamp_kernels_batched_noalloc.argtypes = [vp, C.c_int, _i32p, _i32p, C.c_int, C.c_int, _f64p, _f64p]
     L.pnamp_kernels_batched_noalloc.restype = C.c_int
     L.pnamp_set_q2.argtypes = [vp, _f64p]
@@ -209,6 +211,23 @@ class PNAMP:
         fn = self._L.pnamp_kernels_batched_noalloc if noalloc else self._L.pnamp_kernels_batched
         self._check(fn(self._h, npairs, _p(q0, C.c_int32), _p(q1, C.c_int32), int(zval), int(nval),
                        _p(zres) if zres is not None else None, _p(kern) if kern is not None else None))
+        return (zres, kern) if want_zres else kern
+
+    def kernels_batched_zn(self, pairs, zval, nval, want_zres: bool = False):
+        """Kernels of many pairs with valence numbers per pair (zval[i], nval[i])."""
+        pairs = np.asarray(pairs, dtype=np.int32).reshape(-1, 2)
+        q0 = np.ascontiguousarray(pairs[:, 0])
+        q1 = np.ascontiguousarray(pairs[:, 1])
+        z = np.ascontiguousarray(zval, dtype=np.int32)
+        nv = np.ascontiguousarray(nval, dtype=np.int32)
+        npairs = len(pairs)
+        if z.shape != (npairs,) or nv.shape != (npairs,):
+            raise ValueError("zval, nval need one entry per pair")
+        kern = np.zeros((npairs,) + self._kshape(), dtype=np.complex128)
+        zres = np.zeros((npairs, self.n_euler, NOBS), dtype=np.complex128) if want_zres else None
+        self._check(self._L.pnamp_kernels_batched_zn(self._h, npairs, _p(q0, C.c_int32), _p(q1, C.c_int32),
+                                                     _p(z, C.c_int32), _p(nv, C.c_int32),
+                                                     _p(zres) if zres is not None else None, _p(kern)))
         return (zres, kern) if want_zres else kern
 
     def set_q2(self, q2: np.ndarray):

@@ -209,29 +209,87 @@ def test_factored_form_rejects_mixed_states():
 
 def _modified_sd_interaction(kind):
     """sd interaction with one list entry changed: "unordered" swaps a and b
-    of a proton-proton entry (a > b), "pn_m" moves d of a proton-neutron
-    entry to a state of another m (the entry no longer conserves M)."""
+    of a proton-proton entry (a > b) and negates its value (the same
+    antisymmetrized element), "neutron_first" writes a proton-neutron entry
+    as (b, a, d, c), "pn_m" moves d of a proton-neutron entry to a state of
+    another m (the entry no longer conserves M)."""
     import dataclasses
     b = make_basis("sd")
     inter = synthetic_interaction(b, 1.5, 1001, 1002)
     ka, kb, kc, kd = (np.array(x, copy=True) for x in (inter.ka, inter.kb, inter.kc, inter.kd))
     h = b.n // 2
+    me = np.array(inter.me, copy=True)
     if kind == "unordered":
         q = int(np.flatnonzero((ka <= h) & (kb <= h) & (kc <= h) & (kd <= h))[0])
         ka[q], kb[q] = kb[q], ka[q]
+        me[q] = -me[q]
+    elif kind == "neutron_first":
+        for q in np.flatnonzero((ka <= h) & (kb > h) & (kc <= h) & (kd > h))[:50]:
+            ka[q], kb[q], kc[q], kd[q] = kb[q], ka[q], kd[q], kc[q]
     else:
         q = int(np.flatnonzero((ka <= h) & (kb > h) & (kc <= h) & (kd > h))[0])
         d = int(kd[q]) - 1
         other = [s for s in range(h, b.n) if b.twom[s] != b.twom[d]]
         kd[q] = other[0] + 1
-    return b, dataclasses.replace(inter, ka=ka, kb=kb, kc=kc, kd=kd)
+    return b, dataclasses.replace(inter, ka=ka, kb=kb, kc=kc, kd=kd, me=me)
 
 
-@pytest.mark.parametrize("kind", ["unordered", "pn_m"])
+@pytest.mark.parametrize("kind", ["unordered", "neutron_first"])
+def test_reordered_list_entries_give_the_same_kernels(kind):
+    """Entries written in another pair order (with the antisymmetry sign) are
+    the same interaction: the plain form takes them as they are, the factored
+    form brings them to canonical order; both equal the canonical list."""
+    from sccm_amd.pnamp import FORM_FACTORED, FORM_PLAIN
+    b, inter = _modified_sd_interaction(kind)
+    ref = synthetic_interaction(b, 1.5, 1001, 1002)
+    mesh = make_mesh(2, 3, 2, 3, 2)
+    U, V = configs.make_states(b, 2, 2, configs.gcm_grid(3))
+    out = {}
+    for name, it in (("mod", inter), ("ref", ref)):
+        g = _gpu(b, it, mesh)
+        g.set_states(U, V)
+        for form in (FORM_PLAIN, FORM_FACTORED):
+            g.set_form(form)
+            out[(name, form)] = g.kernel_pair(0, 1, 2, 2)
+        g.close()
+    r = out[("ref", FORM_PLAIN)]
+    for key, k in out.items():
+        assert _blockwise(k, r) < REL, key
+
+
+def test_separability_does_not_depend_on_the_quasiparticle_basis():
+    """U -> U C, V -> V C with C orthogonal and mixing proton and neutron
+    columns leaves the state (and Z = V U^-1) unchanged: it is found
+    separable, and the factored kernels equal the plain ones and those of the
+    unrotated states."""
+    from sccm_amd.pnamp import FORM_FACTORED, FORM_PLAIN
+    b = make_basis("pf")
+    inter = synthetic_interaction(b, 1.2, 201, 202)
+    mesh = make_mesh(2, 3, 2, 5, 4)
+    U, V = configs.make_states(b, 2, 4, configs.gcm_grid(3))
+    C, _ = np.linalg.qr(np.random.default_rng(9).normal(size=(b.n, b.n)))
+    Ur, Vr = U @ C, V @ C
+    h = b.n // 2  # some quasiparticle columns now have proton and neutron components
+    assert ((np.abs(Ur[0][:h]).max(0) > 1e-3) & (np.abs(Ur[0][h:]).max(0) > 1e-3)).any()
+    g = _gpu(b, inter, mesh)
+    g.set_states(U, V)
+    g.set_form(FORM_FACTORED)
+    k0 = g.kernels_batched([(0, 1), (2, 2)], 2, 4)
+    g.set_states(Ur, Vr)
+    assert g.separable
+    kf = g.kernels_batched([(0, 1), (2, 2)], 2, 4)
+    g.set_form(FORM_PLAIN)
+    kp = g.kernels_batched([(0, 1), (2, 2)], 2, 4)
+    for i in range(2):
+        assert _blockwise(kf[i], kp[i]) < REL and _blockwise(kf[i], k0[i]) < REL
+    g.close()
+
+
+@pytest.mark.parametrize("kind", ["pn_m"])
 def test_factored_form_refuses_lists_it_cannot_split(kind):
-    """The factored forms need list entries with a < b, c < d and proton-
-    neutron entries that conserve M; other lists run in the plain form and
-    are refused by the factored one with an error, never computed wrongly."""
+    """The factored forms need proton-neutron entries that conserve M; other
+    lists run in the plain form and are refused by the factored one with an
+    error, never computed wrongly."""
     from sccm_amd.pnamp import FORM_FACTORED, FORM_PLAIN, PnampError
     b, inter = _modified_sd_interaction(kind)
     g = _gpu(b, inter, make_mesh(2, 3, 2, 3, 2))
