@@ -154,11 +154,15 @@ int pnamp_kernels_batched_zn(pnamp_ctx *ctx, int npairs, const int32_t *q0, cons
  * pnamp_kernels_batched with the context in PNAMP_FORM_PLAIN (the form set
  * by pnamp_set_form is ignored).  Every intermediate (pair lists, ZRES,
  * status flags, projected kernels) lives in one static device array of
- * PNAMP_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; pairs
- * are processed in chunks that fit it.  Returns PNAMP_E_SIZE if a single
- * pair needs more (per pair: (8 + 4 n_euler) + 16 n_euler PNAMP_NOBS +
- * 16 PNAMP_NOBS (jmax+1)(2jmax+1)^2 + 4 bytes, about 0.45 MB for the pf
- * workload's 2000 Euler points and J <= 8).  All calls on one GPU share the
+ * PNAMP_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends: half of
+ * it for the pairs, processed in chunks that fit, half for the mesh-point
+ * work buffer, processed in chunks of (pair, Euler point) units.  Returns
+ * PNAMP_E_SIZE if a single pair needs more than its half (per pair:
+ * (8 + 4 n_euler) + 16 n_euler PNAMP_NOBS + 16 PNAMP_NOBS (jmax+1)(2jmax+1)^2
+ * + 4 bytes, about 0.45 MB for 2000 Euler points and J <= 8) or a single
+ * unit more than the other half (112 n^2 + 16 nphi^2 PNAMP_NOBS + 4 nphi^2
+ * bytes rounded up to 256-byte blocks, 0.18 MB at n = 40, nphi = 9).  All
+ * calls on one GPU share the
  * array: calls that overlap in time on the same GPU (from several threads
  * or contexts) race and must be serialised by the caller. */
 #define PNAMP_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)

@@ -8,15 +8,17 @@
 //                 projection d^J_{K K'}(beta), d^J_{K K'}(pi - beta), per beta.
 //   k_state_prep  per HFB state: U^{-1} (Gauss-Jordan), Z = V U^{-1},
 //                 vacuum overlap det(1 + Z^T Z)^{-1/4} (Cholesky).
-//   k_point       one workgroup per (pair, Euler point); it loops over the
-//                 nphi^2 gauge points of that Euler point and keeps every
-//                 n x n complex intermediate in LDS (five slots of n^2
-//                 complex doubles = 151 KiB at n = 44).  Per gauge point:
-//                 Pfaffian of the 2n x 2n skew matrix (full pivoting, the
-//                 ZPfaffianF algorithm), T = V0^T A V1 + U0^T A* U1, T^{-1}
-//                 (Gauss-Jordan, partial pivoting), rho / kappa10 / kappa01,
-//                 Tr rho per species, E_kin and the two-body energy from the
-//                 m-scheme list.  Writes ZRES(pair, Euler, obs).
+//   k_plain_*     plain form for n <= 40 (plain_gpu.hip): per (pair, Euler point) the
+//                 gauge-independent rotation products, then one workgroup
+//                 per gauge point: Pfaffian of the 2n x 2n skew matrix (full
+//                 pivoting, the ZPfaffianF algorithm), T^{-1} applied by
+//                 Gauss-Jordan, rho / kappa, Tr rho per species, E_kin, <J^2>
+//                 and the two-body energy from the regrouped m-scheme list;
+//                 then the gauge sum into ZRES(pair, Euler, obs).
+//   k_point       plain form for 40 < n <= 44: one workgroup per (pair, Euler
+//                 point) looping over its gauge points, every n x n complex
+//                 intermediate in LDS (five n^2 slots); same steps.
+//   k_fac_*       factored form for states without proton-neutron mixing.
 //   k_project     one workgroup per (pair, observable): J-projection sums
 //                 over the Euler mesh (plain or literal fold).
 //   k_hw_*        Hill-Wheeler: Jacobi eigen-solves of N and N^-1/2 H N^-1/2.
@@ -35,66 +37,24 @@
 #include <vector>
 
 #include "../../include/pnamp_gpu.h"
+#include "plain_gpu.h"
+#include "pnamp_dev.h"
 
 namespace {
 
+using namespace pnamp_dev;
+
 constexpr int NMAX = PNAMP_NMAX;      // largest n (m-states, both species)
 constexpr int HMAX = NMAX / 2;        // largest species half
+constexpr int FAC_HMAX = 22;          // largest species half of the factored and mixed forms
 constexpr int RPACK_MAX = 256;        // packed rotation-block entries per species
-constexpr int NT = 512;               // threads of the mesh-point workgroup
+constexpr int NT = 512;               // threads of the k_point workgroup
 constexpr int NWAVE = NT / 64;
+constexpr int PLAIN_NEW_NMAX = 40;    // n above this takes k_point, at most up to n = NMAX
 constexpr int JMAX_MAX = 12;          // largest projected J supported
 constexpr int NOBS = PNAMP_NOBS;
 
 __constant__ double c_fact[40];
-
-// ------------------------------------------------------------------ complex
-__device__ __forceinline__ double2 cz() { return make_double2(0.0, 0.0); }
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cneg(double2 a) { return make_double2(-a.x, -a.y); }
-__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
-__device__ __forceinline__ double2 cscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ void cfma(double2 &acc, double2 a, double2 b) {
-  acc.x = fma(a.x, b.x, acc.x);
-  acc.x = fma(-a.y, b.y, acc.x);
-  acc.y = fma(a.x, b.y, acc.y);
-  acc.y = fma(a.y, b.x, acc.y);
-}
-__device__ __forceinline__ void cfms(double2 &acc, double2 a, double2 b) {  // acc -= a * b
-  acc.x = fma(-a.x, b.x, acc.x);
-  acc.x = fma(a.y, b.y, acc.x);
-  acc.y = fma(-a.x, b.y, acc.y);
-  acc.y = fma(-a.y, b.x, acc.y);
-}
-__device__ __forceinline__ void rfma(double2 &acc, double a, double2 b) {  // acc += a * b, a real
-  acc.x = fma(a, b.x, acc.x);
-  acc.y = fma(a, b.y, acc.y);
-}
-__device__ __forceinline__ double2 crecip(double2 a) {
-  double d = a.x * a.x + a.y * a.y;
-  return make_double2(a.x / d, -a.y / d);
-}
-__device__ __forceinline__ double2 crecip1(double2 a) {  // one division
-  const double r = 1.0 / (a.x * a.x + a.y * a.y);
-  return make_double2(a.x * r, -a.y * r);
-}
-// Read-only view in the constant address space: wave-uniform reads through it
-// become scalar loads (a generic pointer that stores in the kernel might
-// alias is read with per-lane vector loads)
-template <typename T>
-__device__ __forceinline__ const __attribute__((address_space(4))) T *ro_const(const T *q) {
-  return (const __attribute__((address_space(4))) T *)q;
-}
-__device__ __forceinline__ double cabs1(double2 a) { return fabs(a.x) + fabs(a.y); }
-__device__ __forceinline__ double2 cexpi(double t) {
-  double s, c;
-  sincos(t, &s, &c);
-  return make_double2(c, s);
-}
 
 // d^j_{m1 m2}(beta) = <j m1|exp(-i beta J_y)|j m2>, doubled arguments.
 __device__ double wigner_d_dev(int tj, int tm1, int tm2, double beta) {
@@ -159,12 +119,10 @@ struct PrepParams {
 };
 
 __global__ __launch_bounds__(256) void k_state_prep(PrepParams p) {
-  __shared__ double A[NMAX * NMAX];
-  __shared__ double B[NMAX * NMAX];
-  __shared__ double fcol[NMAX];
+  extern __shared__ double sprep[];  // A, B (n x n each), fcol (n)
   __shared__ int piv;
-  __shared__ double red[4];
   const int q = blockIdx.x, n = p.n, nn = n * n, tid = threadIdx.x;
+  double *const A = sprep, *const B = sprep + nn, *const fcol = sprep + 2 * nn;
   const double *U = p.U + (size_t)q * nn, *V = p.V + (size_t)q * nn;
   double *Z = p.Z + (size_t)q * nn;
   int status = 0;
@@ -248,15 +206,6 @@ __global__ __launch_bounds__(256) void k_state_prep(PrepParams p) {
 }
 
 // ------------------------------------------------------------------ block-level LA
-// Block-wide argmax helper: ties go to the smaller key.
-__device__ __forceinline__ void wave_best(double &v, int &k) {
-  for (int off = 32; off > 0; off >>= 1) {
-    const double ov = __shfl_xor(v, off);
-    const int ok = __shfl_xor(k, off);
-    if (ov > v || (ov == v && ok < k)) { v = ov; k = ok; }
-  }
-}
-
 // Pfaffian of the N2 x N2 skew matrix S (both triangles stored, column-major,
 // leading dimension N2), by ZPfaffianF's algorithm
 // (Codes/PNAMP/PNAMP_Proj.f90:218-350): at step IB the largest |S(i,j)| of the
@@ -2452,7 +2401,15 @@ struct pnamp_ctx {
   double *d_rtab = nullptr, *d_ptab = nullptr;
   int *d_pk_a = nullptr, *d_pk_b = nullptr, *d_blk_off = nullptr, *d_blk_sz = nullptr,
       *d_blk_base = nullptr, *d_twoj = nullptr, *d_twom = nullptr;
-  VEntry *d_vlist = nullptr;
+  // plain form: the regrouped energy list (LDS offsets, matrix elements) and
+  // the work buffer of k_plain_* (units of the current chunk)
+  uint4 *d_plofs = nullptr;
+  double *d_plme = nullptr;
+  VEntry *d_vlist = nullptr;        // the list as given (k_point, n above PLAIN_NEW_NMAX)
+  double2 *d_jpack = nullptr;       // packed J_x, J_y, J_z and squares (k_point)
+  int pl_n = 0, pl_bytes = 0;
+  void *d_pwork = nullptr;
+  size_t pwork_bytes = 0;
   double *d_t1b = nullptr;
   double *d_jtab = nullptr;
   double *d_U = nullptr, *d_V = nullptr, *d_Z = nullptr, *d_det = nullptr;
@@ -2486,7 +2443,6 @@ struct pnamp_ctx {
   int *d_pn_blk = nullptr, *d_pn_ycol = nullptr;
   double *d_pn_w = nullptr;
   int pn_ncol = 0;
-  double2 *d_jpack = nullptr;
   size_t fac_cap = 0;  // pairs the factored scratch holds
   // captured launch sequence of the factored form, valid for (npairs, zval,
   // nval) and the current buffers; dropped whenever a buffer is reallocated
@@ -2680,6 +2636,12 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
   if (!c->scratch) return bail(PNAMP_E_DEVICE, c->err);
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto &e : c->ev) HIPCHK(c, hipEventCreate(&e));
+  if (plain_supported(n)) {
+    HIPCHK(c, plain_prepare(n, false));
+    HIPCHK(c, plain_prepare(n, true));
+  }
+  HIPCHK(c, hipFuncSetAttribute((const void *)k_state_prep, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(sizeof(double) * (2 * (size_t)n * n + n))));
   double fact[40];
   fact[0] = 1.0;
   for (int i = 1; i < 40; ++i) fact[i] = fact[i - 1] * i;
@@ -2750,17 +2712,54 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
       (r = dupload(c, &c->d_twoj, (const int *)basis->twoj, h)) ||
       (r = dupload(c, &c->d_twom, (const int *)basis->twom, h)))
     return bail(r, c->err);
-  std::vector<VEntry> vl(inter->nme);
-  for (int64_t q = 0; q < inter->nme; ++q) {
-    const int a = inter->ka[q] - 1, b = inter->kb[q] - 1, cc = inter->kc[q] - 1, d = inter->kd[q] - 1;
-    if (a < 0 || b < 0 || cc < 0 || d < 0 || a >= n || b >= n || cc >= n || d >= n)
-      return bail(PNAMP_E_ARG, "interaction index out of range");
-    vl[q].me = inter->me2b[q];
-    vl[q].idx = (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)cc << 16) | ((uint32_t)d << 24);
-    vl[q].pad = 0;
+  // plain-form energy list: each entry (a b | V | c d) with its pairs in
+  // increasing order (<ba|V|cd> = -<ab|V|cd>), as the term
+  //   V [rho(c,a) rho(d,b) - rho(d,a) rho(c,b) + A01(a,b) A10(c,d) / 4]
+  // and, when a + b != c + d, its (cd)(ab) partner; an entry with a = b or
+  // c = d adds exactly zero to the eight-term sum of HFB_Energy_COMPLEX and is
+  // dropped.  Operands by LDS offset (rho at leading dimension n + 1, then
+  // the A-block: A01(a,b) at a + b n, A10(c,d) / 4 at d + c n): bytes when
+  // they fit 16 bits, else elements.
+  {
+    const unsigned ldr = n + 1, abase = n * ldr;
+    c->pl_bytes = (16u * (abase + (unsigned)n * n) <= 65535u) ? 1 : 0;
+    const unsigned sc = c->pl_bytes ? 16u : 1u;
+    auto rb = [&](int row, int col) { return sc * ((unsigned)row + (unsigned)col * ldr); };
+    auto ab = [&](int row, int col) { return sc * (abase + (unsigned)row + (unsigned)col * n); };
+    std::vector<uint4> lo;
+    std::vector<double> lm;
+    auto term = [&](int i, int j, int k, int l, double me) {
+      lo.push_back(make_uint4(rb(k, i) | (rb(l, j) << 16), rb(l, i) | (rb(k, j) << 16), ab(i, j) | (ab(l, k) << 16), 0u));
+      lm.push_back(me);
+    };
+    for (int64_t q = 0; q < inter->nme; ++q) {
+      int a = inter->ka[q] - 1, b = inter->kb[q] - 1, cc = inter->kc[q] - 1, d = inter->kd[q] - 1;
+      if (a < 0 || b < 0 || cc < 0 || d < 0 || a >= n || b >= n || cc >= n || d >= n)
+        return bail(PNAMP_E_ARG, "interaction index out of range");
+      double me = inter->me2b[q];
+      if (a == b || cc == d) continue;
+      if (a > b) { std::swap(a, b); me = -me; }
+      if (cc > d) { std::swap(cc, d); me = -me; }
+      term(a, b, cc, d, me);
+      if (cc + d != a + b) term(cc, d, a, b, me);
+    }
+    while (lo.size() % 256 || lo.empty()) {  // zero terms read rho(0,0) and add nothing
+      lo.push_back(make_uint4(0u, 0u, 0u, 0u));
+      lm.push_back(0.0);
+    }
+    c->pl_n = (int)lo.size();
+    std::vector<VEntry> vl(inter->nme);
+    for (int64_t q = 0; q < inter->nme; ++q) {
+      vl[q].me = inter->me2b[q];
+      vl[q].idx = (uint32_t)(inter->ka[q] - 1) | ((uint32_t)(inter->kb[q] - 1) << 8) |
+                  ((uint32_t)(inter->kc[q] - 1) << 16) | ((uint32_t)(inter->kd[q] - 1) << 24);
+      vl[q].pad = 0;
+    }
+    if ((r = dupload(c, &c->d_vlist, vl.data(), vl.size()))) return bail(r, c->err);
+    if ((r = dupload(c, &c->d_plofs, lo.data(), lo.size())) || (r = dupload(c, &c->d_plme, lm.data(), lm.size())) ||
+        (r = dupload(c, &c->d_t1b, inter->t1b, (size_t)n * n)))
+      return bail(r, c->err);
   }
-  if ((r = dupload(c, &c->d_vlist, vl.data(), vl.size())) || (r = dupload(c, &c->d_t1b, inter->t1b, (size_t)n * n)))
-    return bail(r, c->err);
   // species lists of the factored form: pp and nn with species-local indices,
   // pn (a, c proton; b, d neutron) with the (ab) <-> (cd) partner flag in pad
   {
@@ -2982,7 +2981,7 @@ extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, cons
       (r = dalloc(c, &c->d_state_status, (size_t)nstates)))
     return r;
   PrepParams pp{c->n, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status};
-  hipLaunchKernelGGL(k_state_prep, dim3(nstates), dim3(256), 0, c->stream, pp);
+  hipLaunchKernelGGL(k_state_prep, dim3(nstates), dim3(256), sizeof(double) * (2 * nn + c->n), c->stream, pp);
   HIPCHK(c, hipGetLastError());
   std::vector<int> st(nstates);
   HIPCHK(c, hipMemcpyAsync(st.data(), c->d_state_status, sizeof(int) * nstates, hipMemcpyDeviceToHost, c->stream));
@@ -3130,6 +3129,25 @@ static PointParams point_params(const pnamp_ctx *c, int zval, int nval, const in
   return pp;
 }
 
+// Arguments of the plain-form kernels for the pairs q0/q1 (device), writing
+// zres/status per (pair, Euler point).
+static PlainArgs plain_args(const pnamp_ctx *c, int zval, int nval, const int *q0, const int *q1, double2 *zres,
+                            int *status) {
+  PlainArgs a{};
+  a.n = c->n; a.h = c->h; a.nalp = c->nalp; a.nbet = c->nbet; a.ngam = c->ngam; a.NE = c->NE; a.nphi = c->nphi;
+  a.npack = c->npack; a.zval = zval; a.nval = nval;
+  a.cscale = c->cscale; a.vac_scale = c->vac_scale; a.snpfaf = c->snpfaf; a.e0 = c->e0;
+  a.q0 = q0; a.q1 = q1; a.pz = nullptr; a.pn = nullptr;
+  a.U = c->d_U; a.V = c->d_V; a.Z = c->d_Z; a.det = c->d_det;
+  a.alp = c->d_alp; a.gam = c->d_gam; a.rtab = c->d_rtab; a.pk_a = c->d_pk_a; a.pk_b = c->d_pk_b;
+  a.blk_off = c->d_blk_off; a.blk_sz = c->d_blk_sz; a.blk_base = c->d_blk_base; a.twom = c->d_twom;
+  a.mtab = c->d_mtab; a.t1b = c->d_t1b; a.lofs = c->d_plofs; a.lme = c->d_plme; a.nl = c->pl_n;
+  a.lbytes = c->pl_bytes;
+  a.q2 = nullptr; a.orbp = nullptr; a.nob2 = 0;
+  a.zres = zres; a.status = status; a.ze2 = nullptr; a.zrho = nullptr;
+  return a;
+}
+
 static ProjParams proj_params(const pnamp_ctx *c, const double2 *zres, const int *status, double2 *kern,
                               int *pair_status) {
   return ProjParams{c->nalp, c->nbet, c->ngam, c->NE, c->jmax, c->fold, c->npj, NOBS, c->d_alp, c->d_walp,
@@ -3184,14 +3202,6 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
     HIPCHK(c, hipMemcpyAsync(c->d_pz, zv, sizeof(int) * npairs, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_pn, nv, sizeof(int) * npairs, hipMemcpyHostToDevice, c->stream));
   }
-  PointParams pp = point_params(c, zval, nval, c->d_q0, c->d_q1, c->d_zres, c->d_status);
-  if (per_pair) {
-    pp.pz = c->d_pz;
-    pp.pn = c->d_pn;
-  }
-  if (e2) {
-    pp.q2 = c->d_q2; pp.orbp = c->d_orbp; pp.nob2 = c->nob2; pp.ze2 = c->d_ze2; pp.zrho = c->d_zrho;
-  }
   if (c->form != PNAMP_FORM_PLAIN) {
     if (!c->separable)
       return fail(c, PNAMP_E_ARG, "factored and mixed forms need states without proton-neutron mixing");
@@ -3202,7 +3212,7 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
     const size_t comb_lds = sizeof(double2) * (size_t)c->nphi * hh + sizeof(int) * (size_t)c->pn_ncol;
     if (comb_lds > 150 * 1024) return fail(c, PNAMP_E_SIZE, "nphi h^2 complex exceeds the combine LDS");
     if (hh > 2 * NTC) return fail(c, PNAMP_E_SIZE, "h^2 exceeds the combine rows per workgroup");
-    if (h > HMAX || (h & 1)) return fail(c, PNAMP_E_SIZE, "factored form: species size h must be even and <= 22");
+    if (h > FAC_HMAX || (h & 1)) return fail(c, PNAMP_E_SIZE, "factored form: species size h must be even and <= 22");
     // per pair: rho [NE][2][nphi][h^2], setup blocks [NE][2][5][h^2], per-phi scalars and status;
     // mixed forms add the GEMM operands (4 x 2 NE nphi columns x kp, 4 bytes per element),
     // A01 [NE][2][nphi][h^2] and G_pn [NE][nphi][h^2] (FP32 complex); chunks up to 24 GiB
@@ -3348,11 +3358,43 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
     if (!direct) HIPCHK(c, hipGraphLaunch(c->fac_graph, c->stream));
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  } else {
+  } else if (!plain_supported(c->n)) {
+    // n above PLAIN_NEW_NMAX: one workgroup per (pair, Euler point) over all gauge points
+    PointParams pp = point_params(c, zval, nval, c->d_q0, c->d_q1, c->d_zres, c->d_status);
+    if (per_pair) {
+      pp.pz = c->d_pz;
+      pp.pn = c->d_pn;
+    }
+    if (e2) {
+      pp.q2 = c->d_q2; pp.orbp = c->d_orbp; pp.nob2 = c->nob2; pp.ze2 = c->d_ze2; pp.zrho = c->d_zrho;
+    }
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     if (e2) hipLaunchKernelGGL(k_point<true>, dim3(c->NE, npairs), dim3(NT), 0, c->stream, pp);
     else hipLaunchKernelGGL(k_point<false>, dim3(c->NE, npairs), dim3(NT), 0, c->stream, pp);
     HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  } else {
+    PlainArgs pa = plain_args(c, zval, nval, c->d_q0, c->d_q1, c->d_zres, c->d_status);
+    if (per_pair) {
+      pa.pz = c->d_pz;
+      pa.pn = c->d_pn;
+    }
+    if (e2) {
+      pa.q2 = c->d_q2; pa.orbp = c->d_orbp; pa.nob2 = c->nob2; pa.ze2 = c->d_ze2; pa.zrho = c->d_zrho;
+    }
+    // units (pair, Euler point) in chunks whose work buffer stays under 4 GiB
+    const size_t ub = plain_unit_bytes(pa), units = (size_t)npairs * c->NE;
+    if (units > (size_t)INT32_MAX / 2) return fail(c, PNAMP_E_SIZE, "too many (pair, Euler point) units in one call");
+    const size_t cu = std::max<size_t>(1, std::min(units, ((size_t)4 << 30) / ub));
+    if (c->pwork_bytes < cu * ub) {
+      free_void(c, &c->d_pwork);
+      c->pwork_bytes = 0;
+      HIPCHK(c, hipMalloc(&c->d_pwork, cu * ub));
+      c->pwork_bytes = cu * ub;
+    }
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    for (size_t u0 = 0; u0 < units; u0 += cu)
+      HIPCHK(c, plain_run(pa, (int)u0, (int)std::min(cu, units - u0), c->d_pwork, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   }
   const ProjParams pr = proj_params(c, c->d_zres, c->d_status, c->d_kern, c->d_pair_status);
@@ -3528,15 +3570,16 @@ extern "C" int pnamp_kernels_batched_noalloc(pnamp_ctx *c, int npairs, const int
   for (int i = 0; i < npairs; ++i)
     if (q0[i] < 0 || q1[i] < 0 || q0[i] >= c->nstates || q1[i] >= c->nstates)
       return fail(c, PNAMP_E_ARG, "pair index out of range");
-  // scratch regions per chunk of pairs, each 256-byte aligned: q0, q1, ZRES,
-  // per-Euler status, projected kernels, per-pair status
+  // scratch: the first half for the pair chunk (q0, q1, ZRES, per-Euler
+  // status, projected kernels, per-pair status, each 256-byte aligned), the
+  // second half for the plain kernels' work buffer (units in sub-chunks)
   const size_t kper = (size_t)NOBS * (c->jmax + 1) * (2 * c->jmax + 1) * (2 * c->jmax + 1);
   const size_t per_pair = 2 * sizeof(int) + sizeof(double2) * (size_t)c->NE * NOBS + sizeof(int) * (size_t)c->NE +
                           sizeof(double2) * kper + sizeof(int);
-  const size_t cap = PNAMP_NOALLOC_SCRATCH_BYTES - 6 * 256;
+  const size_t half = (size_t)PNAMP_NOALLOC_SCRATCH_BYTES / 2, cap = half - 6 * 256;
   if (per_pair > cap)
     return fail(c, PNAMP_E_SIZE, "one pair needs " + std::to_string(per_pair) + " bytes of scratch; the static array holds " +
-                                     std::to_string((size_t)PNAMP_NOALLOC_SCRATCH_BYTES));
+                                     std::to_string((size_t)PNAMP_NOALLOC_SCRATCH_BYTES) + " (half for the pairs)");
   const int chunk = (int)std::min<size_t>({cap / per_pair, (size_t)npairs, (size_t)65535});
   HIPCHK(c, hipSetDevice(c->device));
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
@@ -3547,16 +3590,32 @@ extern "C" int pnamp_kernels_batched_noalloc(pnamp_ctx *c, int npairs, const int
   int *sst = (int *)((char *)szr + al(sizeof(double2) * (size_t)chunk * c->NE * NOBS));
   double2 *sk = (double2 *)((char *)sst + al(sizeof(int) * (size_t)chunk * c->NE));
   int *spst = (int *)((char *)sk + al(sizeof(double2) * kper * chunk));
+  void *swork = base + half;
+  const bool newk = plain_supported(c->n);
+  PlainArgs pa = plain_args(c, zval, nval, sq0, sq1, szr, sst);
+  const size_t ub = plain_unit_bytes(pa);
+  if (newk && ub > half)
+    return fail(c, PNAMP_E_SIZE, "one (pair, Euler point) unit needs " + std::to_string(ub) + " bytes of work buffer; " +
+                                     std::to_string(half) + " available");
+  const size_t wu = std::max<size_t>(1, half / ub);  // units per launch
   std::vector<int> pst(chunk);
   for (int p0 = 0; p0 < npairs; p0 += chunk) {
     const int np = std::min(chunk, npairs - p0);
     HIPCHK(c, hipMemcpyAsync(sq0, q0 + p0, sizeof(int) * np, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(sq1, q1 + p0, sizeof(int) * np, hipMemcpyHostToDevice, c->stream));
-    // k_point writes every ZRES and status entry of the chunk and k_project
-    // every kernel and pair-status entry: nothing left by an earlier call is read
-    const PointParams pp = point_params(c, zval, nval, sq0, sq1, szr, sst);
-    hipLaunchKernelGGL(k_point<false>, dim3(c->NE, np), dim3(NT), 0, c->stream, pp);
-    HIPCHK(c, hipGetLastError());
+    // k_plain_sum writes every ZRES and status entry of the chunk and
+    // k_project every kernel and pair-status entry; each k_plain_gauge
+    // workgroup writes its own gauge-point slots before k_plain_sum reads
+    // them: nothing left by an earlier call is read
+    const size_t units = (size_t)np * c->NE;
+    if (newk) {
+      for (size_t u0 = 0; u0 < units; u0 += wu)
+        HIPCHK(c, plain_run(pa, (int)u0, (int)std::min(wu, units - u0), swork, c->stream));
+    } else {
+      const PointParams pp = point_params(c, zval, nval, sq0, sq1, szr, sst);
+      hipLaunchKernelGGL(k_point<false>, dim3(c->NE, np), dim3(NT), 0, c->stream, pp);
+      HIPCHK(c, hipGetLastError());
+    }
     HIPCHK(c, launch_project(c, np, proj_params(c, szr, sst, sk, spst)));
     if (kern)
       HIPCHK(c, hipMemcpyAsync(kern + 2 * kper * p0, sk, sizeof(double2) * kper * np, hipMemcpyDeviceToHost, c->stream));
@@ -3731,11 +3790,11 @@ extern "C" void pnamp_destroy(pnamp_ctx *c) {
   drop_graph(c);
   void *bufs[] = {c->d_alp, c->d_walp, c->d_bet, c->d_wbet, c->d_gam, c->d_wgam, c->d_rtab, c->d_ptab,
                   c->d_pk_a, c->d_pk_b, c->d_blk_off, c->d_blk_sz, c->d_blk_base, c->d_twoj, c->d_twom,
-                  c->d_vlist, c->d_t1b, c->d_jtab, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status, c->d_q0,
+                  c->d_plofs, c->d_plme, c->d_pwork, c->d_vlist, c->d_jpack, c->d_t1b, c->d_jtab, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status, c->d_q0,
                   c->d_q1, c->d_pz, c->d_pn, c->d_zres, c->d_kern, c->d_status, c->d_pair_status,
                   c->d_lofs[0], c->d_lofs[1], c->d_lme[0], c->d_lme[1], c->d_Uf, c->d_Vf,
                   c->d_UfT, c->d_VfT, c->d_mtab, c->d_fout, c->d_frho,
-                  c->d_sbuf, c->d_fstatus, c->d_jpack, c->d_pn_rows,
+                  c->d_sbuf, c->d_fstatus, c->d_pn_rows,
                   c->d_pn_blk, c->d_pn_ycol, c->d_pn_w};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
