@@ -41,6 +41,31 @@ def plain_flops_per_point(n: int, nme: int) -> float:
     return 147.0 * n ** 3 + 190.0 * nme
 
 
+def plain_terms(inter, n: int) -> int:
+    """Terms of the plain kernels' regrouped energy list: entries with a != b
+    and c != d, doubled when a + b != c + d."""
+    import numpy as np
+    idx = np.stack([inter.ka, inter.kb, inter.kc, inter.kd], 1) - 1
+    keep = (idx[:, 0] != idx[:, 1]) & (idx[:, 2] != idx[:, 3])
+    cross = idx[:, 0] + idx[:, 1] != idx[:, 2] + idx[:, 3]
+    return int(keep.sum() + (keep & cross).sum())
+
+
+def plain_own_flops(n: int, nterms: int, nphi: int) -> float:
+    """Real flops of one mesh point along the plain kernels' own algorithm
+    (complex multiply = 6, complex multiply-add = 8): the full-pivot Pfaffian
+    (22 per trailing entry per step, 3 per pivot-scanned entry), Gauss-Jordan
+    column operations on the stacked 3n x n matrix (24 n^3), the three density
+    products (12 n^3), T assembly and phases (42 n^2), the regrouped list
+    (26 per term), <J^2> / E_kin / traces (40 n^2), and the setup's
+    gauge-independent products shared by the nphi^2 gauge points (8 n^3 / nphi^2)."""
+    pf = 0.0
+    for k in range(n - 1):
+        m = 2 * n - 2 * k
+        pf += 22.0 * (m - 2) * (m - 3) / 2 + 3.0 * m * (m - 1) / 2
+    return pf + 36.0 * n ** 3 + 82.0 * n ** 2 + 26.0 * nterms + 8.0 * n ** 3 / (nphi * nphi)
+
+
 def factored_flops(h: int, sum_b2: int, nphi: int, l_species: int, l_cross: int, nnz_pn: int) -> float:
     """Algorithmic real flops of the factored form per (pair, Euler point),
     complex multiply-add = 8 flops.  Per species: R V1, R* U1, R Z1 (block-sparse,
@@ -241,7 +266,7 @@ def main():
             kname += " + k_vgemm"
     else:
         fl = plain_flops_per_point(w.n, w.inter.nme) * pts_step
-        kname, tform = "k_point", "plain_fp64"
+        kname, tform = "k_plain_setup + k_plain_gauge + k_plain_sum", "plain_fp64"
     achieved = fl / (k_avg * 1e-3) / 1e12
     # the reference call sequence's count (147 n^3 + 190 L per mesh point) at
     # the same time: what the step's work costs done the reference's way
@@ -253,7 +278,9 @@ def main():
         FORM_MIXED_FP32: "factored, V contraction as FP32 GEMMs on MFMA, everything else FP64, hipGraph",
         FORM_MIXED_BF16: "factored, V contraction as split-BF16 GEMMs on MFMA (FP32 accumulation), everything "
                          "else FP64, hipGraph",
-        FORM_PLAIN: "plain FP64 drop-in: the reference call sequence per mesh point",
+        FORM_PLAIN: "plain FP64 drop-in, any states: per (pair, Euler point) the gauge-independent rotation "
+                    "products, then per gauge point the full-pivot Pfaffian, T^-1 by Gauss-Jordan, densities, "
+                    "regrouped energy list, <J^2>; FP64 throughout",
     }[form]
     out = {
         "metric": "mesh-pts/s: mixed+batched+surrogate vs plain FP64 GPU at |dE_GCM|<1 keV",
@@ -291,6 +318,11 @@ def main():
             "reference_sequence_equivalent_tflops": plain_equiv,
         },
     }
+    if form == FORM_PLAIN:
+        own = plain_own_flops(w.n, plain_terms(w.inter, w.n), w.mesh.nphi) * pts_step
+        out["roofline"]["own_algorithm_flops_per_launch"] = own
+        out["roofline"]["own_algorithm_tflops"] = own / (k_avg * 1e-3) / 1e12
+        out["roofline"]["own_algorithm_frac"] = own / (k_avg * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
     if form != FORM_PLAIN and not args.no_plain:
         log("bench: plain FP64 reference pass over all pairs")
         ctx.set_form(FORM_PLAIN)

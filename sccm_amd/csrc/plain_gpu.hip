@@ -181,10 +181,10 @@ __device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v,
 
 template <int N>
 __host__ __device__ constexpr size_t gauge_misc_bytes() {
-  // wave keys [2][NW] (8 B), reductions [NW][20] (8 B), m / c / jj tables
-  // [3][N] (8 B), Gauss-Jordan row buffers [2][N] (16 B), broadcast (16 B),
-  // pivot order [N] (4 B)
-  return 16 * PlainCfg<N>::NW + 160 * PlainCfg<N>::NW + 24 * N + 32 * N + 16 + 4 * N;
+  // wave keys [2][NW] (8 B), reductions [NW][20] (8 B), broadcast (16 B),
+  // pivot order [N] (4 B): 880 B at N = 40, so three workgroups of n = 40
+  // (52.7 KB) share a CU's 160 KB
+  return 16 * PlainCfg<N>::NW + 160 * PlainCfg<N>::NW + 16 + 4 * N;
 }
 
 template <int N, bool E2>
@@ -198,10 +198,9 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
   char *const misc = (char *)(AB + nn);
   unsigned long long *const wkey = (unsigned long long *)misc;
   double *const red = (double *)(misc + 16 * NW);
-  double *const tmab = red + 20 * NW;  // m_a, c_a, j_a(j_a+1) for a < n at [0, N), [N, 2N), [2N, 3N)
-  double2 *const rowb = (double2 *)(tmab + 3 * N);
-  double2 *const bcast = rowb + 2 * N;
+  double2 *const bcast = (double2 *)(red + 20 * NW);
   unsigned *const kinv = (unsigned *)(bcast + 1);  // [N]: solution index of each pivot column, times 8 n
+  double2 *const rowb = AB;                        // Gauss-Jordan row buffers [2][N] (the A-block is free then)
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // XCD-aware order: the nphi^2 workgroups of one unit stay on one XCD (and
@@ -219,12 +218,6 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
   const double2 *const H = p.hoist + (size_t)ul * 7 * nn;
   const double *const Z0 = p.Z + (size_t)q0 * nn;
 
-  for (int a = tid; a < n; a += NTH) {
-    const int al = a >= h ? a - h : a;
-    tmab[a] = p.mtab[al];
-    tmab[N + a] = p.mtab[h + al];
-    tmab[2 * N + a] = p.mtab[2 * h + al];
-  }
   for (int a = tid; a < N; a += NTH) kinv[a] = 0u;  // padding columns (c >= n) read column 0
 
   // ---------------------------------------------------------------- Pfaffian
@@ -260,31 +253,37 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
     }
     bool zero = false;
     double2 *const pbuf = lds;  // [2][4][N2]: old rows NR, NC, I1, I2 (R and AB are free here)
-    for (int k = 0; k < n - 1; ++k) {
-      const int NR = 2 * k, NC = NR + 1, m = N2 - NR;
-      const int cnt = m * (m - 1) / 2, cnt2 = (m - 2) * (m - 3) / 2;
-      double2 *const rNR = pbuf + (k & 1) * 4 * N2, *const rNC = rNR + N2, *const rI1 = rNR + 2 * N2,
-                     *const rI2 = rNR + 3 * N2;
-      unsigned long long key = 0ull;
+    // pivot key of an entry: |S|^2 with the low 14 mantissa bits replaced by
+    // its row-major rank (largest magnitude, then the first in row-major order)
+    auto pkey = [&](double2 v, int i, int j) -> unsigned long long {
+      const double mag = v.x * v.x + v.y * v.y;
+      return ((unsigned long long)__double_as_longlong(mag) & ~0x3FFFull) | (unsigned long long)(0x3FFF - (i * N2 + j));
+    };
+    // step 0's key and old rows 0, 1 (later steps get theirs from the previous update sweep)
+    unsigned long long key = 0ull;
+    {
+      const int cnt = n * (N2 - 1);
+      double2 *const rNR = pbuf, *const rNC = pbuf + N2;
 #pragma unroll
       for (int s = 0; s < SL; ++s) {
         if (s * NTH < cnt && s * NTH + tid < cnt) {
           const int i = sij[s] & 255, j = sij[s] >> 8;
           const double2 v = sv[s];
-          const double mag = v.x * v.x + v.y * v.y;
-          // |S|^2 with the low 14 mantissa bits replaced by the row-major rank
-          const unsigned long long kk = ((unsigned long long)__double_as_longlong(mag) & ~0x3FFFull) |
-                                        (unsigned long long)(0x3FFF - (i * N2 + j));
+          const unsigned long long kk = pkey(v, i, j);
           key = kk > key ? kk : key;
-          if (i == NR) rNR[j] = v;
-          if (i == NC) rNC[j] = v;
-          if (i == NR && j == NC) rNC[NR] = cneg(v);
+          if (i == 0) rNR[j] = v;
+          if (i == 1) rNC[j] = v;
+          if (i == 0 && j == 1) rNC[0] = cneg(v);
         }
       }
-      for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = shfl_xor_u64(key, off);
-        key = o > key ? o : key;
-      }
+    }
+    for (int k = 0; k < n - 1; ++k) {
+      const int NR = 2 * k, NC = NR + 1, m = N2 - NR;
+      const int cnt = m * (m - 1) / 2, cnt2 = (m - 2) * (m - 3) / 2;
+      double2 *const rNR = pbuf + (k & 1) * 4 * N2, *const rNC = rNR + N2, *const rI1 = rNR + 2 * N2,
+                     *const rI2 = rNR + 3 * N2;
+      double2 *const nNR = pbuf + ((k + 1) & 1) * 4 * N2, *const nNC = nNR + N2;  // old rows of step k + 1
+      key = wave_umax64(key);
       if (lane == 0) wkey[(k & 1) * NW + wave] = key;
       __syncthreads();
       unsigned long long best = wkey[(k & 1) * NW];
@@ -315,8 +314,11 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
       pf = cscale(phas, cmul(pf, ss));
       const double2 inv = crecip(ss);
       // composed transposition pi = (NR I1)(NC I2) on the trailing indices:
-      // pi(I1) = NR, pi(I2) = NC (NR if I1 = NC); S'(x, y) = S(pi x, pi y)
+      // pi(I1) = NR, pi(I2) = NC (NR if I1 = NC); S'(x, y) = S(pi x, pi y).
+      // The updated entries are step k + 1's trailing matrix: their key and
+      // its old rows NR + 2, NR + 3 are taken in the same sweep.
       const int pI2 = (I1 == NC) ? NR : NC;
+      key = 0ull;
 #pragma unroll
       for (int s = 0; s < SL; ++s) {
         if (s * NTH < cnt2 && s * NTH + tid < cnt2) {
@@ -331,6 +333,11 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
           cfms(t, rI1[px], rI2[py]);
           cfma(base, t, inv);
           sv[s] = base;
+          const unsigned long long kk = pkey(base, x, y);
+          key = kk > key ? kk : key;
+          if (x == NR + 2) nNR[y] = base;
+          if (x == NR + 3) nNC[y] = base;
+          if (x == NR + 2 && y == NR + 3) nNC[NR + 2] = cneg(base);
         }
       }
     }
@@ -379,14 +386,13 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
       for (int cc = 0; cc < HC; ++cc) b[hf * HC + cc] = col[cc];
     }
     __syncthreads();
-    double kv = -1.0;
-    int kc = lane;
-    if (lane < n && !((used >> lane) & 1ull)) {
-      kv = cabs1(b[lane]);
-      if (!(kv >= 0.0)) kv = 0.0;  // a NaN entry still leaves an unused column as the pivot
-    }
-    wave_best(kv, kc);
-    const int cs = __builtin_amdgcn_readfirstlane(kc);
+    // pivot: largest |re| + |im| among unused columns, ties to the smaller
+    // column (one DPP maximum; a NaN entry counts as zero and still leaves an
+    // unused column as the pivot)
+    unsigned pk = 0u;
+    if (lane < n && !((used >> lane) & 1ull)) pk = pivot_key<6>(cabs1(b[lane]), lane);
+    pk = wave_umax(pk);
+    const int cs = 63 - (int)(pk & 63u);
     used |= 1ull << cs;
     const double2 piv = b[cs];
     if (piv.x == 0.0 && piv.y == 0.0 && !info) info = k + 1;
@@ -396,8 +402,9 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
 #pragma unroll
     for (int cc = 0; cc < HC; ++cc)
       if (cc == cl) x = col[cc];
-    const double2 xo = shfl_xor2(x, 32);
-    const double2 xp = cmul(hf == hs ? x : xo, ipv);  // pivot-column entry of this row, scaled
+    double2 xlo, xhi;
+    halves2(x, xlo, xhi);
+    const double2 xp = cmul(hs ? xhi : xlo, ipv);  // pivot-column entry of this row (owner half), scaled
     if (tid == 0) kinv[cs] = (unsigned)(k * n) * 8u;
 #pragma unroll
     for (int cc = 0; cc < HC; ++cc) {
@@ -410,6 +417,7 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
   // solution W(row, k) = col[cc] for the register column cc with pivot order k
 
   // ---------------------------------------------------------------- densities
+  __syncthreads();  // the row buffers in the A-block are read no more
   for (int e = tid; e < nn; e += NTH) AB[e] = cz();
   __syncthreads();
   {
@@ -423,7 +431,6 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
         double2 a1[2] = {cz(), cz()}, a2[2] = {cz(), cz()};
 #pragma unroll
         for (int cc = 0; cc < HC; ++cc) {
-          if (cc % 4 == 0) __builtin_amdgcn_sched_barrier(0);  // bounds the loads in flight
           const double2 w = col[cc];
           const unsigned o = kinv[hf * HC + cc] + (unsigned)j0 * 8u;
           const double2 v01 = *(const double2 *)((const char *)V0 + o);
@@ -434,11 +441,20 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
             rfma(a2[0], u01.x, w);
             rfma(a2[1], u01.y, w);
           }
+          // interleave: per column its loads, then its multiply-adds (the
+          // scheduler would otherwise issue all loads first and spill)
+          __builtin_amdgcn_sched_group_barrier(0x020, WU ? 2 : 1, 0);  // VMEM reads
+          __builtin_amdgcn_sched_group_barrier(0x002, WU ? 8 : 4, 0);  // VALU
         }
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-          a1[t] = cadd(a1[t], shfl_xor2(a1[t], 32));
-          if constexpr (WU) a2[t] = cadd(a2[t], shfl_xor2(a2[t], 32));
+          double2 lo, hi;
+          halves2(a1[t], lo, hi);
+          a1[t] = cadd(lo, hi);
+          if constexpr (WU) {
+            halves2(a2[t], lo, hi);
+            a2[t] = cadd(lo, hi);
+          }
           const int j = j0 + t;
           if (hf == t) {
             if (w1) {
@@ -477,13 +493,14 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
   for (int e = tid; e < nn; e += NTH) {
     const int a = e % n, b = e / n;
     const double2 rab = R[a + b * ld], rba = R[b + a * ld];
-    const double ma = tmab[a], mb = tmab[b], ca = tmab[N + a], cb = tmab[N + b];
+    const int al = a >= h ? a - h : a, bl = b >= h ? b - h : b;
+    const double ma = p.mtab[al], mb = p.mtab[bl], ca = p.mtab[h + al], cb = p.mtab[h + bl];
     rfma(ek, p.t1b[b + (size_t)a * n], rab);
     double2 q = cscale(ma * mb, rba);
     if (ca != 0.0 && cb != 0.0) rfma(q, ca * cb, R[(b - 1) + (a - 1) * ld]);
     cfms(aj, rab, q);
     if (a == b) {
-      rfma(aj, tmab[2 * N + a], rab);
+      rfma(aj, p.mtab[2 * h + al], rab);
       rfma(tz, ma, rab);
       if (a < h) zp = cadd(zp, rab);
       else zn = cadd(zn, rab);
@@ -494,11 +511,11 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
       const double2 l4 = (a < b) ? AB[b + a * n] : cneg(AB[a + b * n]);
       double2 s = cscale(ma * mb, a01(a, b));
       if (ca != 0.0 && b + 1 < n) {
-        const double cb1 = tmab[N + b + 1];
+        const double cb1 = p.mtab[h + (b + 1 >= h ? b + 1 - h : b + 1)];
         if (cb1 != 0.0) rfma(s, 0.5 * ca * cb1, a01(a - 1, b + 1));
       }
       if (cb != 0.0 && a + 1 < n) {
-        const double ca1 = tmab[N + a + 1];
+        const double ca1 = p.mtab[h + (a + 1 >= h ? a + 1 - h : a + 1)];
         if (ca1 != 0.0) rfma(s, 0.5 * ca1 * cb, a01(a + 1, b - 1));
       }
       cfma(aj, l4, s);

@@ -921,23 +921,6 @@ __device__ __forceinline__ void put_b(void *hi, void *lo, int k, size_t col, dou
 }
 
 template <int CTRL, int RM>
-__device__ __forceinline__ unsigned dpp_umax(unsigned v) {
-  return max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, RM, 0xF, false));
-}
-
-// Maximum of v over the wavefront by DPP (quad exchanges, row mirrors, row
-// broadcasts: lane 63 ends with the maximum), returned wave-uniform.
-__device__ __forceinline__ unsigned wave_umax(unsigned v) {
-  v = dpp_umax<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
-  v = dpp_umax<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
-  v = dpp_umax<0x141, 0xF>(v);  // row_half_mirror
-  v = dpp_umax<0x140, 0xF>(v);  // row_mirror
-  v = dpp_umax<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
-  v = dpp_umax<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
-  return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
-}
-
-template <int CTRL, int RM>
 __device__ __forceinline__ double dpp_f64(double v) {  // DPP move of a double (0 where no source)
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, RM, 0xF, false);
@@ -956,16 +939,6 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
-// Pivot key: a non-negative magnitude as FP32 bits with the low KB bits
-// replaced by (2^KB - 1 - idx), so one unsigned maximum over the wavefront
-// picks the largest magnitude (to 2^-(23-KB) relative) and, among equal keys,
-// the smallest index.  A NaN magnitude counts as zero.
-template <int KB>
-__device__ __forceinline__ unsigned pivot_key(double mag, int idx) {
-  const float f = (mag >= 0.0) ? (float)mag : 0.0f;
-  return (__float_as_uint(f) & ~((1u << KB) - 1u)) | ((1u << KB) - 1u - (unsigned)idx);
 }
 
 // Gauss-Jordan on an H-row system held one column per lane of the wavefront:
