@@ -170,8 +170,9 @@ struct PGauge {
   const double *lme;
   const double *q2;
   const int4 *orbp;
-  double2 *gout;  // [unit - u0][nphi^2][nog]
+  double2 *gout;  // [unit - u0][nphi^2][nog]: observables per gauge point, not yet norm-weighted
   int *gst;       // [unit - u0][nphi^2]
+  double2 *gpf;   // [unit - u0][nphi^2]: Pfaffian of the gauge point's skew matrix
 };
 
 __device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m) {
@@ -181,45 +182,35 @@ __device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v,
 
 template <int N>
 __host__ __device__ constexpr size_t gauge_misc_bytes() {
-  // wave keys [2][NW] (8 B), reductions [NW][20] (8 B), broadcast (16 B),
-  // pivot order [N] (4 B): 880 B at N = 40, so three workgroups of n = 40
-  // (52.7 KB) share a CU's 160 KB
-  return 16 * PlainCfg<N>::NW + 160 * PlainCfg<N>::NW + 16 + 4 * N;
+  // wave keys [2][NW] (8 B), reductions [NW][20] (8 B), pivot order [N]
+  // (4 B): 864 B at N = 40, so three workgroups of n = 40 (52.7 KB) share a
+  // CU's 160 KB
+  return 16 * PlainCfg<N>::NW + 160 * PlainCfg<N>::NW + 4 * N;
 }
 
-template <int N, bool E2>
-__global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_eu(3))) void k_plain_gauge(PGauge p) {
+// Pfaffian per gauge point, its own launch: little LDS (the four old rows,
+// 16 n complex) and ~100 VGPRs, so many workgroups per CU hide the serial
+// steps' barriers and reductions.
+template <int N>
+__global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_eu(4))) void k_plain_pf(PGauge p) {
   using C = PlainCfg<N>;
-  constexpr int NTH = C::NTH, NW = C::NW, HC = C::HC, SL = C::SL;
+  constexpr int NTH = C::NTH, NW = C::NW, SL = C::SL;
   extern __shared__ double2 lds[];
-  const int n = p.n, h = p.h, nn = n * n, N2 = 2 * n, ld = n + 1;
-  double2 *const R = lds;              // rho, column-major with leading dimension n + 1
-  double2 *const AB = lds + n * ld;    // A01 (i < j at i + j n), A10 / 4 (i < j at j + i n)
-  char *const misc = (char *)(AB + nn);
-  unsigned long long *const wkey = (unsigned long long *)misc;
-  double *const red = (double *)(misc + 16 * NW);
-  double2 *const bcast = (double2 *)(red + 20 * NW);
-  unsigned *const kinv = (unsigned *)(bcast + 1);  // [N]: solution index of each pivot column, times 8 n
-  double2 *const rowb = AB;                        // Gauss-Jordan row buffers [2][N] (the A-block is free then)
-
+  const int n = p.n, h = p.h, nn = n * n, N2 = 2 * n;
+  unsigned long long *const wkey = (unsigned long long *)(lds + 8 * N2);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // XCD-aware order: the nphi^2 workgroups of one unit stay on one XCD (and
-  // its L2, where the unit's setup matrices are); gridDim.x is a multiple of 8
   const unsigned L = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
   if (L >= (unsigned)p.nwg) return;
   const int ng = p.nphi * p.nphi;
   const int g = (int)(L % (unsigned)ng), ul = (int)(L / (unsigned)ng), u = p.u0 + ul;
   const int ip = u / p.NE;
-  const int q0 = p.q0[ip], q1 = p.q1[ip];
+  const int q0 = p.q0[ip];
   const int lp = g / p.nphi, ln = g - lp * p.nphi;
   const double phip = (p.nphi == 1) ? 0.0 : (2.0 * M_PI / p.nphi) * lp;
   const double phin = (p.nphi == 1) ? 0.0 : (2.0 * M_PI / p.nphi) * ln;
   const double2 ep = cexpi(phip), en = cexpi(phin);
   const double2 *const H = p.hoist + (size_t)ul * 7 * nn;
   const double *const Z0 = p.Z + (size_t)q0 * nn;
-
-  for (int a = tid; a < N; a += NTH) kinv[a] = 0u;  // padding columns (c >= n) read column 0
-
   // ---------------------------------------------------------------- Pfaffian
   // S = cscale [[D M0 D, -1], [1, -Z0]] (Pfaf_Over, strict upper triangle)
   double2 pf = make_double2(1.0, 0.0);
@@ -343,10 +334,42 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
     }
     if (zero) pf = cz();
     else if (tid == 0) pf = cmul(pf, sv[0]);  // entry 0 is S(2n-2, 2n-1)
-    if (tid == 0) bcast[0] = pf;
-    __syncthreads();
-    pf = bcast[0];
+    if (tid == 0) p.gpf[L] = pf;
   }
+
+}
+
+template <int N, bool E2>
+__global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_eu(3))) void k_plain_gauge(PGauge p) {
+  using C = PlainCfg<N>;
+  constexpr int NTH = C::NTH, NW = C::NW, HC = C::HC;
+  extern __shared__ double2 lds[];
+  const int n = p.n, h = p.h, nn = n * n, ld = n + 1;
+  double2 *const R = lds;              // rho, column-major with leading dimension n + 1
+  double2 *const AB = lds + n * ld;    // A01 (i < j at i + j n), A10 / 4 (i < j at j + i n)
+  char *const misc = (char *)(AB + nn);
+  unsigned long long *const wkey = (unsigned long long *)misc;
+  double *const red = (double *)(misc + 16 * NW);
+  unsigned *const kinv = (unsigned *)(red + 20 * NW);  // [N]: solution index of each pivot column, times 8 n
+  double2 *const rowb = AB;                        // Gauss-Jordan row buffers [2][N] (the A-block is free then)
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // XCD-aware order: the nphi^2 workgroups of one unit stay on one XCD (and
+  // its L2, where the unit's setup matrices are); gridDim.x is a multiple of 8
+  const unsigned L = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  if (L >= (unsigned)p.nwg) return;
+  const int ng = p.nphi * p.nphi;
+  const int g = (int)(L % (unsigned)ng), ul = (int)(L / (unsigned)ng), u = p.u0 + ul;
+  const int ip = u / p.NE;
+  const int q0 = p.q0[ip], q1 = p.q1[ip];
+  const int lp = g / p.nphi, ln = g - lp * p.nphi;
+  const double phip = (p.nphi == 1) ? 0.0 : (2.0 * M_PI / p.nphi) * lp;
+  const double phin = (p.nphi == 1) ? 0.0 : (2.0 * M_PI / p.nphi) * ln;
+  const double2 ep = cexpi(phip), en = cexpi(phin);
+  const double2 *const H = p.hoist + (size_t)ul * 7 * nn;
+  const double *const Z0 = p.Z + (size_t)q0 * nn;
+
+  for (int a = tid; a < N; a += NTH) kinv[a] = 0u;  // padding columns (c >= n) read column 0
 
   // ---------------------------------------------------------------- Gauss-Jordan
   // stacked row r: T (r < n), A V1 (n <= r < 2n), A* U1 (2n <= r < 3n); lanes
@@ -545,9 +568,6 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
     for (int k = 0; k < 16; ++k) red[wave * 20 + k] = v[k];
   __syncthreads();
   double2 *const go = p.gout + (size_t)L * p.nog;
-  const double vac = p.snpfaf * (p.det[q0] * p.vac_scale) * (p.det[q1] * p.vac_scale);
-  const double2 gphase = cexpi(-phip * (p.pz ? p.pz[ip] : p.zval) - phin * (p.pn ? p.pn[ip] : p.nval));
-  const double2 znorm = cmul(cscale(vac, pf), gphase);
   if (tid == 0) {
     double t[16];
     for (int k = 0; k < 16; ++k) t[k] = 0.0;
@@ -557,12 +577,12 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
     const double2 TZ = make_double2(t[6], t[7]), TP = make_double2(t[8], t[9]), TM = make_double2(t[10], t[11]);
     const double2 ZP = make_double2(t[12], t[13]), ZN = make_double2(t[14], t[15]);
     const double2 J2 = cadd(A, cadd(cmul(TZ, TZ), cmul(TP, TM)));
-    go[0] = znorm;
-    go[1] = cmul(znorm, ZP);
-    go[2] = cmul(znorm, ZN);
-    go[3] = cmul(znorm, EK);
-    go[4] = cmul(znorm, cadd(cadd(E2v, EK), make_double2(p.e0, 0.0)));
-    go[5] = cmul(znorm, J2);
+    go[0] = make_double2(1.0, 0.0);
+    go[1] = ZP;
+    go[2] = ZN;
+    go[3] = EK;
+    go[4] = cadd(cadd(E2v, EK), make_double2(p.e0, 0.0));
+    go[5] = J2;
     p.gst[L] = info;
   }
   if (E2) {
@@ -595,34 +615,53 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
         re += red[w * 20 + 2 * tid];
         im += red[w * 20 + 2 * tid + 1];
       }
-      go[NOBS + tid] = cmul(znorm, make_double2(re, im));
+      go[NOBS + tid] = make_double2(re, im);
     }
     // orbit-pair traces sum_m rho(k_a(m), k_b(m)) (ZRho1B_Angles, PNAMP_Rho1B.f90:13-33)
     for (int o = tid; o < p.nob2; o += NTH) {
       const int4 op = p.orbp[o];
       double2 t = cz();
       for (int m = 0; m < op.w; ++m) t = cadd(t, R[(op.y + m) + (op.z + m) * ld]);
-      go[NOG_E2 + o] = cmul(znorm, t);
+      go[NOG_E2 + o] = t;
     }
   }
 }
 
 // ------------------------------------------------------------------ gauge sum
 struct PSum {
-  int ng, nog, nob2, u0, nunits;
-  const double2 *gout;
+  int ng, nog, nob2, u0, nunits, NE, nphi, zval, nval;
+  double vac_scale, snpfaf;
+  const int *q0, *q1, *pz, *pn;
+  const double *det;
+  const double2 *gout, *gpf;
   const int *gst;
   double2 *zres, *ze2, *zrho;
   int *status;
 };
 
+// Fomenko sum over the gauge points of one unit: the norm weight
+// znorm = <0|Phi0><0|Phi1> Pf e^{-i (phi_p Z + phi_n N)} times each observable
+// (PNAMP_Kernel3D.f90:262-387), divided by nphi^2.
 __global__ __launch_bounds__(64) void k_plain_sum(PSum p) {
-  const int ul = blockIdx.x, u = p.u0 + ul;
+  __shared__ double2 zn[256];
+  const int ul = blockIdx.x, u = p.u0 + ul, ip = u / p.NE;
   const double inv = 1.0 / (double)p.ng;
+  const double vac = p.snpfaf * (p.det[p.q0[ip]] * p.vac_scale) * (p.det[p.q1[ip]] * p.vac_scale);
+  const double zv = p.pz ? p.pz[ip] : p.zval, nv = p.pn ? p.pn[ip] : p.nval;
+  for (int g = threadIdx.x; g < p.ng; g += 64) {
+    const int lp = g / p.nphi, ln = g - lp * p.nphi;
+    const double phip = (p.nphi == 1) ? 0.0 : (2.0 * M_PI / p.nphi) * lp;
+    const double phin = (p.nphi == 1) ? 0.0 : (2.0 * M_PI / p.nphi) * ln;
+    zn[g] = cmul(cscale(vac, p.gpf[(size_t)ul * p.ng + g]), cexpi(-phip * zv - phin * nv));
+  }
+  __syncthreads();
   const double2 *gi = p.gout + (size_t)ul * p.ng * p.nog;
   for (int o = threadIdx.x; o < p.nog; o += 64) {
     double2 s = cz();
-    for (int g = 0; g < p.ng; ++g) s = cadd(s, gi[(size_t)g * p.nog + o]);
+    for (int g = 0; g < p.ng; ++g) {
+      if (o == 0) s = cadd(s, zn[g]);
+      else cfma(s, zn[g], gi[(size_t)g * p.nog + o]);
+    }
     s = cscale(inv, s);
     if (o < NOBS) p.zres[(size_t)u * NOBS + o] = s;
     else if (o < NOG_E2) p.ze2[(size_t)u * 10 + (o - NOBS)] = s;
@@ -652,6 +691,11 @@ size_t gauge_lds(int n) {
   return sizeof(double2) * ((size_t)n * (n + 1) + (size_t)n * n) + gauge_misc_bytes<N>();
 }
 
+template <int N>
+size_t pf_lds(int n) {
+  return sizeof(double2) * 16 * (size_t)n + 16 * PlainCfg<N>::NW;
+}
+
 template <int N, bool E2>
 hipError_t prepare_nc(int n) {
   return hipFuncSetAttribute((const void *)k_plain_gauge<N, E2>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -660,6 +704,7 @@ hipError_t prepare_nc(int n) {
 
 template <int N>
 hipError_t launch_gauge(bool e2, unsigned grid, hipStream_t s, const PGauge &gp) {
+  hipLaunchKernelGGL((k_plain_pf<N>), dim3(grid), dim3(PlainCfg<N>::NTH), pf_lds<N>(gp.n), s, gp);
   const size_t lds = gauge_lds<N>(gp.n);
   if (e2) hipLaunchKernelGGL((k_plain_gauge<N, true>), dim3(grid), dim3(PlainCfg<N>::NTH), lds, s, gp);
   else hipLaunchKernelGGL((k_plain_gauge<N, false>), dim3(grid), dim3(PlainCfg<N>::NTH), lds, s, gp);
@@ -675,7 +720,8 @@ bool plain_supported(int n) { return n >= 8 && !(n & 1) && class_of(n) != 0; }
 size_t plain_unit_bytes(const PlainArgs &a) {
   const size_t nn = (size_t)a.n * a.n, ng = (size_t)a.nphi * a.nphi;
   const size_t nog = a.q2 ? (size_t)NOG_E2 + a.nob2 : (size_t)NOBS;
-  return align256(sizeof(double2) * 7 * nn) + align256(sizeof(double2) * ng * nog) + align256(sizeof(int) * ng);
+  return align256(sizeof(double2) * 7 * nn) + align256(sizeof(double2) * ng * nog) + align256(sizeof(int) * ng) +
+         align256(sizeof(double2) * ng);
 }
 
 hipError_t plain_prepare(int n, bool e2) {
@@ -702,6 +748,8 @@ hipError_t plain_run(const PlainArgs &a, int u0, int nu, void *work, hipStream_t
   double2 *hoist = (double2 *)w;
   double2 *gout = (double2 *)(w + align256(sizeof(double2) * 7 * nn * nu));
   int *gst = (int *)((char *)gout + align256(sizeof(double2) * ng * nog * nu));
+  double2 *gpf = (double2 *)((char *)gst + align256(sizeof(int) * ng * nu));
+  if (ng > 256) return hipErrorInvalidValue;  // nphi <= 16
 
   PSetup sp;
   sp.n = a.n; sp.h = a.h; sp.nalp = a.nalp; sp.ngam = a.ngam; sp.npack = a.npack; sp.NE = a.NE; sp.u0 = u0;
@@ -720,7 +768,7 @@ hipError_t plain_run(const PlainArgs &a, int u0, int nu, void *work, hipStream_t
   gp.q0 = a.q0; gp.q1 = a.q1; gp.pz = a.pz; gp.pn = a.pn;
   gp.U = a.U; gp.V = a.V; gp.Z = a.Z; gp.det = a.det; gp.hoist = hoist;
   gp.mtab = a.mtab; gp.t1b = a.t1b; gp.lofs = a.lofs; gp.lme = a.lme; gp.q2 = a.q2; gp.orbp = a.orbp;
-  gp.gout = gout; gp.gst = gst;
+  gp.gout = gout; gp.gst = gst; gp.gpf = gpf;
   const unsigned grid = (unsigned)((ng * nu + 7) / 8 * 8);
   switch (N) {
     case 16: err = launch_gauge<16>(e2, grid, s, gp); break;
@@ -730,7 +778,11 @@ hipError_t plain_run(const PlainArgs &a, int u0, int nu, void *work, hipStream_t
   }
   if (err != hipSuccess) return err;
 
-  PSum sm{(int)ng, nog, a.nob2, u0, nu, gout, gst, a.zres, a.ze2, a.zrho, a.status};
+  PSum sm;
+  sm.ng = (int)ng; sm.nog = nog; sm.nob2 = a.nob2; sm.u0 = u0; sm.nunits = nu; sm.NE = a.NE; sm.nphi = a.nphi;
+  sm.zval = a.zval; sm.nval = a.nval; sm.vac_scale = a.vac_scale; sm.snpfaf = a.snpfaf;
+  sm.q0 = a.q0; sm.q1 = a.q1; sm.pz = a.pz; sm.pn = a.pn; sm.det = a.det;
+  sm.gout = gout; sm.gpf = gpf; sm.gst = gst; sm.zres = a.zres; sm.ze2 = a.ze2; sm.zrho = a.zrho; sm.status = a.status;
   hipLaunchKernelGGL(k_plain_sum, dim3(nu), dim3(64), 0, s, sm);
   return hipGetLastError();
 }
