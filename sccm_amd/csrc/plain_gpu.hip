@@ -464,10 +464,14 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
             rfma(a2[0], u01.x, w);
             rfma(a2[1], u01.y, w);
           }
-          // interleave: per column its loads, then its multiply-adds (the
-          // scheduler would otherwise issue all loads first and spill)
-          __builtin_amdgcn_sched_group_barrier(0x020, WU ? 2 : 1, 0);  // VMEM reads
-          __builtin_amdgcn_sched_group_barrier(0x002, WU ? 8 : 4, 0);  // VALU
+          // at most four columns' loads in flight: the accumulators pass
+          // through an empty asm that also orders memory, so the next
+          // columns' loads wait for these multiply-adds (the scheduler would
+          // otherwise issue all loads first and spill the W row)
+          if (cc % 4 == 3) {
+            __asm__ volatile("" : "+v"(a1[0].x), "+v"(a1[0].y), "+v"(a1[1].x), "+v"(a1[1].y) : : "memory");
+            if constexpr (WU) __asm__ volatile("" : "+v"(a2[0].x), "+v"(a2[0].y), "+v"(a2[1].x), "+v"(a2[1].y) : : "memory");
+          }
         }
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
