@@ -41,7 +41,7 @@ extern "C" {
 #endif
 
 #define PNAMP_NOBS 6
-#define PNAMP_NMAX 44
+#define PNAMP_NMAX 64
 
 enum {
   PNAMP_OK = 0,
@@ -237,6 +237,12 @@ int pnamp_hw_presolve(pnamp_ctx *ctx, int dim, const double *N, const double *H,
 /* Device time of the last kernel call, from HIP events on the context
  * stream: the mesh-point kernel and the whole call (ms). */
 int pnamp_last_timing(pnamp_ctx *ctx, float *ms_points, float *ms_total);
+
+/* Launch-graph counters of the factored and mixed forms: graphs captured
+ * and replays launched since pnamp_create.  A call of the same shape and form
+ * replays the graph; pnamp_set_states with no more states than the context
+ * holds keeps it (the state buffers are rewritten in place). */
+int pnamp_graph_info(pnamp_ctx *ctx, int *captures, int *replays);
 
 /* Number of Euler points of the context's mesh. */
 int pnamp_n_euler(pnamp_ctx *ctx);

@@ -27,6 +27,8 @@ SPACES: dict[str, list[tuple[int, int, int]]] = {
     "pf": [(0, 3, 7), (1, 1, 3), (0, 3, 5), (1, 1, 1)],
     # 0f5/2, 1p3/2, 1p1/2, 0g9/2  (Codes/val/pf5g9.val)
     "f5pg9": [(0, 3, 5), (1, 1, 3), (1, 1, 1), (0, 4, 9)],
+    # 0d5/2, 0d3/2, 1s1/2, 0f7/2, 0f5/2, 1p3/2, 1p1/2  (Codes/val/sdpf.val): n = 64
+    "sdpf": [(0, 2, 5), (0, 2, 3), (1, 0, 1), (0, 3, 7), (0, 3, 5), (1, 1, 3), (1, 1, 1)],
     # tiny spaces used by the exact Fock-space checks
     "p": [(0, 1, 3), (0, 1, 1)],
     "p32": [(0, 1, 3)],

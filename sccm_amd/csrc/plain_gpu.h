@@ -39,8 +39,10 @@ struct PlainArgs {
   double2 *ze2, *zrho;             // [units][10], [units][nob2] (E2 only)
 };
 
-// n supported by the plain kernels (even, 8 <= n <= 40: the Pfaffian's
-// row buffers, 16 n complex, share the 2 n^2 + n complex of the densities)
+// n supported by the plain kernels: even, 8 <= n <= 64 (size classes
+// 16, 24, 32, 40, 48, 64; the Pfaffian's 64-bit pivot keys carry the
+// row-major rank of an entry of the 2n x 2n matrix in 14 bits, and the
+// Gauss-Jordan keeps its used columns in a 64-bit mask)
 bool plain_supported(int n);
 // work-buffer bytes per unit
 size_t plain_unit_bytes(const PlainArgs &a);

@@ -67,6 +67,11 @@ struct PlainCfg {
   static constexpr int HC = N / 2;                 // columns per thread (half a row)
   static constexpr int NU = N * (2 * N - 1);       // strict upper entries of the 2N x 2N matrix
   static constexpr int SL = (NU + NTH - 1) / NTH;  // Pfaffian slots per thread
+  // waves per SIMD the register budgets are fitted to: the Pfaffian kernel's
+  // slots grow as N^2 / NTH, the gauge kernel's register columns as N / 2;
+  // the gauge kernel of N = 64 (133 KB of LDS) runs one workgroup per CU
+  static constexpr int PF_WAVES = N <= 40 ? 4 : (N <= 48 ? 3 : 2);
+  static constexpr int GAUGE_WAVES = N <= 48 ? 3 : 2;
 };
 
 // ------------------------------------------------------------------ setup
@@ -192,7 +197,7 @@ __host__ __device__ constexpr size_t gauge_misc_bytes() {
 // 16 n complex) and ~100 VGPRs, so many workgroups per CU hide the serial
 // steps' barriers and reductions.
 template <int N>
-__global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_eu(4))) void k_plain_pf(PGauge p) {
+__global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_eu(PlainCfg<N>::PF_WAVES))) void k_plain_pf(PGauge p) {
   using C = PlainCfg<N>;
   constexpr int NTH = C::NTH, NW = C::NW, SL = C::SL;
   extern __shared__ double2 lds[];
@@ -340,7 +345,7 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
 }
 
 template <int N, bool E2>
-__global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_eu(3))) void k_plain_gauge(PGauge p) {
+__global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_eu(PlainCfg<N>::GAUGE_WAVES))) void k_plain_gauge(PGauge p) {
   using C = PlainCfg<N>;
   constexpr int NTH = C::NTH, NW = C::NW, HC = C::HC;
   extern __shared__ double2 lds[];
@@ -595,17 +600,18 @@ __global__ __launch_bounds__(PlainCfg<N>::NTH) __attribute__((amdgpu_waves_per_e
     double q[20];
 #pragma unroll
     for (int k = 0; k < 20; ++k) q[k] = 0.0;
-    for (int e = tid; e < 2 * h * h; e += NTH) {
-      const int sp = e / (h * h), rr = e - sp * h * h;
-      const int i = sp * h + rr % h, j = sp * h + rr / h;
-      const double2 rji = R[j + i * ld];
 #pragma unroll
-      for (int mu = 0; mu < 5; ++mu) {
-        const double qv = p.q2[(size_t)mu * nn + i + (size_t)j * n];
-        q[sp * 10 + 2 * mu] = fma(qv, rji.x, q[sp * 10 + 2 * mu]);
-        q[sp * 10 + 2 * mu + 1] = fma(qv, rji.y, q[sp * 10 + 2 * mu + 1]);
+    for (int sp = 0; sp < 2; ++sp)
+      for (int rr = tid; rr < h * h; rr += NTH) {
+        const int i = sp * h + rr % h, j = sp * h + rr / h;
+        const double2 rji = R[j + i * ld];
+#pragma unroll
+        for (int mu = 0; mu < 5; ++mu) {
+          const double qv = p.q2[(size_t)mu * nn + i + (size_t)j * n];
+          q[sp * 10 + 2 * mu] = fma(qv, rji.x, q[sp * 10 + 2 * mu]);
+          q[sp * 10 + 2 * mu + 1] = fma(qv, rji.y, q[sp * 10 + 2 * mu + 1]);
+        }
       }
-    }
 #pragma unroll
     for (int k = 0; k < 20; ++k)
       for (int off = 32; off > 0; off >>= 1) q[k] += __shfl_xor(q[k], off);
@@ -679,10 +685,12 @@ __global__ __launch_bounds__(64) void k_plain_sum(PSum p) {
 }
 
 // ------------------------------------------------------------------ host
-// Size classes of the gauge kernel.  Classes above 40 (workgroup LDS above
-// 64 KiB) are not built: n > 40 takes k_point in pnamp_gpu.hip.
-constexpr int NCLASS = 4;
-constexpr int CLASSES[NCLASS] = {16, 24, 32, 40};
+// Size classes of the kernels (n <= N): sd 24, pf 40, f5pg9 44 -> 48,
+// sdpf 64.  The gauge kernel's LDS is 16 (2 n^2 + n) bytes plus ~1 KB:
+// 52.7 KB at n = 40 (three workgroups per CU), 62.6 KB at n = 44 (two),
+// 133 KB at n = 64 (one).
+constexpr int NCLASS = 6;
+constexpr int CLASSES[NCLASS] = {16, 24, 32, 40, 48, 64};
 
 int class_of(int n) {
   for (int c : CLASSES)
@@ -737,6 +745,8 @@ hipError_t plain_prepare(int n, bool e2) {
     case 24: return e2 ? prepare_nc<24, true>(n) : prepare_nc<24, false>(n);
     case 32: return e2 ? prepare_nc<32, true>(n) : prepare_nc<32, false>(n);
     case 40: return e2 ? prepare_nc<40, true>(n) : prepare_nc<40, false>(n);
+    case 48: return e2 ? prepare_nc<48, true>(n) : prepare_nc<48, false>(n);
+    case 64: return e2 ? prepare_nc<64, true>(n) : prepare_nc<64, false>(n);
     default: return hipErrorInvalidValue;
   }
 }
@@ -778,7 +788,9 @@ hipError_t plain_run(const PlainArgs &a, int u0, int nu, void *work, hipStream_t
     case 16: err = launch_gauge<16>(e2, grid, s, gp); break;
     case 24: err = launch_gauge<24>(e2, grid, s, gp); break;
     case 32: err = launch_gauge<32>(e2, grid, s, gp); break;
-    default: err = launch_gauge<40>(e2, grid, s, gp); break;
+    case 40: err = launch_gauge<40>(e2, grid, s, gp); break;
+    case 48: err = launch_gauge<48>(e2, grid, s, gp); break;
+    default: err = launch_gauge<64>(e2, grid, s, gp); break;
   }
   if (err != hipSuccess) return err;
 

@@ -8,16 +8,13 @@
 //                 projection d^J_{K K'}(beta), d^J_{K K'}(pi - beta), per beta.
 //   k_state_prep  per HFB state: U^{-1} (Gauss-Jordan), Z = V U^{-1},
 //                 vacuum overlap det(1 + Z^T Z)^{-1/4} (Cholesky).
-//   k_plain_*     plain form for n <= 40 (plain_gpu.hip): per (pair, Euler point) the
+//   k_plain_*     plain form (plain_gpu.hip): per (pair, Euler point) the
 //                 gauge-independent rotation products, then one workgroup
 //                 per gauge point: Pfaffian of the 2n x 2n skew matrix (full
 //                 pivoting, the ZPfaffianF algorithm), T^{-1} applied by
 //                 Gauss-Jordan, rho / kappa, Tr rho per species, E_kin, <J^2>
 //                 and the two-body energy from the regrouped m-scheme list;
 //                 then the gauge sum into ZRES(pair, Euler, obs).
-//   k_point       plain form for 40 < n <= 44: one workgroup per (pair, Euler
-//                 point) looping over its gauge points, every n x n complex
-//                 intermediate in LDS (five n^2 slots); same steps.
 //   k_fac_*       factored form for states without proton-neutron mixing.
 //   k_project     one workgroup per (pair, observable): J-projection sums
 //                 over the Euler mesh (plain or literal fold).
@@ -48,9 +45,7 @@ constexpr int NMAX = PNAMP_NMAX;      // largest n (m-states, both species)
 constexpr int HMAX = NMAX / 2;        // largest species half
 constexpr int FAC_HMAX = 22;          // largest species half of the factored and mixed forms
 constexpr int RPACK_MAX = 256;        // packed rotation-block entries per species
-constexpr int NT = 512;               // threads of the k_point workgroup
-constexpr int NWAVE = NT / 64;
-constexpr int PLAIN_NEW_NMAX = 40;    // n above this takes k_point, at most up to n = NMAX
+constexpr int NOB2_MAX = 2 * 8 * 8;  // orbit pairs of both species (<= 8 orbits per species)
 constexpr int JMAX_MAX = 12;          // largest projected J supported
 constexpr int NOBS = PNAMP_NOBS;
 
@@ -205,550 +200,13 @@ __global__ __launch_bounds__(256) void k_state_prep(PrepParams p) {
   }
 }
 
-// ------------------------------------------------------------------ block-level LA
-// Pfaffian of the N2 x N2 skew matrix S (both triangles stored, column-major,
-// leading dimension N2), by ZPfaffianF's algorithm
-// (Codes/PNAMP/PNAMP_Proj.f90:218-350): at step IB the largest |S(i,j)| of the
-// remaining upper triangle (first in row-major order on ties) is brought to
-// (NR, NC) by the two row/column exchanges of ZExch, the rank-2 Schur update
-// follows, and a pivot of modulus <= 1e-13 gives Pf = 0.  The exchanges are
-// kept as a logical -> physical index map instead of moving data (two maps of
-// N2 ints in mpbuf, read from one and written to the other each step), so a
-// step costs two barriers.  S is destroyed.  All NTH threads of the block must
-// call it; redv/redk need NTH/64 entries.
-template <int NTH>
-__device__ double2 block_pfaffian(double2 *S, const int N2, int *mpbuf, double *redv, int *redk) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  constexpr int NW = NTH / 64;
-  for (int i = tid; i < N2; i += NTH) mpbuf[i] = i;
-  __syncthreads();
-  double2 pf = make_double2(1.0, 0.0);
-  for (int IB = 0; IB < N2 / 2 - 1; ++IB) {
-    const int NR = 2 * IB, NC = NR + 1;
-    const int *mp = mpbuf + (IB & 1) * N2;
-    int *mpn = mpbuf + ((IB + 1) & 1) * N2;
-    double bv = -1.0;
-    int bk = 0x7fffffff;
-    for (int j = NR + 1 + tid / 64; j < N2; j += NW) {
-      const int pj = mp[j] * N2;
-      for (int i = NR + lane; i < j; i += 64) {
-        const double2 v = S[mp[i] + pj];
-        const double a = v.x * v.x + v.y * v.y;
-        const int key = i * N2 + j;
-        if (a > bv || (a == bv && key < bk)) { bv = a; bk = key; }
-      }
-    }
-    wave_best(bv, bk);
-    if (lane == 0) { redv[wave] = bv; redk[wave] = bk; }
-    __syncthreads();
-    bv = redv[0];
-    bk = redk[0];
-    for (int w = 1; w < NW; ++w)
-      if (redv[w] > bv || (redv[w] == bv && redk[w] < bk)) { bv = redv[w]; bk = redk[w]; }
-    const int I1 = bk / N2, I2 = bk % N2;
-    double phas = 1.0;
-    if (I1 == NR) phas = -phas;
-    if (I2 == NC) phas = -phas;
-    // The two exchanges of ZExch, (NR I1) then (NC I2), applied to the index
-    // map: every thread evaluates them on the four entries involved; the map
-    // of the next step is written to the other buffer.
-    int ix[4] = {NR, I1, NC, I2};
-    int vx[4] = {mp[NR], mp[I1], mp[NC], mp[I2]};
-    for (int sw = 0; sw < 2; ++sw) {
-      const int x = sw ? NC : NR, y = sw ? I2 : I1;
-      if (x == y) continue;
-      int vxx = 0, vyy = 0;
-      for (int k = 0; k < 4; ++k) {
-        if (ix[k] == x) vxx = vx[k];
-        if (ix[k] == y) vyy = vx[k];
-      }
-      for (int k = 0; k < 4; ++k) {
-        if (ix[k] == x) vx[k] = vyy;
-        else if (ix[k] == y) vx[k] = vxx;
-      }
-    }
-    auto newmp = [&](int x) -> int {
-      for (int k = 0; k < 4; ++k)
-        if (ix[k] == x) return vx[k];
-      return mp[x];
-    };
-    const int pR = vx[0], pC = vx[2];
-    const double2 ss = S[pR + pC * N2];
-    if (sqrt(ss.x * ss.x + ss.y * ss.y) <= 1.0e-13) {
-      __syncthreads();
-      return cz();
-    }
-    const double2 inv = crecip(ss);
-    const int r0 = NR + 2;
-    for (int j = r0 + 1 + tid / 64; j < N2; j += NW) {
-      const int pj = newmp(j);
-      const double2 snr_j = S[pR + pj * N2], snc_j = S[pC + pj * N2];
-      for (int i = r0 + lane; i < j; i += 64) {
-        const int pi = newmp(i);
-        double2 t = cmul(S[pC + pi * N2], snr_j);
-        t = csub(t, cmul(S[pR + pi * N2], snc_j));
-        const double2 v = cadd(S[pi + pj * N2], cmul(t, inv));
-        S[pi + pj * N2] = v;
-        S[pj + pi * N2] = cneg(v);
-      }
-    }
-    for (int x = tid; x < N2; x += NTH) mpn[x] = newmp(x);
-    pf = cscale(phas, cmul(pf, ss));
-    __syncthreads();
-  }
-  const int *mpl = mpbuf + ((N2 / 2 - 1) & 1) * N2;
-  const double2 last = S[mpl[N2 - 2] + mpl[N2 - 1] * N2];
-  __syncthreads();
-  return cmul(pf, last);
-}
-
-// Gauss-Jordan with partial pivoting (pivot = max |re|+|im|, first maximum,
-// as ZGETRF) on [A | B] (n x n each): A -> 1, B -> A^{-1} B.  Row exchanges
-// are kept in the map mp (n ints): logical row k of the result is physical
-// row mp[k] of B.  Two barriers per step.  Returns the LAPACK-style INFO of
-// the first exactly-zero pivot (0 if none).  fcol needs n, prow 2n entries.
-template <int NTH>
-__device__ int block_gauss_jordan(double2 *A, double2 *B, const int n, int *mp, double2 *fcol, double2 *prow,
-                                  int *redk) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  int info = 0;
-  for (int i = tid; i < n; i += NTH) mp[i] = i;
-  __syncthreads();
-  for (int k = 0; k < n; ++k) {
-    if (wave == 0) {
-      double bv = -1.0;
-      int bk = 0x7fffffff;
-      for (int i = k + lane; i < n; i += 64) {
-        const double a = cabs1(A[mp[i] + k * n]);
-        if (a > bv || (a == bv && i < bk)) { bv = a; bk = i; }
-      }
-      wave_best(bv, bk);
-      const int rk = mp[bk], rold = mp[k];
-      for (int j = lane; j < 2 * n; j += 64) prow[j] = j < n ? A[rk + j * n] : B[rk + (j - n) * n];
-      for (int r = lane; r < n; r += 64) fcol[r] = A[r + k * n];
-      if (lane == 0) {
-        mp[bk] = rold;
-        mp[k] = rk;
-        redk[0] = rk;
-      }
-    }
-    __syncthreads();
-    const int rk = redk[0];
-    const double2 pv = prow[k];
-    if (pv.x == 0.0 && pv.y == 0.0 && info == 0) info = k + 1;
-    const double2 ipv = crecip(pv);
-    // columns k+1..n-1 of A and all of B; row rk is scaled, the others updated
-    const int ncol = (n - k - 1) + n;
-    for (int c = tid / 64; c < ncol; c += NTH / 64) {
-      const int j = c < n - k - 1 ? k + 1 + c : n + (c - (n - k - 1));
-      double2 *M = j < n ? A : B;
-      const int jj = j < n ? j : j - n;
-      const double2 pr = cmul(prow[j], ipv);
-      for (int r = lane; r < n; r += 64)
-        M[r + jj * n] = (r == rk) ? pr : csub(M[r + jj * n], cmul(fcol[r], pr));
-    }
-    __syncthreads();
-  }
-  return info;
-}
-
-// Pieces of <J^2> for one set of transition densities (ZJ2_0,
-// Codes/PNAMP/PNAMP_ME2B.f90:1-152): for k = x, y, z with the one-body J_k
-// (block-diagonal over orbits, packed like R: J[k][e], J^2 at [3 + k][e]),
-//   A   = sum_k Tr(J_k^2 rho) - Tr(J_k rho J_k rho) + Tr(J_k kappa10 J_k^T kappa01^T)
-//   t_k = Tr(J_k rho)
-// so that <J^2> = A + sum_k t_k^2 for one density set, and for the species
-// factorisation <J^2> = A_p + A_n + sum_k (t_kp + t_kn)^2.  rho, kappa are
-// m x m (ld m) with orbit blocks repeating every h rows; scrA, scrB are
-// m x m scratch.  Returns in out[0..3] = (A, t_x, t_y, t_z) on every thread.
-template <int NTH>
-__device__ __noinline__ void block_j2(const double2 *ro, const double2 *k10, const double2 *k01, const int m, const int h,
-                         const double2 *Jpk, const int npack, const short *m_off, const short *m_sz,
-                         const short *m_base, double2 *scrA, double2 *scrB, double *red, double2 out[4]) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  constexpr int NW = NTH / 64;
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // A (re, im), t_x, t_y, t_z
-  for (int k = 0; k < 3; ++k) {
-    const double2 *Jk = Jpk + (size_t)k * npack, *J2k = Jpk + (size_t)(3 + k) * npack;
-    double2 a = cz(), t2 = cz();
-    // Tr(J^2 rho), Tr(J rho); scrA = J rho; scrB = J kappa10
-    for (int e = tid; e < m * m; e += NTH) {
-      const int i = e % m, c = e / m;
-      const int sp = i / h, il = i - sp * h;
-      const int off = m_off[il], sz = m_sz[il], base = m_base[il];
-      double2 jr = cz(), jk = cz();
-      for (int q = 0; q < sz; ++q) {
-        const int b = sp * h + off + q;
-        const double2 jv = Jk[base + (il - off) + q * sz];
-        cfma(jr, jv, ro[b + c * m]);
-        cfma(jk, jv, k10[b + c * m]);
-        if (c == i) {
-          cfma(a, J2k[base + (il - off) + q * sz], ro[b + i * m]);
-          cfma(t2, jv, ro[b + i * m]);
-        }
-      }
-      scrA[e] = jr;
-      scrB[e] = jk;
-    }
-    __syncthreads();
-    // - Tr(J rho J rho)
-    for (int e = tid; e < m * m; e += NTH) {
-      const int i = e % m, c = e / m;
-      a = csub(a, cmul(scrA[i + c * m], scrA[c + i * m]));
-    }
-    __syncthreads();
-    // + sum (J kappa10)(i,c) (kappa01 J)(i,c): scrA = kappa01 J
-    for (int e = tid; e < m * m; e += NTH) {
-      const int i = e % m, c = e / m;
-      const int sp = c / h, cl = c - sp * h;
-      const int off = m_off[cl], sz = m_sz[cl], base = m_base[cl];
-      double2 kj = cz();
-      for (int q = 0; q < sz; ++q) cfma(kj, k01[i + (sp * h + off + q) * m], Jk[base + q + (cl - off) * sz]);
-      a = cadd(a, cmul(scrB[e], kj));
-    }
-    acc[0] += a.x;
-    acc[1] += a.y;
-    acc[2 + 2 * k] += t2.x;
-    acc[3 + 2 * k] += t2.y;
-    __syncthreads();
-  }
-  for (int v = 0; v < 8; ++v)
-    for (int off = 32; off > 0; off >>= 1) acc[v] += __shfl_xor(acc[v], off);
-  if (lane == 0)
-    for (int v = 0; v < 8; ++v) red[wave * 8 + v] = acc[v];
-  __syncthreads();
-  double tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int w = 0; w < NW; ++w)
-    for (int v = 0; v < 8; ++v) tot[v] += red[w * 8 + v];
-  __syncthreads();
-  for (int v = 0; v < 4; ++v) out[v] = make_double2(tot[2 * v], tot[2 * v + 1]);
-}
-
-// ------------------------------------------------------------------ mesh points
+// One m-scheme list entry on the host: V, the four 0-based indices packed
+// a | b << 8 | c << 16 | d << 24, and a flag word.
 struct VEntry {
   double me;
-  uint32_t idx;  // a | b << 8 | c << 16 | d << 24, 0-based
+  uint32_t idx;
   uint32_t pad;
 };
-
-struct PointParams {
-  int n, nalp, nbet, ngam, nphi, zval, nval, npack, NE;
-  int nme;
-  double cscale, vac_scale, snpfaf, e0;
-  const double *alp, *bet, *gam;
-  const double *U, *V, *Z, *det;
-  const double *rtab;               // [nbet][npack]
-  const int *pk_a, *pk_b;           // packed entry -> (a, b) local
-  const int *blk_off, *blk_sz, *blk_base;  // per local m-state
-  const int *twom;                  // per local m-state (doubled m)
-  const int *q0, *q1;               // [npairs]
-  const int *pz, *pn;               // valence numbers per pair, or NULL (zval, nval for all)
-  const VEntry *vlist;
-  const double *t1b;
-  const double2 *jpack;             // [6][npack]: J_x, J_y, J_z, J_x^2, J_y^2, J_z^2 blocks
-  double2 *zres;                    // [npairs][NE][NOBS]
-  int *status;                      // [npairs][NE]
-  // E2 observables (k_point<true>): Q_{2mu} [5][n*n] column-major; orbit
-  // pairs [nob2] (species, first m-state of a, of b, size; size 0 where
-  // l or j differ); outputs [npairs][NE][10] and [npairs][NE][nob2]
-  const double *q2;
-  const int4 *orbp;
-  int nob2;
-  double2 *ze2, *zrho;
-};
-
-constexpr int NOB2_MAX = 2 * 8 * 8;   // orbit pairs of both species (<= 8 orbits per species)
-
-template <bool WITH_E2>
-__global__ __launch_bounds__(NT) void k_point(PointParams p) {
-  __shared__ double2 sm[5 * NMAX * NMAX];
-  __shared__ double2 sR[RPACK_MAX];
-  __shared__ double2 fcol[NMAX];
-  __shared__ double2 prow[2 * NMAX];
-  __shared__ int pfmap[2 * 2 * NMAX], gjmap[NMAX];
-  __shared__ double redv[NWAVE * 8];
-  __shared__ int redk[NWAVE];
-  __shared__ short m_off[HMAX], m_sz[HMAX], m_base[HMAX], m_twom[HMAX];
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int iang = blockIdx.x, ip = blockIdx.y;
-  double2 accq[WITH_E2 ? 10 : 1];  // tid 0: gauge sums of znorm tr_s(Q_mu rho)
-  double2 accd = cz();        // tid < nob2: gauge sum of znorm times one orbit-pair trace
-  if (WITH_E2)
-    for (int k = 0; k < 10; ++k) accq[k] = cz();
-  const int n = p.n, h = n >> 1, nn = n * n, N2 = 2 * n;
-  const int ig = iang % p.ngam, ia = (iang / p.ngam) % p.nalp, ib = iang / (p.ngam * p.nalp);
-  const double alp = p.alp[ia], bet = p.bet[ib], gam = p.gam[ig];
-  const int q0 = p.q0[ip], q1 = p.q1[ip];
-  const double *U0 = p.U + (size_t)q0 * nn, *V0 = p.V + (size_t)q0 * nn;
-  const double *U1 = p.U + (size_t)q1 * nn, *V1 = p.V + (size_t)q1 * nn;
-  const double *Z0 = p.Z + (size_t)q0 * nn, *Z1 = p.Z + (size_t)q1 * nn;
-  const double vac = p.snpfaf * (p.det[q0] * p.vac_scale) * (p.det[q1] * p.vac_scale);
-  (void)bet;
-
-  for (int i = tid; i < h; i += NT) {
-    m_off[i] = (short)p.blk_off[i];
-    m_sz[i] = (short)p.blk_sz[i];
-    m_base[i] = (short)p.blk_base[i];
-    m_twom[i] = (short)p.twom[i];
-  }
-  // R(a, b) = exp(-i alp m_a) d(beta)_{ab} exp(-i gam m_b), packed blocks
-  for (int e = tid; e < p.npack; e += NT) {
-    const int a = p.pk_a[e], b = p.pk_b[e];
-    const double d = p.rtab[(size_t)ib * p.npack + e];
-    const double2 ph = cexpi(-alp * 0.5 * p.twom[a] - gam * 0.5 * p.twom[b]);
-    sR[e] = cscale(d, ph);
-  }
-  __syncthreads();
-
-  double2 *S = sm;                  // 2n x 2n skew matrix (Pfaffian phase)
-  double2 *s0 = sm, *s1 = sm + nn, *s2 = sm + 2 * nn, *s3 = sm + 3 * nn, *s4 = sm + 4 * nn;
-
-  double2 acc[NOBS];
-  for (int o = 0; o < NOBS; ++o) acc[o] = cz();
-  int status = 0;
-  const int nphi = p.nphi;
-  for (int g = 0; g < nphi * nphi; ++g) {
-    const int lp = g / nphi, ln = g % nphi;
-    const double phip = (nphi == 1) ? 0.0 : (2.0 * M_PI / nphi) * lp;
-    const double phin = (nphi == 1) ? 0.0 : (2.0 * M_PI / nphi) * ln;
-    const double2 eph[2] = {cexpi(phip), cexpi(phin)};
-
-    // ---------------------------------------------------------------- Pfaffian
-    // X = A Z1 -> s4 ; A = diag(e^{i phi_p} R, e^{i phi_n} R) (UVphiomega)
-    for (int e = tid; e < nn; e += NT) {
-      const int i = e % n, j = e / n;
-      const int sp = i >= h, il = i - sp * h;
-      const int off = m_off[il], sz = m_sz[il], base = m_base[il];
-      double2 s = cz();
-      for (int k = 0; k < sz; ++k) rfma(s, Z1[(sp * h + off + k) + j * n], sR[base + (il - off) + k * sz]);
-      s4[e] = cmul(eph[sp], s);
-    }
-    __syncthreads();
-    // S = cscale [[A Z1 A^T, -1], [1, -Z0]] (upper triangle; Pfaf_Over with
-    // conj(Vbar) conj(Ubar)^{-1} = A (V1 U1^{-1}) A^T since A is unitary)
-    for (int e = tid; e < N2 * N2; e += NT) {
-      const int i = e % N2, j = e / N2;
-      if (i >= j) {
-        if (i == j) S[e] = cz();
-        continue;
-      }
-      double2 v;
-      if (j < n) {
-        const int sp = j >= h, jl = j - sp * h;
-        const int off = m_off[jl], sz = m_sz[jl], base = m_base[jl];
-        double2 s = cz();
-        for (int k = 0; k < sz; ++k) cfma(s, s4[i + (sp * h + off + k) * n], sR[base + (jl - off) + k * sz]);
-        v = cmul(eph[sp], s);
-      } else if (i < n) {
-        v = make_double2((j - n == i) ? -1.0 : 0.0, 0.0);
-      } else {
-        v = make_double2(-Z0[(i - n) + (j - n) * n], 0.0);
-      }
-      S[i + j * N2] = cscale(p.cscale, v);
-      S[j + i * N2] = cscale(-p.cscale, v);
-    }
-    __syncthreads();
-    const double2 pf = block_pfaffian<NT>(S, N2, pfmap, redv, redk);
-    const double2 gphase = cexpi(-phip * (p.pz ? p.pz[ip] : p.zval) - phin * (p.pn ? p.pn[ip] : p.nval));
-    const double2 znorm = cmul(cscale(vac, pf), gphase);
-    __syncthreads();
-
-    // ---------------------------------------------------------------- densities
-    // s0 = A V1, s1 = A* U1 (UVphiomega / ROKAPPAphiomega)
-    for (int e = tid; e < nn; e += NT) {
-      const int i = e % n, j = e / n;
-      const int sp = i >= h, il = i - sp * h;
-      const int off = m_off[il], sz = m_sz[il], base = m_base[il];
-      double2 a = cz(), b = cz();
-      for (int k = 0; k < sz; ++k) {
-        const double2 r = sR[base + (il - off) + k * sz];
-        const int row = sp * h + off + k;
-        rfma(a, V1[row + j * n], r);
-        rfma(b, U1[row + j * n], cconj(r));
-      }
-      s0[e] = cmul(eph[sp], a);
-      s1[e] = cmul(cconj(eph[sp]), b);
-    }
-    __syncthreads();
-    // s2 = T = V0^T s0 + U0^T s1 ; s3 = 1
-    for (int t = tid; t < (n / 2) * (n / 2); t += NT) {
-      const int i = 2 * (t % (n / 2)), j = 2 * (t / (n / 2));
-      double2 c00 = cz(), c10 = cz(), c01 = cz(), c11 = cz();
-      for (int k = 0; k < n; ++k) {
-        const double v0 = V0[k + i * n], v1 = V0[k + (i + 1) * n];
-        const double u0 = U0[k + i * n], u1 = U0[k + (i + 1) * n];
-        const double2 a0 = s0[k + j * n], a1 = s0[k + (j + 1) * n];
-        const double2 b0 = s1[k + j * n], b1 = s1[k + (j + 1) * n];
-        rfma(c00, v0, a0); rfma(c00, u0, b0);
-        rfma(c10, v1, a0); rfma(c10, u1, b0);
-        rfma(c01, v0, a1); rfma(c01, u0, b1);
-        rfma(c11, v1, a1); rfma(c11, u1, b1);
-      }
-      s2[i + j * n] = c00; s2[i + 1 + j * n] = c10;
-      s2[i + (j + 1) * n] = c01; s2[i + 1 + (j + 1) * n] = c11;
-    }
-    for (int e = tid; e < nn; e += NT) s3[e] = make_double2((e % n) == (e / n) ? 1.0 : 0.0, 0.0);
-    __syncthreads();
-    // Gauss-Jordan: s2 -> 1, s3 -> T^{-1}
-    {
-      const int info = block_gauss_jordan<NT>(s2, s3, n, gjmap, fcol, prow, redk);
-      if (info && !status) status = info;
-    }
-    // s2 = W1 = (A V1) T^{-1}, s4 = W2 = (A* U1) T^{-1}
-    for (int t = tid; t < (n / 2) * (n / 2); t += NT) {
-      const int i = 2 * (t % (n / 2)), j = 2 * (t / (n / 2));
-      double2 a00 = cz(), a10 = cz(), a01 = cz(), a11 = cz();
-      double2 b00 = cz(), b10 = cz(), b01 = cz(), b11 = cz();
-      for (int k = 0; k < n; ++k) {
-        const double2 x0 = s0[i + k * n], x1 = s0[i + 1 + k * n];
-        const double2 y0 = s1[i + k * n], y1 = s1[i + 1 + k * n];
-        const double2 t0 = s3[gjmap[k] + j * n], t1 = s3[gjmap[k] + (j + 1) * n];
-        cfma(a00, x0, t0); cfma(a10, x1, t0); cfma(a01, x0, t1); cfma(a11, x1, t1);
-        cfma(b00, y0, t0); cfma(b10, y1, t0); cfma(b01, y0, t1); cfma(b11, y1, t1);
-      }
-      s2[i + j * n] = a00; s2[i + 1 + j * n] = a10; s2[i + (j + 1) * n] = a01; s2[i + 1 + (j + 1) * n] = a11;
-      s4[i + j * n] = b00; s4[i + 1 + j * n] = b10; s4[i + (j + 1) * n] = b01; s4[i + 1 + (j + 1) * n] = b11;
-    }
-    __syncthreads();
-    // rho = W1 V0^T -> s0, kappa10 = W1 U0^T -> s1, kappa01 = -W2 V0^T -> s3
-    for (int t = tid; t < (n / 2) * (n / 2); t += NT) {
-      const int i = 2 * (t % (n / 2)), j = 2 * (t / (n / 2));
-      double2 r00 = cz(), r10 = cz(), r01 = cz(), r11 = cz();
-      double2 k00 = cz(), k10 = cz(), k01 = cz(), k11 = cz();
-      double2 l00 = cz(), l10 = cz(), l01 = cz(), l11 = cz();
-      for (int k = 0; k < n; ++k) {
-        const double2 w0 = s2[i + k * n], w1 = s2[i + 1 + k * n];
-        const double2 x0 = s4[i + k * n], x1 = s4[i + 1 + k * n];
-        const double v0 = V0[j + k * n], v1 = V0[j + 1 + k * n];
-        const double u0 = U0[j + k * n], u1 = U0[j + 1 + k * n];
-        rfma(r00, v0, w0); rfma(r10, v0, w1); rfma(r01, v1, w0); rfma(r11, v1, w1);
-        rfma(k00, u0, w0); rfma(k10, u0, w1); rfma(k01, u1, w0); rfma(k11, u1, w1);
-        rfma(l00, v0, x0); rfma(l10, v0, x1); rfma(l01, v1, x0); rfma(l11, v1, x1);
-      }
-      s0[i + j * n] = r00; s0[i + 1 + j * n] = r10; s0[i + (j + 1) * n] = r01; s0[i + 1 + (j + 1) * n] = r11;
-      s1[i + j * n] = k00; s1[i + 1 + j * n] = k10; s1[i + (j + 1) * n] = k01; s1[i + 1 + (j + 1) * n] = k11;
-      s3[i + j * n] = cneg(l00); s3[i + 1 + j * n] = cneg(l10);
-      s3[i + (j + 1) * n] = cneg(l01); s3[i + 1 + (j + 1) * n] = cneg(l11);
-    }
-    __syncthreads();
-
-    // ---------------------------------------------------------------- observables
-    double2 j2p[4];
-    block_j2<NT>(s0, s1, s3, n, h, p.jpack, p.npack, m_off, m_sz, m_base, s2, s4, redv, j2p);
-    const double2 J2 = cadd(j2p[0], cadd(cmul(j2p[1], j2p[1]), cadd(cmul(j2p[2], j2p[2]), cmul(j2p[3], j2p[3]))));
-    const double2 *ro = s0, *k10 = s1, *k01 = s3;
-    double2 e2 = cz(), ek = cz(), zp = cz(), zn = cz();
-    for (int q = tid; q < p.nme; q += NT) {
-      const VEntry ve = p.vlist[q];
-      const int i = ve.idx & 255, j = (ve.idx >> 8) & 255, k = (ve.idx >> 16) & 255, l = ve.idx >> 24;
-      const double2 rki = ro[k + i * n], rlj = ro[l + j * n], rli = ro[l + i * n], rkj = ro[k + j * n];
-      const double2 rik = ro[i + k * n], rjl = ro[j + l * n], rjk = ro[j + k * n], ril = ro[i + l * n];
-      const double2 pij = k01[i + j * n], pji = k01[j + i * n], pkl = k01[k + l * n], plk = k01[l + k * n];
-      const double2 qkl = k10[k + l * n], qlk = k10[l + k * n], qij = k10[i + j * n], qji = k10[j + i * n];
-      // HFB_Energy_COMPLEX, PNAMP_Energy_OpenMP.f90:40-78
-      const double2 r_ijkl = cadd(csub(cmul(rki, rlj), cmul(rli, rkj)), cmul(pij, qkl));
-      const double2 r_jikl = cadd(csub(cmul(rkj, rli), cmul(rlj, rki)), cmul(pji, qkl));
-      const double2 r_jilk = cadd(csub(cmul(rlj, rki), cmul(rkj, rli)), cmul(pji, qlk));
-      const double2 r_ijlk = cadd(csub(cmul(rli, rkj), cmul(rki, rlj)), cmul(pij, qlk));
-      const double2 r_klij = cadd(csub(cmul(rik, rjl), cmul(rjk, ril)), cmul(pkl, qij));
-      const double2 r_klji = cadd(csub(cmul(rjk, ril), cmul(rik, rjl)), cmul(pkl, qji));
-      const double2 r_lkij = cadd(csub(cmul(ril, rjk), cmul(rjl, rik)), cmul(plk, qij));
-      const double2 r_lkji = cadd(csub(cmul(rjl, rik), cmul(ril, rjk)), cmul(plk, qji));
-      const double d = (k + l == i + j) ? 0.0 : 1.0;
-      double2 s = csub(r_ijkl, r_jikl);
-      s = csub(s, r_ijlk);
-      s = cadd(s, r_jilk);
-      double2 t = csub(r_klij, r_klji);
-      t = csub(t, r_lkij);
-      t = cadd(t, r_lkji);
-      s = cadd(s, cscale(d, t));
-      e2 = cadd(e2, cscale(0.25 * ve.me, s));
-    }
-    for (int e = tid; e < nn; e += NT) {
-      const int a = e % n, c = e / n;  // E_kin = sum rho(c,a) t(a,c)
-      ek = cadd(ek, cscale(p.t1b[a + c * n], ro[c + a * n]));
-    }
-    for (int i = tid; i < h; i += NT) {
-      zp = cadd(zp, ro[i + i * n]);
-      zn = cadd(zn, ro[(i + h) + (i + h) * n]);
-    }
-    double vals[8] = {e2.x, e2.y, ek.x, ek.y, zp.x, zp.y, zn.x, zn.y};
-    for (int v = 0; v < 8; ++v)
-      for (int off = 32; off > 0; off >>= 1) vals[v] += __shfl_xor(vals[v], off);
-    if (lane == 0)
-      for (int v = 0; v < 8; ++v) redv[wave * 8 + v] = vals[v];
-    __syncthreads();
-    if (tid == 0) {
-      double tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      for (int w = 0; w < NWAVE; ++w)
-        for (int v = 0; v < 8; ++v) tot[v] += redv[w * 8 + v];
-      const double2 E2 = make_double2(tot[0], tot[1]), EK = make_double2(tot[2], tot[3]);
-      const double2 ZP = make_double2(tot[4], tot[5]), ZN = make_double2(tot[6], tot[7]);
-      acc[0] = cadd(acc[0], znorm);
-      acc[1] = cadd(acc[1], cmul(znorm, ZP));
-      acc[2] = cadd(acc[2], cmul(znorm, ZN));
-      acc[3] = cadd(acc[3], cmul(znorm, EK));
-      acc[4] = cadd(acc[4], cmul(znorm, cadd(cadd(E2, EK), make_double2(p.e0, 0.0))));
-      acc[5] = cadd(acc[5], cmul(znorm, J2));
-    }
-    if (WITH_E2) {
-      double *rede2 = (double *)prow;  // NWAVE x 20 doubles; prow is free after the Gauss-Jordan
-      // tr_s(Q_mu rho) = sum_{i, j in s} Q_mu(i, j) rho(j, i) (ZQ2, PNAMP_ME1B.f90:52-75)
-      double q[20];
-      for (int k = 0; k < 20; ++k) q[k] = 0.0;
-      for (int e = tid; e < 2 * h * h; e += NT) {
-        const int sp = e / (h * h), r = e - sp * h * h;
-        const int i = sp * h + r % h, j = sp * h + r / h;
-        const double2 rji = ro[j + i * n];
-        for (int mu = 0; mu < 5; ++mu) {
-          const double qv = p.q2[(size_t)mu * nn + i + (size_t)j * n];
-          q[sp * 10 + 2 * mu] = fma(qv, rji.x, q[sp * 10 + 2 * mu]);
-          q[sp * 10 + 2 * mu + 1] = fma(qv, rji.y, q[sp * 10 + 2 * mu + 1]);
-        }
-      }
-      for (int k = 0; k < 20; ++k)
-        for (int off = 32; off > 0; off >>= 1) q[k] += __shfl_xor(q[k], off);
-      if (lane == 0)
-        for (int k = 0; k < 20; ++k) rede2[wave * 20 + k] = q[k];
-      // orbit-pair traces sum_m rho(k_a(m), k_b(m)) (ZRho1B_Angles, PNAMP_Rho1B.f90:13-33)
-      if (tid < p.nob2) {
-        const int4 op = p.orbp[tid];
-        double2 t = cz();
-        for (int m = 0; m < op.w; ++m) t = cadd(t, ro[(op.y + m) + (op.z + m) * n]);
-        accd = cadd(accd, cmul(znorm, t));
-      }
-      __syncthreads();
-      if (tid == 0)
-        for (int k = 0; k < 10; ++k) {
-          double re = 0.0, im = 0.0;
-          for (int w = 0; w < NWAVE; ++w) {
-            re += rede2[w * 20 + 2 * k];
-            im += rede2[w * 20 + 2 * k + 1];
-          }
-          accq[k] = cadd(accq[k], cmul(znorm, make_double2(re, im)));
-        }
-    }
-    __syncthreads();
-  }
-  // the singular-T flag is per thread; combine through LDS
-  if (lane == 0) redk[wave] = 0;
-  __syncthreads();
-  if (status) redk[wave] = status;  // any nonzero value marks the point
-  __syncthreads();
-  if (tid == 0) {
-    int st = 0;
-    for (int w = 0; w < NWAVE; ++w) st = st ? st : redk[w];
-    const double inv = 1.0 / (double)(nphi * nphi);
-    double2 *out = p.zres + ((size_t)ip * p.NE + iang) * NOBS;
-    for (int o = 0; o < NOBS; ++o) out[o] = cscale(inv, acc[o]);
-    p.status[(size_t)ip * p.NE + iang] = st;
-    if (WITH_E2)
-      for (int k = 0; k < 10; ++k) p.ze2[((size_t)ip * p.NE + iang) * 10 + k] = cscale(inv, accq[k]);
-  }
-  if (WITH_E2 && tid < p.nob2)
-    p.zrho[((size_t)ip * p.NE + iang) * p.nob2 + tid] = cscale(1.0 / (double)(nphi * nphi), accd);
-}
 
 // ------------------------------------------------------------------ factored form
 // For states without proton-neutron mixing (U, V block-diagonal by species
@@ -1093,13 +551,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
     // (as registers they would spill at five waves per SIMD); padding lanes
     // get i = j = 2H, which matches no row or column
     __shared__ unsigned sijL[SL * 64];
-    // columns r (colR), r + 1 (colA) and c (colC) of the step and the pivot
-    __shared__ double2 colbuf[3 * N2 + 1];
-    constexpr unsigned OFF_R = 0, OFF_A = 16 * N2, OFF_C = 32 * N2, OFF_P = 48 * N2;
+    // columns r (colR), r + 1 (colA) and c (colC) of the step, the pivot,
+    // and one slot per lane that takes the writes a lane does not make (the
+    // extraction writes unconditionally: no per-lane branches)
+    __shared__ double2 colbuf[3 * N2 + 1 + 64];
+    constexpr unsigned OFF_R = 0, OFF_A = 16 * N2, OFF_C0 = 32 * N2, OFF_P = 48 * N2;
+    const unsigned off_dummy = 16 * (3 * N2 + 1) + 16u * lane;
     auto at = [&](unsigned off) -> double2 & { return *(double2 *)((char *)colbuf + off); };
     static constexpr PfIndex<N2> pf_index{};
-#pragma unroll
-    for (int s = 0; s < SL; ++s) {
+    // S(i, j) of entry e = 64 s + lane
+    auto entry = [&](int s) -> double2 {
       const int e = s * 64 + lane;
       const unsigned ij = pf_index.v[e];
       const int i = (int)((ij & 0xffffu) >> 4), j = (int)(ij >> 20);
@@ -1110,8 +571,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
         else v = make_double2(-Z0[(i - H) + (size_t)(j - H) * n], 0.0);
         v = cscale(p.cscale, v);
       }
-      sv[s] = v;
-      sijL[s * 64 + lane] = ij;
+      return v;
+    };
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      sv[s] = entry(s);
+      sijL[s * 64 + lane] = pf_index.v[s * 64 + lane];
     }
     // Entries that left the trailing matrix may be overwritten: they are
     // never read again (the last one, S(2H-2, 2H-1), is entry 0).
@@ -1125,7 +590,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
       for (int s = 0; s < SL; ++s) {
         if (s * 64 < cnt && (s + 1) * 64 > lo) {
           const unsigned ij = sijL[s * 64 + lane];
-          const unsigned pk = pivot_key<6>(sv[s].x * sv[s].x + sv[s].y * sv[s].y, (int)(ij >> 20));
+          const double2 v = sv[s];
+          const unsigned pk = pivot_key<6>(v.x * v.x + v.y * v.y, (int)(ij >> 20));
           key = ((ij & 0xffffu) == r16) ? max(key, pk) : key;
         }
       }
@@ -1133,24 +599,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
       const int c = 63 - (int)(key & 63u);
       const unsigned c16 = 16u * c;
       const bool swap = c != r + 1;
+      // without the exchange (c = r + 1) column c is column r + 1
+      const unsigned OFF_C = swap ? OFF_C0 : OFF_A;
       // columns of the step with the exchange c <-> r + 1 of the trailing
       // indices folded in (only indices >= r + 2 are read later):
       //   colR[x] = S(x, r), colR[c] <- S(r + 1, r), pivot S(r, c) aside;
       //   colA[x] = S(x, r + 1);  colC[x] = S(x, c), colC[c] <- S(r + 1, c).
       // Rows r, r + 1 are entries [cnt2, cnt), row and column c [C(2H-1-c, 2), cnt).
+      // An entry S(i, j) (i < j) of row r, r + 1 or c goes to one column
+      // (first write), one of column c (i > r) to colC (second write); the
+      // other lanes write their own dummy slot.
       const int loc = min(cnt2, (N2 - 1 - c) * (N2 - 2 - c) / 2);
 #pragma unroll
       for (int s = 0; s < SL; ++s) {
         if (s * 64 < cnt && (s + 1) * 64 > loc) {
           const unsigned ij = sijL[s * 64 + lane], i16 = ij & 0xffffu, j16 = ij >> 16;
           const double2 v = sv[s];
-          if (i16 == r16) {
-            if (swap && j16 == c16) at(OFF_P) = v;
-            else at(OFF_R + ((swap && j16 == r16b) ? c16 : j16)) = cneg(v);
-          }
-          if (i16 == r16b) at(OFF_A + j16) = cneg(v);
-          if (i16 == c16) at(OFF_C + j16) = cneg(v);
-          if (j16 == c16 && i16 > r16) at(OFF_C + ((i16 == r16b) ? c16 : i16)) = v;
+          const bool rowR = i16 == r16, rowA = i16 == r16b, rowC = i16 == c16;
+          const bool piv = rowR && swap && j16 == c16;
+          unsigned o1 = off_dummy;
+          o1 = rowC ? OFF_C + j16 : o1;
+          o1 = rowA ? OFF_A + j16 : o1;
+          o1 = rowR ? OFF_R + ((swap && j16 == r16b) ? c16 : j16) : o1;
+          o1 = piv ? OFF_P : o1;
+          at(o1) = piv ? v : cneg(v);
+          const bool colc = j16 == c16 && i16 > r16;
+          at(colc ? OFF_C + (rowA ? c16 : i16) : off_dummy) = v;
         }
       }
       __syncthreads();
@@ -1161,27 +635,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART
       }
       const double2 inv = crecip1(spiv);
       pf = cmul(pf, swap ? cneg(spiv) : spiv);
+      // colC / S(r, c) once per step (the update then needs no product with 1 / pivot)
+      if (lane >= r + 2 && lane < N2) at(OFF_C + 16u * lane) = cmul(at(OFF_C + 16u * lane), inv);
+      __syncthreads();
       // the entries of row and column c take those of r + 1 (S'(x, c) =
       // S(x, r + 1) = colA[x]), then the Schur update of the trailing entries
-      // i, j >= r + 2: S' = S + (S'(i,r+1) S'(j,r) - S'(i,r) S'(j,r+1)) / S(r, c)
+      // i, j >= r + 2: S' = S + (S'(i,r+1) S'(j,r) - S'(i,r) S'(j,r+1)) / S(r, c),
+      // with colC already divided by S(r, c)
       if (c >= r + 2) {
         const int lof = (N2 - 1 - c) * (N2 - 2 - c) / 2;
 #pragma unroll
         for (int s = 0; s < SL; ++s) {
           if (s * 64 < cnt2 && (s + 1) * 64 > lof) {
             const unsigned ij = sijL[s * 64 + lane], i16 = ij & 0xffffu, j16 = ij >> 16;
-            if (j16 == c16) sv[s] = at(OFF_A + i16);
-            else if (i16 == c16) sv[s] = cneg(at(OFF_A + j16));
+            const bool jc = j16 == c16, ic = i16 == c16;
+            const double2 x = at(OFF_A + (jc ? i16 : j16));  // in colbuf also for padding lanes
+            sv[s] = jc ? x : (ic ? cneg(x) : sv[s]);
           }
         }
       }
 #pragma unroll
       for (int s = 0; s < SL; ++s) {
+        if (s % 2 == 0) __builtin_amdgcn_sched_barrier(0);  // bounds the LDS reads in flight
         if (s * 64 < cnt2) {
           const unsigned ij = sijL[s * 64 + lane], i16 = ij & 0xffffu, j16 = ij >> 16;
-          double2 t = cmul(at(OFF_C + i16), at(OFF_R + j16));
-          cfms(t, at(OFF_R + i16), at(OFF_C + j16));
-          cfma(sv[s], t, inv);
+          cfma(sv[s], at(OFF_C + i16), at(OFF_R + j16));
+          cfms(sv[s], at(OFF_R + i16), at(OFF_C + j16));
         }
       }
       __syncthreads();
@@ -2378,8 +1857,6 @@ struct pnamp_ctx {
   // the work buffer of k_plain_* (units of the current chunk)
   uint4 *d_plofs = nullptr;
   double *d_plme = nullptr;
-  VEntry *d_vlist = nullptr;        // the list as given (k_point, n above PLAIN_NEW_NMAX)
-  double2 *d_jpack = nullptr;       // packed J_x, J_y, J_z and squares (k_point)
   int pl_n = 0, pl_bytes = 0;
   void *d_pwork = nullptr;
   size_t pwork_bytes = 0;
@@ -2420,6 +1897,8 @@ struct pnamp_ctx {
   // captured launch sequence of the factored form, valid for (npairs, zval,
   // nval) and the current buffers; dropped whenever a buffer is reallocated
   hipGraphExec_t fac_graph = nullptr;
+  int state_cap = 0, sep_cap = 0;   // states the state buffers (and the species blocks) hold
+  int graph_captures = 0, graph_replays = 0;
   int g_np = -1, g_z = 0, g_n = 0, g_form = -1, g_zn = 0;
   // mixed forms: the dense V-contraction matrices (W_pp, W_nn, Q_pp, Q_nn,
   // W_pn; kp x kp row-major FP64 on the host), their fragment-ordered device
@@ -2534,7 +2013,7 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
   };
   const int n = basis->n;
   if (n <= 0 || (n & 1)) return bail(PNAMP_E_ARG, "n must be positive and even");
-  if (n > NMAX) return bail(PNAMP_E_SIZE, "n exceeds PNAMP_NMAX");
+  if (n > NMAX || !plain_supported(n)) return bail(PNAMP_E_SIZE, "n exceeds PNAMP_NMAX");
   const int h = n / 2;
   for (int i = 0; i < n; ++i)
     if (basis->t[i] != (i < h ? -1 : 1)) return bail(PNAMP_E_ARG, "basis must list protons (t=-1) then neutrons (t=+1)");
@@ -2564,38 +2043,6 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
     i += s;
   }
   if ((int)pa.size() > RPACK_MAX) return bail(PNAMP_E_SIZE, "rotation blocks exceed the packed LDS table");
-  // one-body J_x, J_y, J_z and their squares in the packed block layout
-  std::vector<double2> jpk(6 * pa.size());
-  {
-    const size_t np = pa.size();
-    std::vector<double> jpl(np, 0.0);  // <a|J_+|b>
-    for (size_t e = 0; e < np; ++e) {
-      const int a = pa[e], b = pb[e];
-      const double j = basis->twoj[a] / 2.0, ma = basis->twom[a] / 2.0, mb = basis->twom[b] / 2.0;
-      if (std::fabs(ma - mb - 1.0) < 1e-9) jpl[e] = std::sqrt(j * (j + 1) - mb * (mb + 1));
-    }
-    auto at = [&](int a, int b) -> size_t {  // packed index of (a, b) in a's block
-      return (size_t)base[a] + (a - off[a]) + (size_t)(b - off[a]) * sz[a];
-    };
-    for (size_t e = 0; e < np; ++e) {
-      const int a = pa[e], b = pb[e];
-      const double p = jpl[e], m = jpl[at(b, a)];
-      jpk[e] = make_double2(0.5 * (p + m), 0.0);                    // J_x
-      jpk[np + e] = make_double2(0.0, -0.5 * (p - m));              // J_y = (J+ - J-)/(2i)
-      jpk[2 * np + e] = make_double2(a == b ? basis->twom[a] / 2.0 : 0.0, 0.0);
-    }
-    for (int k = 0; k < 3; ++k)
-      for (size_t e = 0; e < np; ++e) {
-        const int a = pa[e], b = pb[e];
-        double2 acc = make_double2(0.0, 0.0);
-        for (int q = off[a]; q < off[a] + sz[a]; ++q) {
-          const double2 x = jpk[k * np + at(a, q)], y = jpk[k * np + at(q, b)];
-          acc.x += x.x * y.x - x.y * y.y;
-          acc.y += x.x * y.y + x.y * y.x;
-        }
-        jpk[(3 + k) * np + e] = acc;
-      }
-  }
   if (mesh->nalp < 1 || mesh->nbet < 1 || mesh->ngam < 1 || mesh->nphi < 1 || mesh->jmax < 0)
     return bail(PNAMP_E_ARG, "bad mesh sizes");
   if (mesh->jmax > JMAX_MAX) return bail(PNAMP_E_SIZE, "jmax too large");
@@ -2609,10 +2056,8 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
   if (!c->scratch) return bail(PNAMP_E_DEVICE, c->err);
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto &e : c->ev) HIPCHK(c, hipEventCreate(&e));
-  if (plain_supported(n)) {
-    HIPCHK(c, plain_prepare(n, false));
-    HIPCHK(c, plain_prepare(n, true));
-  }
+  HIPCHK(c, plain_prepare(n, false));
+  HIPCHK(c, plain_prepare(n, true));
   HIPCHK(c, hipFuncSetAttribute((const void *)k_state_prep, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)(sizeof(double) * (2 * (size_t)n * n + n))));
   double fact[40];
@@ -2647,7 +2092,6 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
       (r = dupload(c, &c->d_bet, bet.data(), c->nbet)) || (r = dupload(c, &c->d_wbet, wbet.data(), c->nbet)) ||
       (r = dupload(c, &c->d_gam, mesh->gam, c->ngam)) || (r = dupload(c, &c->d_wgam, mesh->wgam, c->ngam)))
     return bail(r, c->err);
-  if ((r = dupload(c, &c->d_jpack, jpk.data(), jpk.size()))) return bail(r, c->err);
   {
     // ladder-operator table of the factored form, per species-local state a
     // (m = j .. -j inside an orbit, so a - 1 is the m + 1 partner of a)
@@ -2721,14 +2165,6 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
       lm.push_back(0.0);
     }
     c->pl_n = (int)lo.size();
-    std::vector<VEntry> vl(inter->nme);
-    for (int64_t q = 0; q < inter->nme; ++q) {
-      vl[q].me = inter->me2b[q];
-      vl[q].idx = (uint32_t)(inter->ka[q] - 1) | ((uint32_t)(inter->kb[q] - 1) << 8) |
-                  ((uint32_t)(inter->kc[q] - 1) << 16) | ((uint32_t)(inter->kd[q] - 1) << 24);
-      vl[q].pad = 0;
-    }
-    if ((r = dupload(c, &c->d_vlist, vl.data(), vl.size()))) return bail(r, c->err);
     if ((r = dupload(c, &c->d_plofs, lo.data(), lo.size())) || (r = dupload(c, &c->d_plme, lm.data(), lm.size())) ||
         (r = dupload(c, &c->d_t1b, inter->t1b, (size_t)n * n)))
       return bail(r, c->err);
@@ -2949,10 +2385,18 @@ extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, cons
   const size_t nn = (size_t)c->n * c->n;
   int r = 0;
   c->separable = false;
-  if ((r = dupload(c, &c->d_U, U, nn * nstates)) || (r = dupload(c, &c->d_V, V, nn * nstates)) ||
-      (r = dalloc(c, &c->d_Z, nn * nstates)) || (r = dalloc(c, &c->d_det, (size_t)nstates)) ||
-      (r = dalloc(c, &c->d_state_status, (size_t)nstates)))
-    return r;
+  if (nstates <= c->state_cap) {
+    // same buffers: a captured launch graph (which holds their addresses) stays valid
+    HIPCHK(c, hipMemcpy(c->d_U, U, sizeof(double) * nn * nstates, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_V, V, sizeof(double) * nn * nstates, hipMemcpyHostToDevice));
+  } else {
+    c->state_cap = 0;
+    if ((r = dupload(c, &c->d_U, U, nn * nstates)) || (r = dupload(c, &c->d_V, V, nn * nstates)) ||
+        (r = dalloc(c, &c->d_Z, nn * nstates)) || (r = dalloc(c, &c->d_det, (size_t)nstates)) ||
+        (r = dalloc(c, &c->d_state_status, (size_t)nstates)))
+      return r;
+    c->state_cap = nstates;
+  }
   PrepParams pp{c->n, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status};
   hipLaunchKernelGGL(k_state_prep, dim3(nstates), dim3(256), sizeof(double) * (2 * nn + c->n), c->stream, pp);
   HIPCHK(c, hipGetLastError());
@@ -3027,9 +2471,19 @@ extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, cons
           UfT[b * h * h + (size_t)j * h + k] = Uf[b * h * h + j + (size_t)k * h];
           VfT[b * h * h + (size_t)j * h + k] = Vf[b * h * h + j + (size_t)k * h];
         }
-    if ((r2 = dupload(c, &c->d_Uf, Uf.data(), Uf.size())) || (r2 = dupload(c, &c->d_Vf, Vf.data(), Vf.size())) ||
-        (r2 = dupload(c, &c->d_UfT, UfT.data(), UfT.size())) || (r2 = dupload(c, &c->d_VfT, VfT.data(), VfT.size())))
-      return r2;
+    if (nstates <= c->sep_cap) {
+      const size_t b = sizeof(double) * Uf.size();
+      HIPCHK(c, hipMemcpy(c->d_Uf, Uf.data(), b, hipMemcpyHostToDevice));
+      HIPCHK(c, hipMemcpy(c->d_Vf, Vf.data(), b, hipMemcpyHostToDevice));
+      HIPCHK(c, hipMemcpy(c->d_UfT, UfT.data(), b, hipMemcpyHostToDevice));
+      HIPCHK(c, hipMemcpy(c->d_VfT, VfT.data(), b, hipMemcpyHostToDevice));
+    } else {
+      c->sep_cap = 0;
+      if ((r2 = dupload(c, &c->d_Uf, Uf.data(), Uf.size())) || (r2 = dupload(c, &c->d_Vf, Vf.data(), Vf.size())) ||
+          (r2 = dupload(c, &c->d_UfT, UfT.data(), UfT.size())) || (r2 = dupload(c, &c->d_VfT, VfT.data(), VfT.size())))
+        return r2;
+      c->sep_cap = nstates;
+    }
   }
   return PNAMP_OK;
 }
@@ -3084,23 +2538,6 @@ static int ensure_pairs(pnamp_ctx *c, int np) {
   return 0;
 }
 
-// Arguments of k_point for the pairs q0/q1 (device), writing zres/status.
-static PointParams point_params(const pnamp_ctx *c, int zval, int nval, const int *q0, const int *q1,
-                                double2 *zres, int *status) {
-  PointParams pp;
-  pp.n = c->n; pp.nalp = c->nalp; pp.nbet = c->nbet; pp.ngam = c->ngam; pp.nphi = c->nphi;
-  pp.zval = zval; pp.nval = nval; pp.npack = c->npack; pp.NE = c->NE; pp.nme = c->nme;
-  pp.cscale = c->cscale; pp.vac_scale = c->vac_scale; pp.snpfaf = c->snpfaf; pp.e0 = c->e0;
-  pp.alp = c->d_alp; pp.bet = c->d_bet; pp.gam = c->d_gam;
-  pp.U = c->d_U; pp.V = c->d_V; pp.Z = c->d_Z; pp.det = c->d_det;
-  pp.rtab = c->d_rtab; pp.pk_a = c->d_pk_a; pp.pk_b = c->d_pk_b;
-  pp.blk_off = c->d_blk_off; pp.blk_sz = c->d_blk_sz; pp.blk_base = c->d_blk_base; pp.twom = c->d_twom;
-  pp.q0 = q0; pp.q1 = q1; pp.vlist = c->d_vlist; pp.t1b = c->d_t1b; pp.jpack = c->d_jpack;
-  pp.zres = zres; pp.status = status;
-  pp.q2 = nullptr; pp.orbp = nullptr; pp.nob2 = 0; pp.ze2 = nullptr; pp.zrho = nullptr;
-  pp.pz = nullptr; pp.pn = nullptr;
-  return pp;
-}
 
 // Arguments of the plain-form kernels for the pairs q0/q1 (device), writing
 // zres/status per (pair, Euler point).
@@ -3322,28 +2759,17 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
         c->fac_graph = nullptr;
         return fail(c, PNAMP_E_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ins));
       }
+      ++c->graph_captures;
       c->g_np = npairs;
       c->g_z = zval;
       c->g_n = nval;
       c->g_form = c->form;
       c->g_zn = per_pair ? 1 : 0;
     }
-    if (!direct) HIPCHK(c, hipGraphLaunch(c->fac_graph, c->stream));
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  } else if (!plain_supported(c->n)) {
-    // n above PLAIN_NEW_NMAX: one workgroup per (pair, Euler point) over all gauge points
-    PointParams pp = point_params(c, zval, nval, c->d_q0, c->d_q1, c->d_zres, c->d_status);
-    if (per_pair) {
-      pp.pz = c->d_pz;
-      pp.pn = c->d_pn;
+    if (!direct) {
+      HIPCHK(c, hipGraphLaunch(c->fac_graph, c->stream));
+      ++c->graph_replays;
     }
-    if (e2) {
-      pp.q2 = c->d_q2; pp.orbp = c->d_orbp; pp.nob2 = c->nob2; pp.ze2 = c->d_ze2; pp.zrho = c->d_zrho;
-    }
-    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    if (e2) hipLaunchKernelGGL(k_point<true>, dim3(c->NE, npairs), dim3(NT), 0, c->stream, pp);
-    else hipLaunchKernelGGL(k_point<false>, dim3(c->NE, npairs), dim3(NT), 0, c->stream, pp);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   } else {
@@ -3564,10 +2990,9 @@ extern "C" int pnamp_kernels_batched_noalloc(pnamp_ctx *c, int npairs, const int
   double2 *sk = (double2 *)((char *)sst + al(sizeof(int) * (size_t)chunk * c->NE));
   int *spst = (int *)((char *)sk + al(sizeof(double2) * kper * chunk));
   void *swork = base + half;
-  const bool newk = plain_supported(c->n);
   PlainArgs pa = plain_args(c, zval, nval, sq0, sq1, szr, sst);
   const size_t ub = plain_unit_bytes(pa);
-  if (newk && ub > half)
+  if (ub > half)
     return fail(c, PNAMP_E_SIZE, "one (pair, Euler point) unit needs " + std::to_string(ub) + " bytes of work buffer; " +
                                      std::to_string(half) + " available");
   const size_t wu = std::max<size_t>(1, half / ub);  // units per launch
@@ -3581,14 +3006,8 @@ extern "C" int pnamp_kernels_batched_noalloc(pnamp_ctx *c, int npairs, const int
     // workgroup writes its own gauge-point slots before k_plain_sum reads
     // them: nothing left by an earlier call is read
     const size_t units = (size_t)np * c->NE;
-    if (newk) {
-      for (size_t u0 = 0; u0 < units; u0 += wu)
-        HIPCHK(c, plain_run(pa, (int)u0, (int)std::min(wu, units - u0), swork, c->stream));
-    } else {
-      const PointParams pp = point_params(c, zval, nval, sq0, sq1, szr, sst);
-      hipLaunchKernelGGL(k_point<false>, dim3(c->NE, np), dim3(NT), 0, c->stream, pp);
-      HIPCHK(c, hipGetLastError());
-    }
+    for (size_t u0 = 0; u0 < units; u0 += wu)
+      HIPCHK(c, plain_run(pa, (int)u0, (int)std::min(wu, units - u0), swork, c->stream));
     HIPCHK(c, launch_project(c, np, proj_params(c, szr, sst, sk, spst)));
     if (kern)
       HIPCHK(c, hipMemcpyAsync(kern + 2 * kper * p0, sk, sizeof(double2) * kper * np, hipMemcpyDeviceToHost, c->stream));
@@ -3754,6 +3173,13 @@ extern "C" int pnamp_last_timing(pnamp_ctx *c, float *ms_points, float *ms_total
   return PNAMP_OK;
 }
 
+extern "C" int pnamp_graph_info(pnamp_ctx *c, int *captures, int *replays) {
+  if (!c) return PNAMP_E_ARG;
+  if (captures) *captures = c->graph_captures;
+  if (replays) *replays = c->graph_replays;
+  return PNAMP_OK;
+}
+
 extern "C" int pnamp_n_euler(pnamp_ctx *c) { return c ? c->NE : PNAMP_E_ARG; }
 
 extern "C" const char *pnamp_last_error(pnamp_ctx *c) { return c ? c->err.c_str() : "null context"; }
@@ -3763,7 +3189,7 @@ extern "C" void pnamp_destroy(pnamp_ctx *c) {
   drop_graph(c);
   void *bufs[] = {c->d_alp, c->d_walp, c->d_bet, c->d_wbet, c->d_gam, c->d_wgam, c->d_rtab, c->d_ptab,
                   c->d_pk_a, c->d_pk_b, c->d_blk_off, c->d_blk_sz, c->d_blk_base, c->d_twoj, c->d_twom,
-                  c->d_plofs, c->d_plme, c->d_pwork, c->d_vlist, c->d_jpack, c->d_t1b, c->d_jtab, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status, c->d_q0,
+                  c->d_plofs, c->d_plme, c->d_pwork, c->d_t1b, c->d_jtab, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status, c->d_q0,
                   c->d_q1, c->d_pz, c->d_pn, c->d_zres, c->d_kern, c->d_status, c->d_pair_status,
                   c->d_lofs[0], c->d_lofs[1], c->d_lme[0], c->d_lme[1], c->d_Uf, c->d_Vf,
                   c->d_UfT, c->d_VfT, c->d_mtab, c->d_fout, c->d_frho,

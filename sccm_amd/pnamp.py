@@ -92,6 +92,8 @@ def load_library(path: str = LIB_PATH):
     L.pnamp_hw_presolve.restype = C.c_int
     L.pnamp_last_timing.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.pnamp_last_timing.restype = C.c_int
+    L.pnamp_graph_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.pnamp_graph_info.restype = C.c_int
     L.pnamp_n_euler.argtypes = [vp]
     L.pnamp_n_euler.restype = C.c_int
     L.pnamp_last_error.argtypes = [vp]
@@ -277,6 +279,12 @@ class PNAMP:
     def timing(self) -> tuple[float, float]:
         a, b = C.c_float(), C.c_float()
         self._check(self._L.pnamp_last_timing(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def graph_info(self) -> tuple[int, int]:
+        """(graphs captured, replays launched) of the factored and mixed forms."""
+        a, b = C.c_int(), C.c_int()
+        self._check(self._L.pnamp_graph_info(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
     def hw_solve(self, N: np.ndarray, H: np.ndarray, eps: float, nos: int = 0, want_f: bool = True):

@@ -324,6 +324,28 @@ def test_factored_graph_replays_match():
     g.close()
 
 
+def test_factored_graph_survives_set_states():
+    """New states of the same count are written into the same buffers: the
+    captured graph is replayed, not captured again, and gives the kernels of
+    a fresh context with those states."""
+    from sccm_amd.pnamp import FORM_FACTORED
+    w = configs.pf_gcm(nstates=4, nalp=2, nbet=3, nphi=5, jmax=4)
+    U2, V2 = configs.make_states(w.basis, w.zval, w.nval, w.coords, seed=777)
+    g = _gpu(w.basis, w.inter, w.mesh)
+    g.set_states(w.U, w.V)
+    g.set_form(FORM_FACTORED)
+    g.kernels_batched(w.pairs, w.zval, w.nval)
+    g.set_states(U2, V2)
+    k2 = g.kernels_batched(w.pairs, w.zval, w.nval)
+    assert g.graph_info() == (1, 2)
+    f = _gpu(w.basis, w.inter, w.mesh)
+    f.set_states(U2, V2)
+    f.set_form(FORM_FACTORED)
+    assert np.array_equal(f.kernels_batched(w.pairs, w.zval, w.nval), k2)
+    g.close()
+    f.close()
+
+
 # Mixed forms: the V contraction on MFMA in FP32 or split BF16.  Observables
 # without the two-body contraction (norm, Z, N, E_kin, <J^2>) stay FP64 and
 # match the factored form to 1e-10; the energy kernel carries the GEMM
