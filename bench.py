@@ -200,6 +200,7 @@ def main():
     ap.add_argument("--form", choices=("factored", "mixed-fp32", "mixed-bf16", "plain"), default="factored")
     ap.add_argument("--levels", type=int, default=3, help="lowest HW levels per J compared")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="factored form: keep the default launch configuration")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain reference pass and the GCM check")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -227,6 +228,13 @@ def main():
         idx = [(s * P + i) % npairs for i in range(P)]
         return w.pairs[idx]
 
+    tune = None
+    if form == FORM_FACTORED and not args.no_tune:
+        t = time.perf_counter()
+        tune = ctx.autotune(batch(0), w.zval, w.nval)
+        tune["seconds"] = time.perf_counter() - t
+        log(f"bench: autotuned in {tune['seconds']:.1f} s: Pfaffian {tune['pf_waves']} waves/SIMD, "
+            f"densities {tune['rest_waves']}, chunk / {tune['chunk_div']}")
     for s in range(args.warmup):
         ctx.kernels_batched(batch(s), w.zval, w.nval)
         log(f"bench: warmup step {s} done")
@@ -318,6 +326,8 @@ def main():
             "reference_sequence_equivalent_tflops": plain_equiv,
         },
     }
+    if tune is not None:
+        out["autotune"] = tune
     if form == FORM_PLAIN:
         own = plain_own_flops(w.n, plain_terms(w.inter, w.n), w.mesh.nphi) * pts_step
         out["roofline"]["own_algorithm_flops_per_launch"] = own

@@ -238,11 +238,27 @@ int pnamp_hw_presolve(pnamp_ctx *ctx, int dim, const double *N, const double *H,
  * stream: the mesh-point kernel and the whole call (ms). */
 int pnamp_last_timing(pnamp_ctx *ctx, float *ms_points, float *ms_total);
 
+/* Autotuner of the FP64 factored form (no counterpart in the reference):
+ * times every candidate launch configuration on the given pairs (real
+ * kernels on the context's states, each candidate captured and replayed
+ * once) and keeps the fastest for later calls of this context.  Candidates
+ * (species size h >= 8; smaller spaces have one): waves per SIMD of the
+ * Pfaffian kernel {4, 5, 6}, of the density kernel {default (3 while
+ * 3 h <= 64, else 2), 4 (h <= 16 only)}, and the pair chunk divided by
+ * {1, 2, 4}.  choice[3] receives
+ * (Pfaffian waves, density waves, chunk divisor), ms[i] the kernel time of
+ * candidate i (at most PNAMP_TUNE_MAX), *ncand their number; any may be
+ * NULL.  The form must be PNAMP_FORM_FACTORED. */
+#define PNAMP_TUNE_MAX 32
+int pnamp_autotune(pnamp_ctx *ctx, int npairs, const int32_t *q0, const int32_t *q1, int zval, int nval,
+                   int *choice, float *ms, int *ncand);
+
 /* Launch-graph counters of the factored and mixed forms: graphs captured
- * and replays launched since pnamp_create.  A call of the same shape and form
- * replays the graph; pnamp_set_states with no more states than the context
- * holds keeps it (the state buffers are rewritten in place). */
-int pnamp_graph_info(pnamp_ctx *ctx, int *captures, int *replays);
+ * and replays launched since pnamp_create, and the kernel nodes of the last
+ * captured graph (the launches one call replays).  A call of the same shape
+ * and form replays the graph; pnamp_set_states with no more states than the
+ * context holds keeps it (the state buffers are rewritten in place). */
+int pnamp_graph_info(pnamp_ctx *ctx, int *captures, int *replays, int *nodes);
 
 /* Number of Euler points of the context's mesh. */
 int pnamp_n_euler(pnamp_ctx *ctx);

@@ -506,8 +506,14 @@ struct PfIndex {
   }
 };
 
-template <int H, int MIX, int PART>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PART == PART_PF ? 5 : (3 * H <= 64 ? 3 : 2)))) void k_fac_phi(PhiParams p) {
+// Waves per SIMD the register budget is fitted to (the autotuner's knob;
+// these are the defaults): the Pfaffian part five, the rest three while one
+// Gauss-Jordan pass fits the wavefront, else two.
+template <int H, int PART>
+constexpr int fac_default_waves() { return PART == PART_PF ? 5 : (3 * H <= 64 ? 3 : 2); }
+
+template <int H, int MIX, int PART, int WV = fac_default_waves<H, PART>()>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k_fac_phi(PhiParams p) {
   constexpr int N2 = 2 * H, NU = N2 * (N2 - 1) / 2, SL = (NU + 63) / 64, HH = H * H;
   constexpr int KP = (HH + 31) / 32 * 32;  // V-contraction K padded to the MFMA tile
   constexpr int PASSES = (3 * H <= 64) ? 1 : 2;
@@ -1341,29 +1347,42 @@ static bool launch_vgemm(int h, int mix, dim3 grid, hipStream_t s, const VGemmPa
   }
 }
 
+// Occupancy variants of the FP64 factored form that the autotuner times
+// (species sizes h >= 8; smaller spaces take the defaults): Pfaffian part at
+// four, five or six waves per SIMD, the rest at its default or, for h <= 16,
+// four (a tighter register cap).  0 = the default.
+struct FacTune {
+  int pf_waves = 0, rest_waves = 0, chunk_div = 1;
+};
+
 template <int H>
-static void launch_phi(int mix, unsigned grid, hipStream_t s, const PhiParams &pp) {
-  hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF>), dim3(grid), dim3(64), 0, s, pp);
-  if (mix == MIX_FP32) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP32, PART_REST>), dim3(grid), dim3(64), 0, s, pp);
-  else if (mix == MIX_BF16) hipLaunchKernelGGL((k_fac_phi<H, MIX_BF16, PART_REST>), dim3(grid), dim3(64), 0, s, pp);
-  else hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_REST>), dim3(grid), dim3(64), 0, s, pp);
+static void launch_phi(int mix, const FacTune &t, unsigned grid, hipStream_t s, const PhiParams &pp) {
+  const dim3 g(grid), b(64);
+  if (H >= 8 && mix == MIX_FP64 && t.pf_waves == 4) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF, 4>), g, b, 0, s, pp);
+  else if (H >= 8 && mix == MIX_FP64 && t.pf_waves == 6) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF, 6>), g, b, 0, s, pp);
+  else hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF>), g, b, 0, s, pp);
+  if (mix == MIX_FP32) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP32, PART_REST>), g, b, 0, s, pp);
+  else if (mix == MIX_BF16) hipLaunchKernelGGL((k_fac_phi<H, MIX_BF16, PART_REST>), g, b, 0, s, pp);
+  else if (H >= 8 && H <= 16 && t.rest_waves == 4)
+    hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_REST, 4>), g, b, 0, s, pp);
+  else hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_REST>), g, b, 0, s, pp);
 }
 
 // Species sizes h = n / 2 with an instantiated k_fac_phi (every h is even:
 // an orbit holds 2j + 1 states).
-static bool launch_phi_h(int h, int mix, unsigned grid, hipStream_t s, const PhiParams &pp) {
+static bool launch_phi_h(int h, int mix, const FacTune &t, unsigned grid, hipStream_t s, const PhiParams &pp) {
   switch (h) {
-    case 2: launch_phi<2>(mix, grid, s, pp); return true;
-    case 4: launch_phi<4>(mix, grid, s, pp); return true;
-    case 6: launch_phi<6>(mix, grid, s, pp); return true;
-    case 8: launch_phi<8>(mix, grid, s, pp); return true;
-    case 10: launch_phi<10>(mix, grid, s, pp); return true;
-    case 12: launch_phi<12>(mix, grid, s, pp); return true;
-    case 14: launch_phi<14>(mix, grid, s, pp); return true;
-    case 16: launch_phi<16>(mix, grid, s, pp); return true;
-    case 18: launch_phi<18>(mix, grid, s, pp); return true;
-    case 20: launch_phi<20>(mix, grid, s, pp); return true;
-    case 22: launch_phi<22>(mix, grid, s, pp); return true;
+    case 2: launch_phi<2>(mix, t, grid, s, pp); return true;
+    case 4: launch_phi<4>(mix, t, grid, s, pp); return true;
+    case 6: launch_phi<6>(mix, t, grid, s, pp); return true;
+    case 8: launch_phi<8>(mix, t, grid, s, pp); return true;
+    case 10: launch_phi<10>(mix, t, grid, s, pp); return true;
+    case 12: launch_phi<12>(mix, t, grid, s, pp); return true;
+    case 14: launch_phi<14>(mix, t, grid, s, pp); return true;
+    case 16: launch_phi<16>(mix, t, grid, s, pp); return true;
+    case 18: launch_phi<18>(mix, t, grid, s, pp); return true;
+    case 20: launch_phi<20>(mix, t, grid, s, pp); return true;
+    case 22: launch_phi<22>(mix, t, grid, s, pp); return true;
     default: return false;
   }
 }
@@ -1898,7 +1917,8 @@ struct pnamp_ctx {
   // nval) and the current buffers; dropped whenever a buffer is reallocated
   hipGraphExec_t fac_graph = nullptr;
   int state_cap = 0, sep_cap = 0;   // states the state buffers (and the species blocks) hold
-  int graph_captures = 0, graph_replays = 0;
+  int graph_captures = 0, graph_replays = 0, graph_nodes = 0;
+  FacTune tune;                     // occupancy variants and chunking of the FP64 factored form
   int g_np = -1, g_z = 0, g_n = 0, g_form = -1, g_zn = 0;
   // mixed forms: the dense V-contraction matrices (W_pp, W_nn, Q_pp, Q_nn,
   // W_pn; kp x kp row-major FP64 on the host), their fragment-ordered device
@@ -2631,7 +2651,9 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
     const size_t cols_pp = (size_t)2 * c->NE * c->nphi;  // GEMM columns per pair and species
     size_t bytes_pp = 16 * (rho_pp + sb_pp + out_pp) + 4 * st_pp;
     if (mix != MIX_FP64) bytes_pp += 4 * 4 * cols_pp * kp + 16 * rho_pp + 8 * (size_t)c->NE * c->nphi * hh;
-    const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)npairs, ((size_t)24 << 30) / bytes_pp));
+    const size_t chunk_max = std::max<size_t>(1, ((size_t)24 << 30) / bytes_pp);
+    const size_t chunk_t = mix == MIX_FP64 ? std::max<size_t>(1, chunk_max / (size_t)std::max(1, c->tune.chunk_div)) : chunk_max;
+    const int chunk = (int)std::min<size_t>((size_t)npairs, chunk_t);
     if ((size_t)chunk > c->fac_cap) {
       if ((r = dalloc(c, &c->d_frho, rho_pp * chunk)) || (r = dalloc(c, &c->d_sbuf, sb_pp * chunk)) ||
           (r = dalloc(c, &c->d_fout, out_pp * chunk)) || (r = dalloc(c, &c->d_fstatus, st_pp * chunk)))
@@ -2732,7 +2754,7 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
         const size_t nwg = (size_t)np * c->NE * 2 * c->nphi;
         fp.nwg = (int)nwg;
         hipLaunchKernelGGL(k_fac_setup, dim3(c->NE, np, 2), dim3(NTS), setup_lds, c->stream, sp);
-        launch_phi_h(h, mix, (unsigned)((nwg + 7) / 8 * 8), c->stream, fp);
+        launch_phi_h(h, mix, c->tune, (unsigned)((nwg + 7) / 8 * 8), c->stream, fp);
         if (mix != MIX_FP64) {
           const int ncols = (int)(cols_pp * np);
           for (int q = 0; q < NVPROB; ++q) vp.prob[q].ncols = ncols;
@@ -2753,6 +2775,8 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
       hipGraph_t graph = nullptr;
       const hipError_t cap = hipStreamEndCapture(c->stream, &graph);
       if (cap != hipSuccess) return fail(c, PNAMP_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(cap));
+      size_t nodes = 0;
+      if (hipGraphGetNodes(graph, nullptr, &nodes) == hipSuccess) c->graph_nodes = (int)nodes;
       const hipError_t ins = hipGraphInstantiate(&c->fac_graph, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       if (ins != hipSuccess) {
@@ -3173,10 +3197,64 @@ extern "C" int pnamp_last_timing(pnamp_ctx *c, float *ms_points, float *ms_total
   return PNAMP_OK;
 }
 
-extern "C" int pnamp_graph_info(pnamp_ctx *c, int *captures, int *replays) {
+extern "C" int pnamp_autotune(pnamp_ctx *c, int npairs, const int32_t *q0, const int32_t *q1, int zval, int nval,
+                              int *choice, float *ms, int *ncand) {
+  if (!c || npairs <= 0 || !q0 || !q1) return PNAMP_E_ARG;
+  if (c->form != PNAMP_FORM_FACTORED) return fail(c, PNAMP_E_ARG, "autotune: the FP64 factored form only");
+  struct Cand {
+    int pf, rest, div;
+  };
+  std::vector<Cand> cands;
+  const int h = c->h;
+  if (h >= 8) {
+    for (int pf : {4, 5, 6})
+      for (int rest : {0, 4}) {
+        if (rest == 4 && h > 16) continue;
+        for (int div : {1, 2, 4}) cands.push_back({pf, rest, div});
+      }
+  } else {
+    cands.push_back({0, 0, 1});
+  }
+  if ((int)cands.size() > PNAMP_TUNE_MAX) return fail(c, PNAMP_E_SIZE, "autotune: too many candidates");
+  const FacTune keep = c->tune;
+  float best = 0.0f;
+  int ib = -1;
+  for (size_t i = 0; i < cands.size(); ++i) {
+    c->tune.pf_waves = cands[i].pf;
+    c->tune.rest_waves = cands[i].rest;
+    c->tune.chunk_div = cands[i].div;
+    drop_graph(c);
+    int r = kernels_impl(c, npairs, q0, q1, zval, nval, nullptr, nullptr, nullptr);  // captures
+    if (!r) r = kernels_impl(c, npairs, q0, q1, zval, nval, nullptr, nullptr, nullptr);  // replays
+    if (r) {
+      c->tune = keep;
+      drop_graph(c);
+      return r;
+    }
+    if (ms) ms[i] = c->ms_points;
+    if (ib < 0 || c->ms_points < best) {
+      best = c->ms_points;
+      ib = (int)i;
+    }
+  }
+  c->tune.pf_waves = cands[ib].pf;
+  c->tune.rest_waves = cands[ib].rest;
+  c->tune.chunk_div = cands[ib].div;
+  drop_graph(c);
+  if (choice) {
+    choice[0] = cands[ib].pf ? cands[ib].pf : 5;
+    choice[1] = cands[ib].rest ? cands[ib].rest : (3 * h <= 64 ? 3 : 2);  // the default
+    choice[2] = cands[ib].div;
+  }
+  if (ncand) *ncand = (int)cands.size();
+  return PNAMP_OK;
+}
+
+extern "C" int pnamp_graph_info(pnamp_ctx *c, int *captures, int *replays, int *nodes) {
   if (!c) return PNAMP_E_ARG;
   if (captures) *captures = c->graph_captures;
   if (replays) *replays = c->graph_replays;
+  if (nodes) *nodes = c->graph_nodes;
   return PNAMP_OK;
 }
 

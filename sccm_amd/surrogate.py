@@ -17,7 +17,15 @@ random subset of all mesh points of the GCM, disjoint seeded train and test
 sets.  The surrogate kernels replace the exact value at points where the
 predicted |N| >= a threshold and keep the exact FP64 value elsewhere; the
 kernels are then projected on the GPU (pnamp_project_zres) and solved
-(Hill-Wheeler) as the exact ones.
+(Hill-Wheeler) as the exact ones, and the levels compared at PreSolve_HWG's
+natural-state numbers of the exact kernels.
+
+``screen_bound`` measures what any screen or surrogate could gain at all on
+a GCM, from the exact ZRES alone: the |dE| of the levels when the Euler
+points whose number-projected norm |ZRES_0| lies below tau times the pair's
+largest are dropped (an ideal screen, which knows the exact values), and when
+every Euler point's observables carry a relative error eps (a surrogate of
+that accuracy).
 """
 from __future__ import annotations
 
@@ -221,13 +229,56 @@ def assemble_zres(samples_exact: np.ndarray, zres_exact: np.ndarray, Npred: np.n
     return z, float(use.mean())
 
 
+def _levels_dE(ctx, w, kern, lv) -> float:
+    """Largest |dE| (MeV) of ``kern``'s levels against ``lv`` (gcm_levels of
+    the exact kernels) at the same natural-state numbers."""
+    from .driver import levels_at_nos
+    e = levels_at_nos(ctx, kern, w.pairs, len(w.U), w.mesh.jmax, lv, w.hw_eps)
+    d = 0.0
+    for J, l in lv.items():
+        for x, y in zip(e[J], l["e"]):
+            if x is not None and y is not None:
+                d = max(d, abs(x - y))
+    return d
+
+
+def screen_bound(w, ctx, zres_exact, lv, taus=(1e-6, 1e-5, 1e-4, 1e-3), epss=(1e-4, 1e-3, 1e-2),
+                 seed: int = 1, log=None) -> dict:
+    """Ideal screen: |dE| (keV) when the Euler points with |ZRES_0| < tau x
+    (the pair's largest |ZRES_0|) are set to zero, and the fraction dropped;
+    surrogate accuracy: |dE| when each Euler point's six observables are
+    scaled by (1 + eps g), g standard normal per point."""
+    a = np.abs(zres_exact[..., 0])
+    rel = a / np.maximum(a.max(1, keepdims=True), 1e-300)
+    out = {"ideal_screen": [], "per_point_relative_error": []}
+    for tau in taus:
+        z = zres_exact.copy()
+        m = rel < tau
+        z[m] = 0.0
+        out["ideal_screen"].append({"tau": tau, "fraction_dropped": float(m.mean()),
+                                    "max_abs_dE_keV": 1e3 * _levels_dE(ctx, w, ctx.project_zres(z), lv)})
+        if log:
+            b = out["ideal_screen"][-1]
+            log(f"surrogate: ideal screen tau {tau:g}: {100 * b['fraction_dropped']:.3f} % of Euler points "
+                f"dropped, max |dE| {b['max_abs_dE_keV']:.4g} keV")
+    rng = np.random.default_rng(seed)
+    for eps in epss:
+        z = zres_exact * (1.0 + eps * rng.normal(size=zres_exact.shape[:2]))[..., None]
+        out["per_point_relative_error"].append({"eps": eps,
+                                                "max_abs_dE_keV": 1e3 * _levels_dE(ctx, w, ctx.project_zres(z), lv)})
+        if log:
+            log(f"surrogate: per-point relative error {eps:g}: max |dE| "
+                f"{out['per_point_relative_error'][-1]['max_abs_dE_keV']:.4g} keV")
+    return out
+
+
 def run(w, ctx, n_train: int = 200_000, n_test: int = 50_000, width: int = 128, hidden: int = 3,
         epochs: int = 60, batch: int = 4096, lr: float = 2e-3, decay: float = 0.96, threshold: float = 1e-3,
         seed: int = 3001, log=print) -> dict:
     """Train the surrogate on exact samples of the workload w (ctx: a PNAMP
     context with w's states and a factored form selected) and compare the
     Hill-Wheeler energies of surrogate+fallback kernels with the exact ones."""
-    from .driver import gcm_energies
+    from .driver import gcm_levels
 
     t0 = time.perf_counter()
     zres_exact, kern_exact = ctx.kernels_batched(w.pairs, w.zval, w.nval, want_zres=True)
@@ -268,23 +319,18 @@ def run(w, ctx, n_train: int = 200_000, n_test: int = 50_000, width: int = 128, 
     P = sy.inv(net.predict(sx.fwd(Xf), bf16=True)).reshape(T.shape)
     t_inf_ms = net.last_ms
     Np, Ep = targets_to_values(P)
-    ea = gcm_energies(ctx, kern_exact, w.pairs, len(w.U), w.mesh.jmax, w.hw_eps, nlevels=3)
+    lv = gcm_levels(ctx, kern_exact, w.pairs, len(w.U), w.mesh.jmax, w.hw_eps, kmax=3)
     res.update({"threshold_abs_N": threshold, "inference_ms_all_points_bf16": t_inf_ms,
-                "exact_factored_ms_all_points": t_exact_ms})
+                "exact_factored_ms_all_points": t_exact_ms,
+                "levels": "lowest 3 per J at PreSolve_HWG's natural-state numbers of the exact kernels"})
     for policy, thr in (("replace", threshold), ("screen", threshold), ("screen_1e-2", 1e-2)):
         zs, frac = assemble_zres(samples, zres_exact, Np, Ep, thr, policy.split("_")[0])
-        kern_s = ctx.project_zres(zs)
-        eb = gcm_energies(ctx, kern_s, w.pairs, len(w.U), w.mesh.jmax, w.hw_eps, nlevels=3)
-        dE = {}
-        for J in ea:
-            k = min(len(ea[J]), len(eb[J]))
-            if k:
-                dE[str(J)] = [float(x) for x in 1e3 * np.abs(ea[J][:k] - eb[J][:k])]
+        d = _levels_dE(ctx, w, ctx.project_zres(zs), lv)
         res[policy] = {"threshold_abs_N": thr, "fraction_of_mesh_points_from_surrogate": frac,
-                       "abs_dE_keV_per_level": dE,
-                       "max_abs_dE_keV": max(max(v) for v in dE.values()) if dE else None}
+                       "max_abs_dE_keV": 1e3 * d}
         log(f"surrogate: policy {policy} threshold {thr}: {100 * frac:.1f} % of points from the network, "
-            f"max |dE| {res[policy]['max_abs_dE_keV']:.4g} keV")
+            f"max |dE| {1e3 * d:.4g} keV")
+    res["bound"] = screen_bound(w, ctx, zres_exact, lv, log=log)
     net.close()
     return res
 

@@ -337,13 +337,34 @@ def test_factored_graph_survives_set_states():
     g.kernels_batched(w.pairs, w.zval, w.nval)
     g.set_states(U2, V2)
     k2 = g.kernels_batched(w.pairs, w.zval, w.nval)
-    assert g.graph_info() == (1, 2)
+    assert g.graph_info()[:2] == (1, 2)
     f = _gpu(w.basis, w.inter, w.mesh)
     f.set_states(U2, V2)
     f.set_form(FORM_FACTORED)
     assert np.array_equal(f.kernels_batched(w.pairs, w.zval, w.nval), k2)
     g.close()
     f.close()
+
+
+def test_autotune_keeps_the_kernels():
+    """The autotuner times every launch candidate on real pairs and keeps the
+    fastest; the candidates differ only in occupancy and chunking, so the
+    kernels after tuning equal those before."""
+    from sccm_amd.pnamp import FORM_FACTORED
+    w = configs.pf_gcm(nstates=4, nalp=2, nbet=3, nphi=5, jmax=4)
+    g = _gpu(w.basis, w.inter, w.mesh)
+    g.set_states(w.U, w.V)
+    g.set_form(FORM_FACTORED)
+    before = g.kernels_batched(w.pairs, w.zval, w.nval)
+    t = g.autotune(w.pairs, w.zval, w.nval)
+    assert len(t["candidates"]) == 9  # h = 20: Pfaffian waves 4, 5, 6 x chunk divisors 1, 2, 4
+    assert all(c["ms"] > 0 for c in t["candidates"])
+    assert t["pf_waves"] in (4, 5, 6) and t["chunk_div"] in (1, 2, 4)
+    best = min(t["candidates"], key=lambda c: c["ms"])
+    assert (best["pf_waves"], best["chunk_div"]) == (t["pf_waves"], t["chunk_div"])
+    after = g.kernels_batched(w.pairs, w.zval, w.nval)
+    assert np.abs(after - before).max() <= 1e-14 * np.abs(before).max()
+    g.close()
 
 
 # Mixed forms: the V contraction on MFMA in FP32 or split BF16.  Observables

@@ -84,11 +84,12 @@ def test_mlp_training_fits_a_smooth_function():
 
 @pytest.mark.gpu
 def test_surrogate_gain_on_small_pf_gcm():
-    """The network trained on exact mesh samples predicts log|N| on held-out
-    points about three times better than the best constant (the method
-    without it); the Hill-Wheeler energies of the surrogate + FP64-fallback
-    kernels are reported for both policies (their |dE| is far above 1 keV on
-    this GCM, which is why the chosen form evaluates every point exactly)."""
+    """The network trained on exact mesh samples (MFMA kernels) predicts
+    log|N| on held-out points about three times better than the best constant
+    (the method without it); the levels of the surrogate + FP64-fallback
+    kernels are reported for both policies, and the bound on any screen
+    (dropping Euler points by their exact norm, or a given relative error per
+    point) that decides whether a surrogate can meet 1 keV on a GCM."""
     from sccm_amd import configs, surrogate
     from sccm_amd.pnamp import FORM_FACTORED, PNAMP
     w = configs.pf_gcm(nstates=5, nalp=4, nbet=6, nphi=5, jmax=4)
@@ -105,4 +106,20 @@ def test_surrogate_gain_on_small_pf_gcm():
     for policy in ("replace", "screen"):
         assert 0.0 <= r[policy]["fraction_of_mesh_points_from_surrogate"] <= 1.0
         assert r[policy]["max_abs_dE_keV"] is not None
+    # the bound on any screen: dropping nothing changes nothing, dropping more
+    # Euler points (larger tau) drops a larger fraction
+    b = r["bound"]["ideal_screen"]
+    fr = [x["fraction_dropped"] for x in b]
+    assert fr == sorted(fr)
+    z0 = surrogate.screen_bound(w, ctx, ctx.kernels_batched(w.pairs, w.zval, w.nval, want_zres=True)[0],
+                                surrogate_levels(w, ctx), taus=(0.0,), epss=(0.0,))
+    assert z0["ideal_screen"][0]["fraction_dropped"] == 0.0
+    assert z0["ideal_screen"][0]["max_abs_dE_keV"] < 1e-6
+    assert z0["per_point_relative_error"][0]["max_abs_dE_keV"] < 1e-6
     ctx.close()
+
+
+def surrogate_levels(w, ctx):
+    from sccm_amd.driver import gcm_levels
+    k = ctx.kernels_batched(w.pairs, w.zval, w.nval)
+    return gcm_levels(ctx, k, w.pairs, len(w.U), w.mesh.jmax, w.hw_eps, kmax=3)
