@@ -122,6 +122,7 @@ def cpu_baseline(w, threads: int):
     t = time.perf_counter()
     o.kernel_pair(w.U[q0], w.V[q0], w.U[q1], w.V[q1])
     dt_full = time.perf_counter() - t
+    log(f"bench: CPU oracle, full pair ({q0},{q1}): {dt_full:.1f} s")
     pts = m.n_points
     strided = list(range(10, len(w.pairs), 50))
     dt_s, pts_s = 0.0, 0
@@ -175,6 +176,33 @@ def gcm_compare(ctx, w, kern_a, kern_b, nlevels: int):
     return worst, report, worst_eps
 
 
+def mesh_check(w, nang: int, nlevels: int) -> dict:
+    """Levels of the same GCM on a refined alpha = gamma mesh (factored form),
+    each by PreSolve_HWG's least-variance NOS on its own kernels, next to the
+    levels of the benchmark mesh: shows which levels the 10-point alpha/gamma
+    integration has converged."""
+    from sccm_amd import configs
+    from sccm_amd.driver import gcm_levels
+    from sccm_amd.pnamp import FORM_FACTORED, PNAMP
+
+    out = {}
+    for tag, na in (("benchmark", w.mesh.alp.size), ("refined", nang)):
+        wr = configs.pf_gcm(nstates=len(w.U), nalp=na, nbet=w.mesh.bet.size, nphi=w.mesh.nphi, jmax=w.mesh.jmax)
+        c = PNAMP(wr.basis, wr.inter, wr.mesh)
+        c.set_states(wr.U, wr.V)
+        c.set_form(FORM_FACTORED)
+        k = c.kernels_batched(wr.pairs, wr.zval, wr.nval)
+        lv = gcm_levels(c, k, wr.pairs, len(wr.U), wr.mesh.jmax, wr.hw_eps, kmax=nlevels)
+        out[tag] = {"alpha_gamma_points": na,
+                    "per_J": {str(J): {"nos": l["nos"], "E_MeV": [round(x, 4) for x in l["e"]],
+                                       "most_negative_norm_eigenvalue_rel": l["en_min_rel"],
+                                       "E_all_above_cutoff_MeV": [round(x, 4) for x in l["e_eps"]]}
+                              for J, l in lv.items()}}
+        c.close()
+        log(f"bench: mesh check alpha = gamma {na} done")
+    return out
+
+
 def kernel_errors(kern_a, kern_b, jmax: int):
     """Per J: the largest |a - b| / max|b| of the norm (N^J) and energy (H^J)
     kernels over all pairs."""
@@ -201,8 +229,23 @@ def main():
     ap.add_argument("--levels", type=int, default=3, help="lowest HW levels per J compared")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="factored form: keep the default launch configuration")
+    ap.add_argument("--tune", default="", help="factored form: 'pf,rest,div' launch configuration instead of "
+                                               "autotuning (e.g. 6,3,2)")
+    ap.add_argument("--mesh-check", type=int, default=0,
+                    help="also solve the GCM on alpha = gamma = N points (same beta, gauge) and report its levels")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 512 MiB device-to-device copy (the reference kernel of the HBM counters)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain reference pass and the GCM check")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        x = torch.empty(512 << 20, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "device copy", "bytes": 512 << 20}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the pairs would shard across GPUs with no communication until the "
@@ -229,7 +272,11 @@ def main():
         return w.pairs[idx]
 
     tune = None
-    if form == FORM_FACTORED and not args.no_tune:
+    if form == FORM_FACTORED and args.tune:
+        pf_w, rest_w, div = (int(x) for x in args.tune.split(","))
+        ctx.set_tune(pf_w, rest_w, div)
+        tune = {"pf_waves": pf_w, "rest_waves": rest_w, "chunk_div": div, "set": "given, not timed"}
+    elif form == FORM_FACTORED and not args.no_tune:
         t = time.perf_counter()
         tune = ctx.autotune(batch(0), w.zval, w.nval)
         tune["seconds"] = time.perf_counter() - t
@@ -354,6 +401,8 @@ def main():
                       "per_J": report,
                       "max_abs_dE_keV_all_natural_states_above_cutoff": 1e3 * dE_eps,
                       "kernel_rel_err_per_J": kernel_errors(kern_chosen, kern_plain, w.mesh.jmax)}
+    if args.mesh_check and "gcm" in out:
+        out["mesh_check"] = mesh_check(w, args.mesh_check, args.levels)
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
         log(f"bench: CPU oracle sample on {threads} threads ({cpu_model()})")

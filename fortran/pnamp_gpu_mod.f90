@@ -15,6 +15,7 @@ module pnamp_gpu
   public :: pnamp_hw_solve, pnamp_last_timing, pnamp_last_error_f, pnamp_destroy
   public :: pnamp_set_form, pnamp_states_separable, pnamp_project_zres
   public :: pnamp_kernels_batched_noalloc, pnamp_set_q2, pnamp_norb, pnamp_kernels_e2, pnamp_hw_presolve
+  public :: pnamp_autotune, pnamp_set_tune, pnamp_graph_info
   public :: PNAMP_NOBS, PNAMP_FORM_PLAIN, PNAMP_FORM_FACTORED, PNAMP_FORM_MIXED_FP32, PNAMP_FORM_MIXED_BF16
 
   integer(c_int), parameter :: PNAMP_NOBS = 6
@@ -150,6 +151,30 @@ module pnamp_gpu
       real(c_double), intent(out) :: e(*), var(*)
       integer(c_int), intent(out) :: nos_best(*)
       real(c_double), intent(out) :: e_best(*)
+    end function
+
+    integer(c_int) function pnamp_autotune(ctx, npairs, q0, q1, zval, nval, choice, cand, ms, ncand) &
+        bind(C, name="pnamp_autotune")
+      import :: c_int, c_int32_t, c_ptr, c_float
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npairs
+      integer(c_int32_t), intent(in) :: q0(*), q1(*)
+      integer(c_int), value :: zval, nval
+      integer(c_int), intent(out) :: choice(3), cand(*)
+      real(c_float), intent(out) :: ms(*)
+      integer(c_int), intent(out) :: ncand
+    end function
+
+    integer(c_int) function pnamp_set_tune(ctx, pf_waves, rest_waves, chunk_div) bind(C, name="pnamp_set_tune")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: pf_waves, rest_waves, chunk_div
+    end function
+
+    integer(c_int) function pnamp_graph_info(ctx, captures, replays, nodes) bind(C, name="pnamp_graph_info")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: captures, replays, nodes
     end function
 
     integer(c_int) function pnamp_last_timing(ctx, ms_points, ms_total) bind(C, name="pnamp_last_timing")

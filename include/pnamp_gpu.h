@@ -245,13 +245,20 @@ int pnamp_last_timing(pnamp_ctx *ctx, float *ms_points, float *ms_total);
  * (species size h >= 8; smaller spaces have one): waves per SIMD of the
  * Pfaffian kernel {4, 5, 6}, of the density kernel {default (3 while
  * 3 h <= 64, else 2), 4 (h <= 16 only)}, and the pair chunk divided by
- * {1, 2, 4}.  choice[3] receives
- * (Pfaffian waves, density waves, chunk divisor), ms[i] the kernel time of
- * candidate i (at most PNAMP_TUNE_MAX), *ncand their number; any may be
+ * {1, 2, 4}; a kernel variant that spills registers to scratch memory is no
+ * candidate.  choice[3] receives (Pfaffian waves, density waves, chunk
+ * divisor), cand[3 i .. 3 i + 2] the same for candidate i, ms[i] its kernel
+ * time (at most PNAMP_TUNE_MAX candidates), *ncand their number; any may be
  * NULL.  The form must be PNAMP_FORM_FACTORED. */
 #define PNAMP_TUNE_MAX 32
 int pnamp_autotune(pnamp_ctx *ctx, int npairs, const int32_t *q0, const int32_t *q1, int zval, int nval,
-                   int *choice, float *ms, int *ncand);
+                   int *choice, int *cand, float *ms, int *ncand);
+
+/* Sets the factored form's launch configuration directly (a choice of
+ * pnamp_autotune kept from an earlier run): Pfaffian waves 0 (default) / 4 /
+ * 5 / 6, density waves 0 (default) / 2 / 3 / 4 (4 only for h <= 16; other
+ * values give the default), chunk divisor 1 / 2 / 4. */
+int pnamp_set_tune(pnamp_ctx *ctx, int pf_waves, int rest_waves, int chunk_div);
 
 /* Launch-graph counters of the factored and mixed forms: graphs captured
  * and replays launched since pnamp_create, and the kernel nodes of the last

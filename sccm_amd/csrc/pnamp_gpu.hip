@@ -552,7 +552,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
   // ---- Pfaffian
   double2 pf = make_double2(1.0, 0.0);
   {
-    double2 sv[SL];
+    // When the last slot's entries all lie in rows 0 and 1 (step 0 alone
+    // reads them, and step 0 updates no entry of that slot), that slot is
+    // kept in LDS, in the lanes' dummy slots below, instead of registers:
+    // four VGPRs fewer, which keeps five waves per SIMD free of spills.
+    constexpr bool TAIL_LDS = SL >= 2 && (N2 - 2) * (N2 - 3) / 2 <= 64 * (SL - 1);
+    constexpr int SR = TAIL_LDS ? SL - 1 : SL;  // slots in registers
+    double2 sv[SR];
     // byte offsets 16 i | 16 j << 16 of each entry's row and column, in LDS
     // (as registers they would spill at five waves per SIMD); padding lanes
     // get i = j = 2H, which matches no row or column
@@ -581,9 +587,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
     };
 #pragma unroll
     for (int s = 0; s < SL; ++s) {
-      sv[s] = entry(s);
+      if (s < SR) sv[s] = entry(s);
+      else at(off_dummy) = entry(s);
       sijL[s * 64 + lane] = pf_index.v[s * 64 + lane];
     }
+    // slot s as read by the step-0 scans (the tail slot from LDS)
+    auto slot = [&](int s) -> double2 { return (s < SR) ? sv[s < SR ? s : 0] : at(off_dummy); };
     // Entries that left the trailing matrix may be overwritten: they are
     // never read again (the last one, S(2H-2, 2H-1), is entry 0).
     bool zero = false;
@@ -596,7 +605,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
       for (int s = 0; s < SL; ++s) {
         if (s * 64 < cnt && (s + 1) * 64 > lo) {
           const unsigned ij = sijL[s * 64 + lane];
-          const double2 v = sv[s];
+          const double2 v = slot(s);
           const unsigned pk = pivot_key<6>(v.x * v.x + v.y * v.y, (int)(ij >> 20));
           key = ((ij & 0xffffu) == r16) ? max(key, pk) : key;
         }
@@ -616,11 +625,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
       // (first write), one of column c (i > r) to colC (second write); the
       // other lanes write their own dummy slot.
       const int loc = min(cnt2, (N2 - 1 - c) * (N2 - 2 - c) / 2);
+      // (highest slot first: the tail slot is read before any dummy write)
 #pragma unroll
-      for (int s = 0; s < SL; ++s) {
+      for (int s = SL - 1; s >= 0; --s) {
         if (s * 64 < cnt && (s + 1) * 64 > loc) {
           const unsigned ij = sijL[s * 64 + lane], i16 = ij & 0xffffu, j16 = ij >> 16;
-          const double2 v = sv[s];
+          const double2 v = slot(s);
           const bool rowR = i16 == r16, rowA = i16 == r16b, rowC = i16 == c16;
           const bool piv = rowR && swap && j16 == c16;
           unsigned o1 = off_dummy;
@@ -651,7 +661,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
       if (c >= r + 2) {
         const int lof = (N2 - 1 - c) * (N2 - 2 - c) / 2;
 #pragma unroll
-        for (int s = 0; s < SL; ++s) {
+        for (int s = 0; s < SR; ++s) {
           if (s * 64 < cnt2 && (s + 1) * 64 > lof) {
             const unsigned ij = sijL[s * 64 + lane], i16 = ij & 0xffffu, j16 = ij >> 16;
             const bool jc = j16 == c16, ic = i16 == c16;
@@ -661,7 +671,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
         }
       }
 #pragma unroll
-      for (int s = 0; s < SL; ++s) {
+      for (int s = 0; s < SR; ++s) {
         if (s % 2 == 0) __builtin_amdgcn_sched_barrier(0);  // bounds the LDS reads in flight
         if (s * 64 < cnt2) {
           const unsigned ij = sijL[s * 64 + lane], i16 = ij & 0xffffu, j16 = ij >> 16;
@@ -1366,6 +1376,37 @@ static void launch_phi(int mix, const FacTune &t, unsigned grid, hipStream_t s, 
   else if (H >= 8 && H <= 16 && t.rest_waves == 4)
     hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_REST, 4>), g, b, 0, s, pp);
   else hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_REST>), g, b, 0, s, pp);
+}
+
+// Scratch (register spill) bytes per lane of the FP64 variant kernels a
+// tuning choice launches; the autotuner only takes spill-free variants.
+template <int H>
+static size_t phi_scratch(const FacTune &t) {
+  const void *pf = (const void *)k_fac_phi<H, MIX_FP64, PART_PF>;
+  if (H >= 8 && t.pf_waves == 4) pf = (const void *)k_fac_phi<H, MIX_FP64, PART_PF, 4>;
+  if (H >= 8 && t.pf_waves == 6) pf = (const void *)k_fac_phi<H, MIX_FP64, PART_PF, 6>;
+  const void *rest = (const void *)k_fac_phi<H, MIX_FP64, PART_REST>;
+  if (H >= 8 && H <= 16 && t.rest_waves == 4) rest = (const void *)k_fac_phi<H, MIX_FP64, PART_REST, 4>;
+  hipFuncAttributes a{}, b{};
+  if (hipFuncGetAttributes(&a, pf) != hipSuccess || hipFuncGetAttributes(&b, rest) != hipSuccess) return (size_t)-1;
+  return a.localSizeBytes + b.localSizeBytes;
+}
+
+static size_t phi_scratch_h(int h, const FacTune &t) {
+  switch (h) {
+    case 2: return phi_scratch<2>(t);
+    case 4: return phi_scratch<4>(t);
+    case 6: return phi_scratch<6>(t);
+    case 8: return phi_scratch<8>(t);
+    case 10: return phi_scratch<10>(t);
+    case 12: return phi_scratch<12>(t);
+    case 14: return phi_scratch<14>(t);
+    case 16: return phi_scratch<16>(t);
+    case 18: return phi_scratch<18>(t);
+    case 20: return phi_scratch<20>(t);
+    case 22: return phi_scratch<22>(t);
+    default: return (size_t)-1;
+  }
 }
 
 // Species sizes h = n / 2 with an instantiated k_fac_phi (every h is even:
@@ -3198,7 +3239,7 @@ extern "C" int pnamp_last_timing(pnamp_ctx *c, float *ms_points, float *ms_total
 }
 
 extern "C" int pnamp_autotune(pnamp_ctx *c, int npairs, const int32_t *q0, const int32_t *q1, int zval, int nval,
-                              int *choice, float *ms, int *ncand) {
+                              int *choice, int *cand, float *ms, int *ncand) {
   if (!c || npairs <= 0 || !q0 || !q1) return PNAMP_E_ARG;
   if (c->form != PNAMP_FORM_FACTORED) return fail(c, PNAMP_E_ARG, "autotune: the FP64 factored form only");
   struct Cand {
@@ -3210,11 +3251,14 @@ extern "C" int pnamp_autotune(pnamp_ctx *c, int npairs, const int32_t *q0, const
     for (int pf : {4, 5, 6})
       for (int rest : {0, 4}) {
         if (rest == 4 && h > 16) continue;
+        FacTune t;
+        t.pf_waves = pf;
+        t.rest_waves = rest;
+        if (phi_scratch_h(h, t) != 0) continue;  // a variant that spills registers is no candidate
         for (int div : {1, 2, 4}) cands.push_back({pf, rest, div});
       }
-  } else {
-    cands.push_back({0, 0, 1});
   }
+  if (cands.empty()) cands.push_back({0, 0, 1});
   if ((int)cands.size() > PNAMP_TUNE_MAX) return fail(c, PNAMP_E_SIZE, "autotune: too many candidates");
   const FacTune keep = c->tune;
   float best = 0.0f;
@@ -3232,6 +3276,11 @@ extern "C" int pnamp_autotune(pnamp_ctx *c, int npairs, const int32_t *q0, const
       return r;
     }
     if (ms) ms[i] = c->ms_points;
+    if (cand) {
+      cand[3 * i] = cands[i].pf ? cands[i].pf : 5;
+      cand[3 * i + 1] = cands[i].rest ? cands[i].rest : (3 * h <= 64 ? 3 : 2);
+      cand[3 * i + 2] = cands[i].div;
+    }
     if (ib < 0 || c->ms_points < best) {
       best = c->ms_points;
       ib = (int)i;
@@ -3247,6 +3296,19 @@ extern "C" int pnamp_autotune(pnamp_ctx *c, int npairs, const int32_t *q0, const
     choice[2] = cands[ib].div;
   }
   if (ncand) *ncand = (int)cands.size();
+  return PNAMP_OK;
+}
+
+extern "C" int pnamp_set_tune(pnamp_ctx *c, int pf_waves, int rest_waves, int chunk_div) {
+  if (!c) return PNAMP_E_ARG;
+  if ((pf_waves != 0 && pf_waves != 4 && pf_waves != 5 && pf_waves != 6) ||
+      (rest_waves != 0 && rest_waves != 2 && rest_waves != 3 && rest_waves != 4) ||
+      (chunk_div != 1 && chunk_div != 2 && chunk_div != 4))
+    return fail(c, PNAMP_E_ARG, "pnamp_set_tune: waves 4, 5, 6 (Pfaffian), 2, 3, 4 (densities), chunk divisor 1, 2, 4");
+  c->tune.pf_waves = (pf_waves == 5) ? 0 : pf_waves;                // 5 is the default
+  c->tune.rest_waves = (rest_waves == 4 && c->h <= 16) ? 4 : 0;     // 2 and 3 name the default
+  c->tune.chunk_div = chunk_div;
+  drop_graph(c);
   return PNAMP_OK;
 }
 

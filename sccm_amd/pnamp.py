@@ -94,9 +94,11 @@ def load_library(path: str = LIB_PATH):
     L.pnamp_last_timing.restype = C.c_int
     L.pnamp_graph_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.pnamp_graph_info.restype = C.c_int
-    L.pnamp_autotune.argtypes = [vp, C.c_int, _i32p, _i32p, C.c_int, C.c_int, _i32p, C.POINTER(C.c_float),
+    L.pnamp_autotune.argtypes = [vp, C.c_int, _i32p, _i32p, C.c_int, C.c_int, _i32p, _i32p, C.POINTER(C.c_float),
                                  C.POINTER(C.c_int)]
     L.pnamp_autotune.restype = C.c_int
+    L.pnamp_set_tune.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.pnamp_set_tune.restype = C.c_int
     L.pnamp_n_euler.argtypes = [vp]
     L.pnamp_n_euler.restype = C.c_int
     L.pnamp_last_error.argtypes = [vp]
@@ -286,24 +288,25 @@ class PNAMP:
 
     def autotune(self, pairs, zval: int, nval: int) -> dict:
         """Times the FP64 factored form's launch candidates on ``pairs`` and
-        keeps the fastest: {"pf_waves", "rest_waves", "chunk_div", "ms"
-        (per candidate), "candidates"}."""
+        keeps the fastest: {"pf_waves", "rest_waves", "chunk_div",
+        "candidates": [{"pf_waves", "rest_waves", "chunk_div", "ms"}]}."""
         pairs = np.asarray(pairs, dtype=np.int32).reshape(-1, 2)
         q0 = np.ascontiguousarray(pairs[:, 0])
         q1 = np.ascontiguousarray(pairs[:, 1])
         ch = np.zeros(3, dtype=np.int32)
+        cand = np.zeros(3 * 32, dtype=np.int32)
         ms = np.zeros(32, dtype=np.float32)
         nc = C.c_int()
         self._check(self._L.pnamp_autotune(self._h, len(q0), q0.ctypes.data_as(_i32p), q1.ctypes.data_as(_i32p),
-                                           zval, nval, ch.ctypes.data_as(_i32p),
+                                           zval, nval, ch.ctypes.data_as(_i32p), cand.ctypes.data_as(_i32p),
                                            ms.ctypes.data_as(C.POINTER(C.c_float)), C.byref(nc)))
-        h = self.basis.n // 2
-        dflt = 3 if 3 * h <= 64 else 2
-        cands = [(pf, rest or dflt, div) for pf in (4, 5, 6) for rest in (0, 4) if not (rest == 4 and h > 16)
-                 for div in (1, 2, 4)] if h >= 8 else [(5, dflt, 1)]
         return {"pf_waves": int(ch[0]), "rest_waves": int(ch[1]), "chunk_div": int(ch[2]),
-                "candidates": [{"pf_waves": a, "rest_waves": b, "chunk_div": d, "ms": float(ms[i])}
-                               for i, (a, b, d) in enumerate(cands[:nc.value])]}
+                "candidates": [{"pf_waves": int(cand[3 * i]), "rest_waves": int(cand[3 * i + 1]),
+                                "chunk_div": int(cand[3 * i + 2]), "ms": float(ms[i])} for i in range(nc.value)]}
+
+    def set_tune(self, pf_waves: int, rest_waves: int, chunk_div: int):
+        """Launch configuration of the factored form (a kept autotune choice)."""
+        self._check(self._L.pnamp_set_tune(self._h, pf_waves, rest_waves, chunk_div))
 
     def graph_info(self) -> tuple[int, int, int]:
         """(graphs captured, replays launched, kernel nodes of the last graph)
