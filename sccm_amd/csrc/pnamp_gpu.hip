@@ -1244,100 +1244,251 @@ struct VGemmParams {
   double2 *out;                     // [npairs][NE][2][nphi][FAC_OUT]
 };
 
+// LDS bytes of k_vgemm<KP, MIX>: one k-step of A (all KP / 32 row blocks,
+// hi and for split BF16 lo), three buffers
+template <int KP, int MIX>
+constexpr size_t vgemm_lds() {
+  // three buffers of the k-step's fragments, rounded up to whole 256-thread passes
+  return (size_t)3 * (((MIX == MIX_BF16 ? 2 : 1) * (KP / 32) * 64 + 255) / 256 * 256) * 16;
+}
+
+// A (W or Q) is stored k-step major: fragment (ks, rb, lane) at
+// (ks * RB + rb) * 64 + lane, 16 bytes each (FP32: 4 k values for four
+// v_mfma_f32_32x32x2_f32; BF16: 8 k values for one v_mfma_f32_32x32x16_bf16),
+// so one k-step of every row block is one contiguous RB KB block.  The four
+// waves of a workgroup (four 32-column tiles) share it through LDS, in three
+// buffers with the global loads two k-steps ahead (one barrier per k-step).
 template <int KP, int MIX>
 __global__ __launch_bounds__(256) void k_vgemm(VGemmParams p) {
   constexpr int RB = KP / 32;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int NPART = (MIX == MIX_BF16) ? 2 : 1;  // hi (, lo)
+  constexpr int FR = RB * 64;                        // 16-byte fragments per part and k-step
+  constexpr int KS = (MIX == MIX_BF16) ? KP / 16 : KP / 8;
+  constexpr int PER = (NPART * FR + 255) / 256;      // fragments each thread stages per k-step
+  constexpr int BUF = PER * 256;                     // LDS buffer stride (the tail of the last pass is padding)
+  extern __shared__ uint4 sA[];                      // [3][BUF]: [hi | lo | padding]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const VProblem &P = p.prob[blockIdx.y];
   const int ct = blockIdx.x * 4 + wave;
-  if (ct * 32 >= P.ncols) return;
+  const bool active = ct * 32 < P.ncols;  // every wave stages and meets the barriers
   f32x16 acc[RB];
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[rb][q] = 0.0f;
-  if (MIX == MIX_FP32) {
-    constexpr int KQ = KP / 8;
-    const float4 *A = (const float4 *)P.ahi;
-    const float4 *B = (const float4 *)P.bhi + (size_t)ct * KQ * 64;
-    for (int kq = 0; kq < KQ; ++kq) {
-      const float4 b = B[kq * 64 + lane];
+  const uint4 *const Ahi = (const uint4 *)P.ahi, *const Alo = (const uint4 *)(NPART == 2 ? P.alo : P.ahi);
+  // this thread's fragments of a k-step: index tid + 256 i of [hi | lo]; past
+  // the end a thread reloads the last fragment and stores it into the padding
+  auto src = [&](int idx, size_t o) -> uint4 {
+    const int j = idx < NPART * FR ? idx : NPART * FR - 1;
+    return j < FR ? Ahi[o + j] : Alo[o + j - FR];
+  };
+  // this wave's B fragments (its 32 columns); B is streamed once:
+  // non-temporal loads keep the re-read A in L2.  A wave past the last tile
+  // (ncols >= 1) reads the last tile's fragments and discards them.
+  const size_t ctb = (size_t)min(ct, (P.ncols - 1) / 32);
+  const uint4 *const Bhi = (const uint4 *)P.bhi + ctb * KS * 64 + lane;
+  const uint4 *const Blo = (const uint4 *)(NPART == 2 ? P.blo : P.bhi) + ctb * KS * 64 + lane;
+  typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+  auto ntload = [](const uint4 *q) -> uint4 {
+    const u32x4 v = __builtin_nontemporal_load((const u32x4 *)q);
+    return make_uint4(v.x, v.y, v.z, v.w);
+  };
+  // Prefetch distance two: k-step ks + 1 is in registers (loaded during
+  // k-step ks - 1) while k-step ks is multiplied from LDS; iteration ks
+  // stores ks + 1 into LDS buffer (ks + 1) % 3 (last read at ks - 2) and
+  // loads ks + 2, so every global load has a whole k-step to arrive.
+  // Every k-step issues the same loads (indices past the last k-step are
+  // clamped to it; the repeat lands in a buffer no later k-step reads), so
+  // the waits count exactly: a k-step waits for the loads of the one before,
+  // never for its own.
+  uint4 pre[PER];
 #pragma unroll
-      for (int rb = 0; rb < RB; ++rb) {
-        const float4 a = A[((size_t)rb * KQ + kq) * 64 + lane];
-        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.x, acc[rb], 0, 0, 0);
-        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b.y, acc[rb], 0, 0, 0);
-        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, b.z, acc[rb], 0, 0, 0);
-        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, b.w, acc[rb], 0, 0, 0);
-      }
-    }
-  } else {
-    constexpr int KB = KP / 16;
-    const bf16x8 *Ah = (const bf16x8 *)P.ahi, *Al = (const bf16x8 *)P.alo;
-    const bf16x8 *Bh = (const bf16x8 *)P.bhi + (size_t)ct * KB * 64;
-    const bf16x8 *Bl = (const bf16x8 *)P.blo + (size_t)ct * KB * 64;
-    for (int kb = 0; kb < KB; ++kb) {
-      const bf16x8 bh = Bh[kb * 64 + lane], bl = Bl[kb * 64 + lane];
+  for (int i = 0; i < PER; ++i) pre[i] = src(tid + 256 * i, 0);
 #pragma unroll
-      for (int rb = 0; rb < RB; ++rb) {
-        const size_t ia = ((size_t)rb * KB + kb) * 64 + lane;
-        const bf16x8 ah = Ah[ia], al = Al[ia];
-        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[rb], 0, 0, 0);
-        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[rb], 0, 0, 0);
-        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[rb], 0, 0, 0);
-      }
-    }
-  }
-  // C(row, col) = acc[rb][q], row = 32 rb + 8 (q / 4) + 4 (lane / 32) + q % 4, col = 32 ct + lane % 32
-  const int hh = p.h * p.h;
-  const size_t col = (size_t)ct * 32 + (lane & 31);
-  const bool cv = col < (size_t)P.ncols;
-  const size_t m = col >> 1;  // phi column pair: m = (pair * NE + Euler) * nphi + phi
-  const int part = (int)(col & 1);
-  if (P.mode == 0) {
-    const size_t ob = ((m / p.nphi) * 2 + P.species) * p.nphi + (m % p.nphi);
-    double sr = 0.0, si = 0.0;
-    if (cv) {
-      const double2 *d = P.dot + ob * hh;
+  for (int i = 0; i < PER; ++i) sA[tid + 256 * i] = pre[i];
 #pragma unroll
-      for (int rb = 0; rb < RB; ++rb)
+  for (int i = 0; i < PER; ++i) pre[i] = src(tid + 256 * i, (size_t)min(1, KS - 1) * FR);
+  // B of a k-step is loaded one k-step ahead into one of two register sets;
+  // the loop is unrolled by two so that the sets alternate by name: no copy
+  // of a loaded register at the end of an iteration, whose wait would expose
+  // the whole HBM latency of the load in the iteration that issued it.
+  uint4 b0h = ntload(Bhi), b0l = (NPART == 2) ? ntload(Blo) : b0h, b1h = b0h, b1l = b0l;
+  __syncthreads();
+  auto kstep = [&](const int ks, const uint4 bch, const uint4 bcl, uint4 &bnh, uint4 &bnl) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          const int row = 32 * rb + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-          if (row < hh) {
-            const double g = (double)acc[rb][q];
-            const double2 dv = d[row];
-            if (part == 0) {
-              sr += dv.x * g;
-              si += dv.y * g;
-            } else {  // i G_im
-              sr -= dv.y * g;
-              si += dv.x * g;
-            }
-          }
+    for (int i = 0; i < PER; ++i) sA[(size_t)((ks + 1) % 3) * BUF + tid + 256 * i] = pre[i];
+    const int k1 = min(ks + 1, KS - 1), k2 = min(ks + 2, KS - 1);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) pre[i] = src(tid + 256 * i, (size_t)k2 * FR);
+    bnh = ntload(Bhi + (size_t)k1 * 64);
+    if (NPART == 2) bnl = ntload(Blo + (size_t)k1 * 64);
+    const uint4 *sh = sA + (size_t)(ks % 3) * BUF;
+    if (active) {
+      // all of the k-step's A fragments are read from LDS first (one wave per
+      // SIMD: the reads' latency overlaps only with each other), then the
+      // MFMAs run back to back
+      if (MIX == MIX_FP32) {
+        const float4 b = make_float4(__uint_as_float(bch.x), __uint_as_float(bch.y), __uint_as_float(bch.z),
+                                     __uint_as_float(bch.w));
+        uint4 u[RB];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) u[rb] = sh[rb * 64 + lane];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+          acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(u[rb].x), b.x, acc[rb], 0, 0, 0);
+          acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(u[rb].y), b.y, acc[rb], 0, 0, 0);
+          acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(u[rb].z), b.z, acc[rb], 0, 0, 0);
+          acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(u[rb].w), b.w, acc[rb], 0, 0, 0);
         }
-    }
-    sr += __shfl_xor(sr, 32);
-    si += __shfl_xor(si, 32);
-    sr += __shfl_xor(sr, 1);
-    si += __shfl_xor(si, 1);
-    if (cv && (lane & 33) == 0) p.out[ob * FAC_OUT + P.slot] = make_double2(sr, si);
-  } else if (cv) {
-    float *g = P.g + m * (size_t)hh * 2 + part;
+      } else {
+        bf16x8 bh, bl;
+        __builtin_memcpy(&bh, &bch, 16);
+        __builtin_memcpy(&bl, &bcl, 16);
+        uint4 uh[RB], ul[RB];
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
+        for (int rb = 0; rb < RB; ++rb) {
+          uh[rb] = sh[rb * 64 + lane];
+          ul[rb] = sh[FR + rb * 64 + lane];
+        }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int row = 32 * rb + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-        if (row < hh) g[(size_t)row * 2] = acc[rb][q];
+        for (int rb = 0; rb < RB; ++rb) {
+          bf16x8 ah, al;
+          __builtin_memcpy(&ah, &uh[rb], 16);
+          __builtin_memcpy(&al, &ul[rb], 16);
+          acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[rb], 0, 0, 0);
+          acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[rb], 0, 0, 0);
+          acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[rb], 0, 0, 0);
+        }
       }
+    }
+    __syncthreads();
+  };
+  int ks = 0;
+  for (; ks + 1 < KS; ks += 2) {
+    kstep(ks, b0h, b0l, b1h, b1l);
+    kstep(ks + 1, b1h, b1l, b0h, b0l);
+  }
+  if (ks < KS) kstep(ks, b0h, b0l, b1h, b1l);
+  if (!active) return;
+  // C(row, col) = acc[rb][q], row = 32 rb + 8 (q / 4) + 4 (lane / 32) + q % 4,
+  // col = 32 ct + lane % 32.  Column pair m = col >> 1 (Re, Im of one phi:
+  // m = (pair * NE + Euler) * nphi + phi).  Each 32 x 32 block of C goes
+  // through this wave's LDS tile T[col][row] (the A buffers are free now),
+  // so that the FP64 vectors of the dot epilogue are read, and G is written,
+  // in contiguous runs of 32 rows per column pair.
+  const int hh = p.h * p.h;
+  float *T = (float *)sA + (size_t)wave * 32 * 33;
+  // this lane's (pair, row) share of a block: pairs mp = lane / 32 + 2 i, row lane % 32
+  const int rl = lane & 31;
+  if (P.mode == 0) {
+    double sr[8], si[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) sr[i] = si[i] = 0.0;
+    // the FP64 vector of each of this lane's column pairs (pairs past ncols
+    // read the last pair's, with weight 0): no load sits in a branch, so the
+    // loads of a block are issued together instead of one wait per load
+    const double2 *dcol[8];
+    double wcol[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const size_t m0 = (size_t)ct * 16 + (lane >> 5) + 2 * i, mlast = (size_t)(P.ncols / 2 - 1);
+      const size_t m = m0 < mlast ? m0 : mlast;
+      dcol[i] = P.dot + (((m / p.nphi) * 2 + P.species) * p.nphi + (m % p.nphi)) * hh;
+      wcol[i] = (2 * m0 < (size_t)P.ncols) ? 1.0 : 0.0;
+    }
+    // rows past h^2 (the K padding of the last block) read row h^2 - 1 with
+    // weight 0; the vectors of block rb + DP are loaded while block rb is
+    // summed (the accumulators' registers are free after the k-loop)
+    constexpr int DP = RB < 4 ? RB : 4;
+    double2 dv[DP][8];
+    // (the scheduling barrier keeps the compiler from sinking the loads to
+    // their uses, which would wait on each load again)
+    auto dload = [&](double2 *d, int rb) {
+      const int row = 32 * rb + rl, rowc = row < hh ? row : hh - 1;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) d[i] = dcol[i][rowc];
+      __builtin_amdgcn_sched_barrier(0);
+    };
+#pragma unroll
+    for (int rb = 0; rb < DP; ++rb) dload(dv[rb], rb);
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) T[(lane & 31) * 33 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3)] = acc[rb][q];
+      const double wr = (32 * rb + rl < hh) ? 1.0 : 0.0;
+      const double2 *d = dv[rb % DP];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int mp = (lane >> 5) + 2 * i;
+        const double w = wr * wcol[i];
+        const double g0 = w * (double)T[(2 * mp) * 33 + rl], g1 = w * (double)T[(2 * mp + 1) * 33 + rl];
+        sr[i] += d[i].x * g0 - d[i].y * g1;  // d (G_re + i G_im)
+        si[i] += d[i].y * g0 + d[i].x * g1;
+      }
+      if (rb + DP < RB) dload(dv[rb % DP], rb + DP);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int off = 16; off > 0; off >>= 1) {
+        sr[i] += __shfl_xor(sr[i], off);
+        si[i] += __shfl_xor(si[i], off);
+      }
+    if (rl == 0)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const size_t m = (size_t)ct * 16 + (lane >> 5) + 2 * i;
+        if (2 * m < (size_t)P.ncols) {
+          const size_t ob = ((m / p.nphi) * 2 + P.species) * p.nphi + (m % p.nphi);
+          p.out[ob * FAC_OUT + P.slot] = make_double2(sr[i], si[i]);
+        }
+      }
+  } else {
+    float2 *g = (float2 *)P.g;  // G[m][row] = (Re, Im)
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) T[(lane & 31) * 33 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3)] = acc[rb][q];
+      const int row = 32 * rb + rl;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int mp = (lane >> 5) + 2 * i;
+        const size_t m = (size_t)ct * 16 + mp;
+        if (2 * m < (size_t)P.ncols && row < hh)
+          g[m * hh + row] = make_float2(T[(2 * mp) * 33 + rl], T[(2 * mp + 1) * 33 + rl]);
+      }
+    }
   }
 }
 
 template <int KP>
 static void launch_vgemm_kp(int mix, dim3 grid, hipStream_t s, const VGemmParams &vp) {
-  if (mix == MIX_FP32) hipLaunchKernelGGL((k_vgemm<KP, MIX_FP32>), grid, dim3(256), 0, s, vp);
-  else hipLaunchKernelGGL((k_vgemm<KP, MIX_BF16>), grid, dim3(256), 0, s, vp);
+  constexpr size_t l32 = vgemm_lds<KP, MIX_FP32>(), l16 = vgemm_lds<KP, MIX_BF16>();
+  if (mix == MIX_FP32) hipLaunchKernelGGL((k_vgemm<KP, MIX_FP32>), grid, dim3(256), l32, s, vp);
+  else hipLaunchKernelGGL((k_vgemm<KP, MIX_BF16>), grid, dim3(256), l16, s, vp);
+}
+
+// Dynamic LDS limits of both precisions (set once per device, before any capture).
+template <int KP>
+static hipError_t vgemm_attr_kp() {
+  hipError_t e = hipFuncSetAttribute((const void *)k_vgemm<KP, MIX_FP32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)vgemm_lds<KP, MIX_FP32>());
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void *)k_vgemm<KP, MIX_BF16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)vgemm_lds<KP, MIX_BF16>());
+  return e;
+}
+
+static hipError_t vgemm_attr() {
+  const hipError_t e[] = {vgemm_attr_kp<32>(), vgemm_attr_kp<64>(), vgemm_attr_kp<128>(), vgemm_attr_kp<160>(),
+                          vgemm_attr_kp<224>(), vgemm_attr_kp<256>(), vgemm_attr_kp<352>(), vgemm_attr_kp<416>(),
+                          vgemm_attr_kp<512>()};
+  for (hipError_t x : e)
+    if (x != hipSuccess) return x;
+  return hipSuccess;
 }
 
 // KP = h^2 rounded up to 32, for the instantiated species sizes
@@ -2017,7 +2168,7 @@ static int upload_vmat(pnamp_ctx *c, int idx, int mix) {
       for (int kq = 0; kq < kq_n; ++kq)
         for (int l = 0; l < 64; ++l)
           for (int e = 0; e < 4; ++e)
-            sw[(((size_t)rb * kq_n + kq) * 64 + l) * 4 + e] =
+            sw[(((size_t)kq * (kp / 32) + rb) * 64 + l) * 4 + e] =
                 (float)A[(size_t)(32 * rb + (l & 31)) * kp + (8 * kq + 2 * e + (l >> 5))];
     HIPCHK(c, hipMalloc(&c->d_va[idx], sw.size() * sizeof(float)));
     HIPCHK(c, hipMemcpy(c->d_va[idx], sw.data(), sw.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -2028,7 +2179,7 @@ static int upload_vmat(pnamp_ctx *c, int idx, int mix) {
       for (int kb = 0; kb < kb_n; ++kb)
         for (int l = 0; l < 64; ++l)
           for (int e = 0; e < 8; ++e) {
-            const size_t i = (((size_t)rb * kb_n + kb) * 64 + l) * 8 + e;
+            const size_t i = (((size_t)kb * (kp / 32) + rb) * 64 + l) * 8 + e;
             const double v = A[(size_t)(32 * rb + (l & 31)) * kp + (16 * kb + 8 * (l >> 5) + e)];
             hi[i] = bf16_rne((float)v);
             lo[i] = bf16_rne((float)(v - (double)bf16_float(hi[i])));
@@ -2119,6 +2270,7 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
   for (auto &e : c->ev) HIPCHK(c, hipEventCreate(&e));
   HIPCHK(c, plain_prepare(n, false));
   HIPCHK(c, plain_prepare(n, true));
+  HIPCHK(c, vgemm_attr());
   HIPCHK(c, hipFuncSetAttribute((const void *)k_state_prep, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)(sizeof(double) * (2 * (size_t)n * n + n))));
   double fact[40];
