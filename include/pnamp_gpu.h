@@ -243,7 +243,7 @@ int pnamp_last_timing(pnamp_ctx *ctx, float *ms_points, float *ms_total);
  * kernels on the context's states, each candidate captured and replayed
  * once) and keeps the fastest for later calls of this context.  Candidates
  * (species size h >= 8; smaller spaces have one): waves per SIMD of the
- * Pfaffian kernel {4, 5, 6}, of the density kernel {default (3 while
+ * Pfaffian kernel {3, 4, 5}, of the density kernel {default (3 while
  * 3 h <= 64, else 2), 4 (h <= 16 only)}, and the pair chunk divided by
  * {1, 2, 4}; a kernel variant that spills registers to scratch memory is no
  * candidate.  choice[3] receives (Pfaffian waves, density waves, chunk
@@ -255,8 +255,8 @@ int pnamp_autotune(pnamp_ctx *ctx, int npairs, const int32_t *q0, const int32_t 
                    int *choice, int *cand, float *ms, int *ncand);
 
 /* Sets the factored form's launch configuration directly (a choice of
- * pnamp_autotune kept from an earlier run): Pfaffian waves 0 (default) / 4 /
- * 5 / 6, density waves 0 (default) / 2 / 3 / 4 (4 only for h <= 16; other
+ * pnamp_autotune kept from an earlier run): Pfaffian waves 0 (default) / 3 /
+ * 4 / 5, density waves 0 (default) / 2 / 3 / 4 (4 only for h <= 16; other
  * values give the default), chunk divisor 1 / 2 / 4. */
 int pnamp_set_tune(pnamp_ctx *ctx, int pf_waves, int rest_waves, int chunk_div);
 

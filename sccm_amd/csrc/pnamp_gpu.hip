@@ -484,33 +484,19 @@ __device__ int wave_gauss_jordan(double2 (&col)[H], int *qmap) {
 // h = 20, so three waves per SIMD).
 constexpr int PART_PF = 1, PART_REST = 2;
 
-// Row / column byte offsets 16 i | 16 j << 16 of the packed strict upper
-// triangle of an n2 x n2 matrix, entries numbered from the bottom-right
-// corner (entry e is in row n2 - 1 - t with t(t-1)/2 <= e < t(t+1)/2);
-// padding entries get i = j = n2.  Built at compile time.
-template <int N2>
-struct PfIndex {
-  static constexpr int SIZE = ((N2 * (N2 - 1) / 2 + 63) / 64) * 64;
-  unsigned v[SIZE];
-  constexpr PfIndex() : v() {
-    for (int e = 0; e < SIZE; ++e) {
-      int i = N2, j = N2;
-      if (e < N2 * (N2 - 1) / 2) {
-        int t = 1;
-        while ((t + 1) * t / 2 <= e) ++t;
-        i = N2 - 1 - t;
-        j = i + 1 + (e - t * (t - 1) / 2);
-      }
-      v[e] = (unsigned)(16 * i) | ((unsigned)(16 * j) << 16);
-    }
-  }
-};
-
+// VGPRs of the Pfaffian part when it holds the step's column values in
+// registers: the T (T + 1) / 2 complex slots of the 2D layout (T = ceil(2h /
+// 8)), 2T held column values, the row pair and ~28 for the rest.
+constexpr int pf_hold_vgprs(int h) {
+  return 4 * (((2 * h + 7) / 8) * ((2 * h + 7) / 8 + 1) / 2) + 8 * ((2 * h + 7) / 8) + 8 + 20;
+}
 // Waves per SIMD the register budget is fitted to (the autotuner's knob;
-// these are the defaults): the Pfaffian part five, the rest three while one
+// these are the defaults): the Pfaffian part four where the held column
+// values fit 128 VGPRs (h <= 20), else three; the rest three while one
 // Gauss-Jordan pass fits the wavefront, else two.
+constexpr int pf_default_waves(int h) { return pf_hold_vgprs(h) <= 128 ? 4 : 3; }
 template <int H, int PART>
-constexpr int fac_default_waves() { return PART == PART_PF ? 5 : (3 * H <= 64 ? 3 : 2); }
+constexpr int fac_default_waves() { return PART == PART_PF ? pf_default_waves(H) : (3 * H <= 64 ? 3 : 2); }
 
 template <int H, int MIX, int PART, int WV = fac_default_waves<H, PART>()>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k_fac_phi(PhiParams p) {
@@ -550,143 +536,167 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
 
   if constexpr (PART == PART_PF) {
   // ---- Pfaffian
-  double2 pf = make_double2(1.0, 0.0);
+  // 2D block-cyclic layout: lane = 8 a + b holds S(a + 8 p, b + 8 q) for
+  // 0 <= p <= q < T in slot (p, q); the diagonal slots hold the whole 8 x 8
+  // block (both triangles), so every row and every column of the matrix is
+  // T slots of one lane octet, and a lane's row and column indices are its
+  // own constants plus multiples of 8 (LDS addresses with immediate offsets,
+  // no per-entry index tables).  Step k eliminates r = 2k, r + 1 (slot row
+  // P = k / 4, a compile-time constant of the unrolled block); the pivot
+  // column c (runtime) selects one slot column Q = c / 8 by a uniform switch.
+  // Entries that left the trailing matrix keep being updated with values
+  // nobody reads again (cheaper than masking them).
+  constexpr int T = (N2 + 7) / 8, NS = T * (T + 1) / 2;
+  // Holding the step's column values in registers halves the LDS reads of
+  // the update; at five or more waves per SIMD they are re-read per slot.
+  constexpr bool HOLD = pf_hold_vgprs(H) <= 512 / WV;
+  const int a = lane >> 3, b = lane & 7;
+  auto SI = [](int pp, int qq) { return pp * T - pp * (pp - 1) / 2 + (qq - pp); };
+  // LDS vectors of the step, 8T entries each: R = row r, A = row r + 1,
+  // C = column c up to row 8Q + 7 (then the scaled column of the update),
+  // D = row c from column 8Q + 8
+  __shared__ double2 vec[32 * T];
+  double2 *const R = vec, *const A = vec + 8 * T, *const C = vec + 16 * T, *const D = vec + 24 * T;
+  double2 sv[NS];
   {
-    // When the last slot's entries all lie in rows 0 and 1 (step 0 alone
-    // reads them, and step 0 updates no entry of that slot), that slot is
-    // kept in LDS, in the lanes' dummy slots below, instead of registers:
-    // four VGPRs fewer, which keeps five waves per SIMD free of spills.
-    constexpr bool TAIL_LDS = SL >= 2 && (N2 - 2) * (N2 - 3) / 2 <= 64 * (SL - 1);
-    constexpr int SR = TAIL_LDS ? SL - 1 : SL;  // slots in registers
-    double2 sv[SR];
-    // byte offsets 16 i | 16 j << 16 of each entry's row and column, in LDS
-    // (as registers they would spill at five waves per SIMD); padding lanes
-    // get i = j = 2H, which matches no row or column
-    __shared__ unsigned sijL[SL * 64];
-    // columns r (colR), r + 1 (colA) and c (colC) of the step, the pivot,
-    // and one slot per lane that takes the writes a lane does not make (the
-    // extraction writes unconditionally: no per-lane branches)
-    __shared__ double2 colbuf[3 * N2 + 1 + 64];
-    constexpr unsigned OFF_R = 0, OFF_A = 16 * N2, OFF_C0 = 32 * N2, OFF_P = 48 * N2;
-    const unsigned off_dummy = 16 * (3 * N2 + 1) + 16u * lane;
-    auto at = [&](unsigned off) -> double2 & { return *(double2 *)((char *)colbuf + off); };
-    static constexpr PfIndex<N2> pf_index{};
-    // S(i, j) of entry e = 64 s + lane
-    auto entry = [&](int s) -> double2 {
-      const int e = s * 64 + lane;
-      const unsigned ij = pf_index.v[e];
-      const int i = (int)((ij & 0xffffu) >> 4), j = (int)(ij >> 20);
-      double2 v = cz();
-      if (e < NU) {
-        if (j < H) v = cmul(e2, M0[i + j * H]);
-        else if (i < H) v = make_double2((j - H == i) ? -1.0 : 0.0, 0.0);
-        else v = make_double2(-Z0[(i - H) + (size_t)(j - H) * n], 0.0);
-        v = cscale(p.cscale, v);
+    // S(i, j) for the slot's (i, j): every load unconditional (indices
+    // clamped into the blocks), the block chosen by selects
+#pragma unroll
+    for (int pp = 0; pp < T; ++pp)
+#pragma unroll
+      for (int qq = pp; qq < T; ++qq) {
+        const int i = a + 8 * pp, j = b + 8 * qq, lo = min(i, j), hi = max(i, j);
+        const double2 m0v = M0[min(lo, H - 1) + min(hi, H - 1) * H];
+        const double z0v = Z0[(size_t)min(max(lo - H, 0), H - 1) + (size_t)min(max(hi - H, 0), H - 1) * n];
+        double2 up;  // S(lo, hi)
+        if (hi < H) up = cmul(e2, m0v);
+        else if (lo < H) up = make_double2((hi - H == lo) ? -1.0 : 0.0, 0.0);
+        else up = make_double2(-z0v, 0.0);
+        up = (hi < N2 && lo != hi) ? cscale(p.cscale, up) : cz();
+        sv[SI(pp, qq)] = (i < j) ? up : cneg(up);
       }
-      return v;
-    };
+  }
+  double2 pf = make_double2(1.0, 0.0);
+  bool zero = false;
 #pragma unroll
-    for (int s = 0; s < SL; ++s) {
-      if (s < SR) sv[s] = entry(s);
-      else at(off_dummy) = entry(s);
-      sijL[s * 64 + lane] = pf_index.v[s * 64 + lane];
-    }
-    // slot s as read by the step-0 scans (the tail slot from LDS)
-    auto slot = [&](int s) -> double2 { return (s < SR) ? sv[s < SR ? s : 0] : at(off_dummy); };
-    // Entries that left the trailing matrix may be overwritten: they are
-    // never read again (the last one, S(2H-2, 2H-1), is entry 0).
-    bool zero = false;
-    for (int k = 0; k < H - 1; ++k) {
-      const int r = 2 * k, m = N2 - r;
-      const int cnt = m * (m - 1) / 2, lo = (m - 1) * (m - 2) / 2, cnt2 = (m - 2) * (m - 3) / 2;
-      const unsigned r16 = 16u * r, r16b = r16 + 16u;
+  for (int P = 0; P < T; ++P) {
+    if (zero || 4 * P >= H - 1) break;
+    for (int t = 0; t < 4 && 4 * P + t < H - 1; ++t) {
+      const int r = 8 * P + 2 * t;
+      // pivot: the largest |S(r, j)|, j > r (row r is slot row P of lanes a = 2t)
       unsigned key = 0u;
+      if (a == 2 * t) {
 #pragma unroll
-      for (int s = 0; s < SL; ++s) {
-        if (s * 64 < cnt && (s + 1) * 64 > lo) {
-          const unsigned ij = sijL[s * 64 + lane];
-          const double2 v = slot(s);
-          const unsigned pk = pivot_key<6>(v.x * v.x + v.y * v.y, (int)(ij >> 20));
-          key = ((ij & 0xffffu) == r16) ? max(key, pk) : key;
+        for (int q = P; q < T; ++q) {
+          const int j = b + 8 * q;
+          const double2 v = sv[SI(P, q)];
+          const unsigned kk = pivot_key<6>(v.x * v.x + v.y * v.y, j);
+          key = (j > r && j < N2) ? max(key, kk) : key;
         }
       }
       key = wave_umax(key);
       const int c = 63 - (int)(key & 63u);
-      const unsigned c16 = 16u * c;
       const bool swap = c != r + 1;
-      // without the exchange (c = r + 1) column c is column r + 1
-      const unsigned OFF_C = swap ? OFF_C0 : OFF_A;
-      // columns of the step with the exchange c <-> r + 1 of the trailing
-      // indices folded in (only indices >= r + 2 are read later):
-      //   colR[x] = S(x, r), colR[c] <- S(r + 1, r), pivot S(r, c) aside;
-      //   colA[x] = S(x, r + 1);  colC[x] = S(x, c), colC[c] <- S(r + 1, c).
-      // Rows r, r + 1 are entries [cnt2, cnt), row and column c [C(2H-1-c, 2), cnt).
-      // An entry S(i, j) (i < j) of row r, r + 1 or c goes to one column
-      // (first write), one of column c (i > r) to colC (second write); the
-      // other lanes write their own dummy slot.
-      const int loc = min(cnt2, (N2 - 1 - c) * (N2 - 2 - c) / 2);
-      // (highest slot first: the tail slot is read before any dummy write)
+      const int Q = c >> 3, c8 = c & 7;
+      // rows r and r + 1 -> R, A; column c -> C (rows below 8Q + 8), row c -> D
+      if ((a >> 1) == t) {
+        double2 *const dst = ((a & 1) ? A : R) + b;
 #pragma unroll
-      for (int s = SL - 1; s >= 0; --s) {
-        if (s * 64 < cnt && (s + 1) * 64 > loc) {
-          const unsigned ij = sijL[s * 64 + lane], i16 = ij & 0xffffu, j16 = ij >> 16;
-          const double2 v = slot(s);
-          const bool rowR = i16 == r16, rowA = i16 == r16b, rowC = i16 == c16;
-          const bool piv = rowR && swap && j16 == c16;
-          unsigned o1 = off_dummy;
-          o1 = rowC ? OFF_C + j16 : o1;
-          o1 = rowA ? OFF_A + j16 : o1;
-          o1 = rowR ? OFF_R + ((swap && j16 == r16b) ? c16 : j16) : o1;
-          o1 = piv ? OFF_P : o1;
-          at(o1) = piv ? v : cneg(v);
-          const bool colc = j16 == c16 && i16 > r16;
-          at(colc ? OFF_C + (rowA ? c16 : i16) : off_dummy) = v;
-        }
+        for (int q = P; q < T; ++q) dst[8 * q] = sv[SI(P, q)];
       }
-      __syncthreads();
-      const double2 spiv = swap ? at(OFF_P) : cneg(at(OFF_R + c16));  // S(r, c)
-      if (!(spiv.x * spiv.x + spiv.y * spiv.y > 1.0e-26)) {  // |pivot| <= 1e-13
-        zero = true;
-        break;
-      }
-      const double2 inv = crecip1(spiv);
-      pf = cmul(pf, swap ? cneg(spiv) : spiv);
-      // colC / S(r, c) once per step (the update then needs no product with 1 / pivot)
-      if (lane >= r + 2 && lane < N2) at(OFF_C + 16u * lane) = cmul(at(OFF_C + 16u * lane), inv);
-      __syncthreads();
-      // the entries of row and column c take those of r + 1 (S'(x, c) =
-      // S(x, r + 1) = colA[x]), then the Schur update of the trailing entries
-      // i, j >= r + 2: S' = S + (S'(i,r+1) S'(j,r) - S'(i,r) S'(j,r+1)) / S(r, c),
-      // with colC already divided by S(r, c)
-      if (c >= r + 2) {
-        const int lof = (N2 - 1 - c) * (N2 - 2 - c) / 2;
 #pragma unroll
-        for (int s = 0; s < SR; ++s) {
-          if (s * 64 < cnt2 && (s + 1) * 64 > lof) {
-            const unsigned ij = sijL[s * 64 + lane], i16 = ij & 0xffffu, j16 = ij >> 16;
-            const bool jc = j16 == c16, ic = i16 == c16;
-            const double2 x = at(OFF_A + (jc ? i16 : j16));  // in colbuf also for padding lanes
-            sv[s] = jc ? x : (ic ? cneg(x) : sv[s]);
+      for (int qq = P; qq < T; ++qq) {
+        if (qq == Q) {
+          if (b == c8) {
+#pragma unroll
+            for (int pp = P; pp <= qq; ++pp) C[a + 8 * pp] = sv[SI(pp, qq)];
+          }
+          if (a == c8) {
+#pragma unroll
+            for (int q = qq + 1; q < T; ++q) D[b + 8 * q] = sv[SI(qq, q)];
           }
         }
       }
+      __syncthreads();
+      // pivot S(r, c), wave-uniform
+      const double2 sl = R[c];
+      const double2 s = make_double2(
+          __longlong_as_double(((long long)__builtin_amdgcn_readfirstlane((int)(__double_as_longlong(sl.x) >> 32)) << 32) |
+                               (unsigned)__builtin_amdgcn_readfirstlane((int)__double_as_longlong(sl.x))),
+          __longlong_as_double(((long long)__builtin_amdgcn_readfirstlane((int)(__double_as_longlong(sl.y) >> 32)) << 32) |
+                               (unsigned)__builtin_amdgcn_readfirstlane((int)__double_as_longlong(sl.y))));
+      if (!(s.x * s.x + s.y * s.y > 1.0e-26)) {  // |pivot| <= 1e-13
+        zero = true;
+        break;
+      }
+      const double2 inv = crecip1(s);
+      pf = cmul(pf, swap ? cneg(s) : s);
+      // The exchange c <-> r + 1 of the trailing indices: position c takes
+      // index r + 1, so S'(x, c) = S(x, r + 1), S'(c, y) = S(r + 1, y) (= -A, A)
+      if (swap) {
 #pragma unroll
-      for (int s = 0; s < SR; ++s) {
-        if (s % 2 == 0) __builtin_amdgcn_sched_barrier(0);  // bounds the LDS reads in flight
-        if (s * 64 < cnt2) {
-          const unsigned ij = sijL[s * 64 + lane], i16 = ij & 0xffffu, j16 = ij >> 16;
-          cfma(sv[s], at(OFF_C + i16), at(OFF_R + j16));
-          cfms(sv[s], at(OFF_R + i16), at(OFF_C + j16));
+        for (int qq = P; qq < T; ++qq) {
+          if (qq == Q) {
+            if (b == c8) {
+#pragma unroll
+              for (int pp = P; pp <= qq; ++pp) sv[SI(pp, qq)] = cneg(A[a + 8 * pp]);
+            }
+            if (a == c8) {
+#pragma unroll
+              for (int q = qq; q < T; ++q) sv[SI(qq, q)] = A[b + 8 * q];
+            }
+          }
+        }
+      }
+      // Schur update with the columns of the exchanged matrix,
+      //   S'(i, j) = S(i, j) + S(i, r+1) S(j, r) / s - S(i, r) S(j, r+1) / s,
+      // S(x, r) = -R[x] (R[c] <- R[r + 1]), S(x, r+1) / s -> C[x] (C[c] <- S(r + 1, c) / s)
+      if (lane >= r + 2 && lane < N2) {
+        const bool xc = swap && lane == c;
+        const bool low = lane < 8 * Q + 8;
+        const double2 v = xc ? C[r + 1] : (low ? C[lane] : D[lane]);
+        C[lane] = cmul(v, (xc || low) ? inv : cneg(inv));
+        if (xc) R[c] = R[r + 1];
+      }
+      __syncthreads();
+      if constexpr (HOLD) {
+        double2 Rj[T], Cj[T];
+#pragma unroll
+        for (int q = P; q < T; ++q) {
+          Rj[q] = R[b + 8 * q];
+          Cj[q] = C[b + 8 * q];
+        }
+#pragma unroll
+        for (int pp = P; pp < T; ++pp) {
+          const double2 Ri = R[a + 8 * pp], Ci = C[a + 8 * pp];
+#pragma unroll
+          for (int q = pp; q < T; ++q) {
+            cfms(sv[SI(pp, q)], Ci, Rj[q]);  // S += C_i (-R_j) - (-R_i) C_j
+            cfma(sv[SI(pp, q)], Ri, Cj[q]);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int pp = P; pp < T; ++pp) {
+          const double2 Ri = R[a + 8 * pp], Ci = C[a + 8 * pp];
+#pragma unroll
+          for (int q = pp; q < T; ++q) {
+            const double2 Rj = R[b + 8 * q], Cj = C[b + 8 * q];
+            cfms(sv[SI(pp, q)], Ci, Rj);
+            cfma(sv[SI(pp, q)], Ri, Cj);
+          }
         }
       }
       __syncthreads();
     }
-    if (zero) {
-      pf = cz();
-    } else {
-      const double2 last = make_double2(__shfl(sv[0].x, 0), __shfl(sv[0].y, 0));  // S(2H-2, 2H-1)
-      pf = cmul(pf, last);
-    }
+  }
+  if (zero) {
+    pf = cz();
+  } else {
+    // the last 2 x 2 block: S(N2 - 2, N2 - 1), one lane of slot (T - 1, T - 1)
+    constexpr int LL = ((N2 - 2) & 7) * 8 + ((N2 - 1) & 7);
+    const double2 v = sv[SI((N2 - 2) >> 3, (N2 - 1) >> 3)];
+    pf = cmul(pf, make_double2(__shfl(v.x, LL), __shfl(v.y, LL)));
   }
 
   if (lane == 0) p.out[obase * FAC_OUT] = pf;
@@ -1510,8 +1520,8 @@ static bool launch_vgemm(int h, int mix, dim3 grid, hipStream_t s, const VGemmPa
 
 // Occupancy variants of the FP64 factored form that the autotuner times
 // (species sizes h >= 8; smaller spaces take the defaults): Pfaffian part at
-// four, five or six waves per SIMD, the rest at its default or, for h <= 16,
-// four (a tighter register cap).  0 = the default.
+// three, four or five waves per SIMD, the rest at its default or, for
+// h <= 16, four (a tighter register cap).  0 = the default.
 struct FacTune {
   int pf_waves = 0, rest_waves = 0, chunk_div = 1;
 };
@@ -1519,8 +1529,9 @@ struct FacTune {
 template <int H>
 static void launch_phi(int mix, const FacTune &t, unsigned grid, hipStream_t s, const PhiParams &pp) {
   const dim3 g(grid), b(64);
-  if (H >= 8 && mix == MIX_FP64 && t.pf_waves == 4) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF, 4>), g, b, 0, s, pp);
-  else if (H >= 8 && mix == MIX_FP64 && t.pf_waves == 6) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF, 6>), g, b, 0, s, pp);
+  if (H >= 8 && mix == MIX_FP64 && t.pf_waves == 3) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF, 3>), g, b, 0, s, pp);
+  else if (H >= 8 && mix == MIX_FP64 && t.pf_waves == 4) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF, 4>), g, b, 0, s, pp);
+  else if (H >= 8 && mix == MIX_FP64 && t.pf_waves == 5) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF, 5>), g, b, 0, s, pp);
   else hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF>), g, b, 0, s, pp);
   if (mix == MIX_FP32) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP32, PART_REST>), g, b, 0, s, pp);
   else if (mix == MIX_BF16) hipLaunchKernelGGL((k_fac_phi<H, MIX_BF16, PART_REST>), g, b, 0, s, pp);
@@ -1534,8 +1545,9 @@ static void launch_phi(int mix, const FacTune &t, unsigned grid, hipStream_t s, 
 template <int H>
 static size_t phi_scratch(const FacTune &t) {
   const void *pf = (const void *)k_fac_phi<H, MIX_FP64, PART_PF>;
+  if (H >= 8 && t.pf_waves == 3) pf = (const void *)k_fac_phi<H, MIX_FP64, PART_PF, 3>;
   if (H >= 8 && t.pf_waves == 4) pf = (const void *)k_fac_phi<H, MIX_FP64, PART_PF, 4>;
-  if (H >= 8 && t.pf_waves == 6) pf = (const void *)k_fac_phi<H, MIX_FP64, PART_PF, 6>;
+  if (H >= 8 && t.pf_waves == 5) pf = (const void *)k_fac_phi<H, MIX_FP64, PART_PF, 5>;
   const void *rest = (const void *)k_fac_phi<H, MIX_FP64, PART_REST>;
   if (H >= 8 && H <= 16 && t.rest_waves == 4) rest = (const void *)k_fac_phi<H, MIX_FP64, PART_REST, 4>;
   hipFuncAttributes a{}, b{};
@@ -3400,7 +3412,7 @@ extern "C" int pnamp_autotune(pnamp_ctx *c, int npairs, const int32_t *q0, const
   std::vector<Cand> cands;
   const int h = c->h;
   if (h >= 8) {
-    for (int pf : {4, 5, 6})
+    for (int pf : {3, 4, 5})
       for (int rest : {0, 4}) {
         if (rest == 4 && h > 16) continue;
         FacTune t;
@@ -3429,7 +3441,7 @@ extern "C" int pnamp_autotune(pnamp_ctx *c, int npairs, const int32_t *q0, const
     }
     if (ms) ms[i] = c->ms_points;
     if (cand) {
-      cand[3 * i] = cands[i].pf ? cands[i].pf : 5;
+      cand[3 * i] = cands[i].pf ? cands[i].pf : pf_default_waves(h);
       cand[3 * i + 1] = cands[i].rest ? cands[i].rest : (3 * h <= 64 ? 3 : 2);
       cand[3 * i + 2] = cands[i].div;
     }
@@ -3443,7 +3455,7 @@ extern "C" int pnamp_autotune(pnamp_ctx *c, int npairs, const int32_t *q0, const
   c->tune.chunk_div = cands[ib].div;
   drop_graph(c);
   if (choice) {
-    choice[0] = cands[ib].pf ? cands[ib].pf : 5;
+    choice[0] = cands[ib].pf ? cands[ib].pf : pf_default_waves(h);
     choice[1] = cands[ib].rest ? cands[ib].rest : (3 * h <= 64 ? 3 : 2);  // the default
     choice[2] = cands[ib].div;
   }
@@ -3453,11 +3465,11 @@ extern "C" int pnamp_autotune(pnamp_ctx *c, int npairs, const int32_t *q0, const
 
 extern "C" int pnamp_set_tune(pnamp_ctx *c, int pf_waves, int rest_waves, int chunk_div) {
   if (!c) return PNAMP_E_ARG;
-  if ((pf_waves != 0 && pf_waves != 4 && pf_waves != 5 && pf_waves != 6) ||
+  if ((pf_waves != 0 && pf_waves != 3 && pf_waves != 4 && pf_waves != 5) ||
       (rest_waves != 0 && rest_waves != 2 && rest_waves != 3 && rest_waves != 4) ||
       (chunk_div != 1 && chunk_div != 2 && chunk_div != 4))
-    return fail(c, PNAMP_E_ARG, "pnamp_set_tune: waves 4, 5, 6 (Pfaffian), 2, 3, 4 (densities), chunk divisor 1, 2, 4");
-  c->tune.pf_waves = (pf_waves == 5) ? 0 : pf_waves;                // 5 is the default
+    return fail(c, PNAMP_E_ARG, "pnamp_set_tune: waves 3, 4, 5 (Pfaffian), 2, 3, 4 (densities), chunk divisor 1, 2, 4");
+  c->tune.pf_waves = pf_waves;                                      // 0: the default for h
   c->tune.rest_waves = (rest_waves == 4 && c->h <= 16) ? 4 : 0;     // 2 and 3 name the default
   c->tune.chunk_div = chunk_div;
   drop_graph(c);
