@@ -951,13 +951,30 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
   const size_t base = ((size_t)ip * p.NE + iang) * 2;
   double2 *rn = dsm;
   int *ycol = (int *)(dsm + (size_t)NP * hh);
+  // rho_n (or G) into LDS: every load of a thread issued before the first
+  // store (indices clamped, no load in a branch), so the HBM latency is
+  // paid once per thread rather than once per element
   for (int e = tid; e < p.pn_ncol; e += NTC) ycol[e] = p.pn_ycol[e];
-  for (int e = tid; e < NP * hh; e += NTC) {
-    if (GMIX) {
-      const float2 gv = p.gpn[((size_t)ip * p.NE + iang) * NP * hh + e];
-      rn[e] = make_double2(gv.x, gv.y);
-    } else {
-      rn[e] = p.rho[(base + 1) * NP * hh + e];
+  {
+    constexpr int NB = 8;  // loads in flight per thread (the register cap is 128)
+    const int nt = NP * hh;
+    for (int e0 = 0; e0 < nt; e0 += NB * NTC) {
+      double2 tmp[NB];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int e = min(e0 + tid + NTC * u, nt - 1);
+        if (GMIX) {
+          const float2 gv = p.gpn[((size_t)ip * p.NE + iang) * NP * hh + e];
+          tmp[u] = make_double2(gv.x, gv.y);
+        } else {
+          tmp[u] = p.rho[(base + 1) * NP * hh + e];
+        }
+      }
+      // unconditional stores: a clamped element is element nt - 1, stored
+      // again with its own value (a store in a branch would pull its load
+      // into the branch and wait on it there)
+#pragma unroll
+      for (int u = 0; u < NB; ++u) rn[min(e0 + tid + NTC * u, nt - 1)] = tmp[u];
     }
   }
   __syncthreads();
@@ -981,23 +998,26 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
       const int *yc = ycol + bk[2];
       const double *wb = p.pn_w + bk[4] + (px - bk[0]);
       xrow[r] = rw.x;
-      int k = 0;
-      for (; k + 4 <= ncol; k += 4) {  // four block entries loaded ahead of their use
-        double v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = wb[(size_t)(k + u) * nrows];
+      // four block entries per group, the next group's loads issued before
+      // this group's products (indices clamped to the last entry, weight 0
+      // past it)
+      double v0 = wb[0], v1 = wb[(size_t)min(1, ncol - 1) * nrows], v2 = wb[(size_t)min(2, ncol - 1) * nrows],
+             v3 = wb[(size_t)min(3, ncol - 1) * nrows];
+      for (int k = 0; k < ncol; k += 4) {
+        const double c0 = v0, c1 = v1, c2 = v2, c3 = v3;
+        v0 = wb[(size_t)min(k + 4, ncol - 1) * nrows];
+        v1 = wb[(size_t)min(k + 5, ncol - 1) * nrows];
+        v2 = wb[(size_t)min(k + 6, ncol - 1) * nrows];
+        v3 = wb[(size_t)min(k + 7, ncol - 1) * nrows];
+        const double cv[4] = {c0, c1, c2, c3};
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const int y = yc[k + u];
+          const bool ok = k + u < ncol;
+          const int y = yc[ok ? k + u : 0];
+          const double vv = ok ? cv[u] : 0.0;
 #pragma unroll
-          for (int f = 0; f < NP; ++f) rfma(g[r][f], v[u], rn[(size_t)f * hh + y]);
+          for (int f = 0; f < NP; ++f) rfma(g[r][f], vv, rn[(size_t)f * hh + y]);
         }
-      }
-      for (; k < ncol; ++k) {
-        const double v = wb[(size_t)k * nrows];
-        const int y = yc[k];
-#pragma unroll
-        for (int f = 0; f < NP; ++f) rfma(g[r][f], v, rn[(size_t)f * hh + y]);
       }
     }
   }
@@ -1018,10 +1038,19 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
 #pragma unroll
     for (int f = 0; f < NP; ++f) acc[f] = cz();
     const double2 *rpf = p.rho + (base * NP + fp) * hh;
-    for (int x = gl; x < hh; x += 32) {
-      const double2 a = rpf[x];
+    // this lane's rho_p elements x = gl + 32 i, eight loads issued together
+    // (clamped; weight 0 past h^2)
+    for (int x0 = 0; x0 < hh; x0 += 8 * 32) {
+      double2 ra[8];
 #pragma unroll
-      for (int f = 0; f < NP; ++f) cfma(acc[f], a, rn[(size_t)f * hh + x]);
+      for (int i = 0; i < 8; ++i) ra[i] = rpf[min(x0 + gl + 32 * i, hh - 1)];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int x = x0 + gl + 32 * i, xc = min(x, hh - 1);
+        const double2 av = (x < hh) ? ra[i] : cz();
+#pragma unroll
+        for (int f = 0; f < NP; ++f) cfma(acc[f], av, rn[(size_t)f * hh + xc]);
+      }
     }
 #pragma unroll
     for (int f = 0; f < NP; ++f) {
