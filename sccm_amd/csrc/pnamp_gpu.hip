@@ -229,7 +229,8 @@ struct VEntry {
 //   k_fac_combine  one workgroup per (pair, Euler point): the p-n bilinear form
 //                  and the nphi^2 gauge sum.
 constexpr int NTS = 256;     // threads of the setup workgroup
-constexpr int NTC = 288;     // threads of the combine workgroup (9 groups of 32)
+constexpr int NTC = 512;     // threads of the combine workgroup: one G row per thread (h^2 <= 484),
+                              // so the longest row (pf: 60 entries) bounds the pass; eight waves
 constexpr int NPHI_MAX = 16;
 constexpr int FAC_OUT = 9;   // per (pair, Euler, species, phi): Pf, E2, Ekin, Tr rho, A, t_+, t_-, t_z, E2 (kappa part, mixed forms)
 constexpr int MIX_FP64 = 0, MIX_FP32 = 1, MIX_BF16 = 2;  // precision of the V contraction
@@ -941,10 +942,12 @@ struct CombParams {
 // ZRES.  NP = nphi is a template argument so the per-gauge register arrays
 // stay in registers.  With GMIX (mixed forms) G comes from k_vgemm instead.
 template <int NP, bool GMIX>
-__global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
+// (up to nine gauge angles four waves per SIMD: two eight-wave workgroups per
+// CU, as many as their LDS allows; more angles need more than 128 VGPRs)
+__global__ __launch_bounds__(NTC) __attribute__((amdgpu_waves_per_eu(NP <= 9 ? 4 : 2))) void k_fac_combine(CombParams p) {
   extern __shared__ double2 dsm[];  // rho_n -> G [NP][h^2], then the block column lists; rho_p from global memory
   __shared__ double2 epn[NP * NP];
-  __shared__ double red[NTC / 64 * 2 * NOBS];
+  __shared__ double red[(NTC + 63) / 64 * 2 * NOBS];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int iang = blockIdx.x, ip = blockIdx.y;
   const int h = p.h, hh = h * h;
@@ -979,10 +982,10 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
   }
   __syncthreads();
   if (!GMIX) {
-  // G rows in block order (row positions tid, tid + NTC; h^2 <= 2 NTC), all
-  // phi_n, in registers.  The lanes of one block walk the same column list,
-  // so each rho_n read is an LDS broadcast.
-  constexpr int RPT = 2;
+  // G rows in block order (row position tid + r NTC), all phi_n, in
+  // registers.  The lanes of one block walk the same column list, so each
+  // rho_n read is an LDS broadcast.
+  constexpr int RPT = (FAC_HMAX * FAC_HMAX + NTC - 1) / NTC;
   double2 g[RPT][NP];
   int xrow[RPT];
 #pragma unroll
@@ -1102,7 +1105,7 @@ __global__ __launch_bounds__(NTC) void k_fac_combine(CombParams p) {
     double2 *z = p.zres + ((size_t)ip * p.NE + iang) * NOBS;
     for (int k = 0; k < NOBS; ++k) {
       double re = 0, im = 0;
-      for (int w = 0; w < NTC / 64; ++w) {
+      for (int w = 0; w < (NTC + 63) / 64; ++w) {
         re += red[w * 2 * NOBS + 2 * k];
         im += red[w * 2 * NOBS + 2 * k + 1];
       }
@@ -2875,7 +2878,8 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
     if (c->nphi > NPHI_MAX) return fail(c, PNAMP_E_SIZE, "factored form supports nphi <= 16");
     const size_t comb_lds = sizeof(double2) * (size_t)c->nphi * hh + sizeof(int) * (size_t)c->pn_ncol;
     if (comb_lds > 150 * 1024) return fail(c, PNAMP_E_SIZE, "nphi h^2 complex exceeds the combine LDS");
-    if (hh > 2 * NTC) return fail(c, PNAMP_E_SIZE, "h^2 exceeds the combine rows per workgroup");
+    if (hh > ((FAC_HMAX * FAC_HMAX + NTC - 1) / NTC) * NTC)
+      return fail(c, PNAMP_E_SIZE, "h^2 exceeds the combine rows per workgroup");
     if (h > FAC_HMAX || (h & 1)) return fail(c, PNAMP_E_SIZE, "factored form: species size h must be even and <= 22");
     // per pair: rho [NE][2][nphi][h^2], setup blocks [NE][2][5][h^2], per-phi scalars and status;
     // mixed forms add the GEMM operands (4 x 2 NE nphi columns x kp, 4 bytes per element),
