@@ -321,12 +321,12 @@ struct PhiParams {
   const double *t1b;                // n x n
   const double *mtab;               // [3][h]: m, <m+1|J_+|m> (0 at m = j), j(j+1) per species-local state
   const double2 *jtab;              // [2][h^2][4]: coefficients of rho(a,b) in Ekin and the <J^2> pieces (see pnamp_create)
-  // species lists (p, n), one term per entry: LDS byte offsets of rho(k,i),
-  // rho(l,j), rho(l,i), rho(k,j), A01(i,j), A10(k,l)/4 as 16-bit pairs in
-  // x, y, z; the matrix element in lme; padded with zero terms to a multiple
-  // of 256
+  // species lists (p, n), one term per entry, nl entries padded with zero
+  // terms to a multiple of 256: entry q is lofs[q] = LDS byte offsets of
+  // rho(k,i), rho(l,j), rho(l,i), rho(k,j) and lofs[nl + q] = those of
+  // A01(i,j), A10(k,l)/4 and the matrix element (low, high word); full
+  // 32-bit offsets, so the loop unpacks nothing
   const uint4 *lofs[2];
-  const double *lme[2];
   int nl[2];
   const int *q0, *q1;
   double2 *out;                     // [npairs][NE][2][nphi][FAC_OUT]
@@ -858,26 +858,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
   } else {
     // ---- species list: V [rho_ki rho_lj - rho_li rho_kj + A01_ij A10_kl / 4]
     const uint4 *lo = p.lofs[sp];
-    const double *lm = p.lme[sp];
     const int nl = p.nl[sp];
     const char *lds = (const char *)sA;
     auto ld = [&](unsigned off) { return *(const double2 *)(lds + off); };
     for (int q0 = lane; q0 < nl; q0 += 4 * 64) {
-      uint4 o[4];  // four entries loaded ahead of their LDS reads
-      double me[4];
+      uint4 o[4], w[4];  // four entries loaded ahead of their LDS reads
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         o[u] = lo[q0 + 64 * u];
-        me[u] = lm[q0 + 64 * u];
+        w[u] = lo[nl + q0 + 64 * u];
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const double2 r1 = ld(o[u].x & 0xffffu), r2 = ld(o[u].x >> 16), r3 = ld(o[u].y & 0xffffu),
-                      r4 = ld(o[u].y >> 16), x = ld(o[u].z & 0xffffu), y = ld(o[u].z >> 16);
+        const double2 r1 = ld(o[u].x), r2 = ld(o[u].y), r3 = ld(o[u].z), r4 = ld(o[u].w), x = ld(w[u].x),
+                      y = ld(w[u].y);
+        const double me = __hiloint2double((int)w[u].w, (int)w[u].z);
         double2 t = cmul(r1, r2);
         cfms(t, r3, r4);
         cfma(t, x, y);
-        rfma(e2b, me[u], t);
+        rfma(e2b, me, t);
       }
     }
   }
@@ -2136,7 +2135,6 @@ struct pnamp_ctx {
   bool separable = false;
   int nme_s[3] = {0, 0, 0};
   uint4 *d_lofs[2] = {nullptr, nullptr};            // species lists of k_fac_phi (p, n)
-  double *d_lme[2] = {nullptr, nullptr};
   int nl[2] = {0, 0};
   bool fac_list_ok = false;
   std::string fac_reason;
@@ -2470,14 +2468,15 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
     // of an entry with a + b != c + d as an entry of its own, LDS byte
     // offsets precomputed (rho at ld h + 1, then the A-block at ld h)
     for (int k = 0; k < 2; ++k) {
-      std::vector<uint4> lo;
-      std::vector<double> lm;
+      std::vector<uint4> lo, lw;
       const unsigned hp = h + 1, msz = h * (h + 1);
       auto rb = [&](int row, int col) { return (unsigned)(row + col * hp) * 16u; };
       auto ab = [&](int row, int col) { return (msz + (unsigned)(row + col * h)) * 16u; };
       auto term = [&](int i, int j, int kk, int l, double me) {
-        lo.push_back(make_uint4(rb(kk, i) | (rb(l, j) << 16), rb(l, i) | (rb(kk, j) << 16), ab(i, j) | (ab(l, kk) << 16), 0u));
-        lm.push_back(me);
+        uint64_t mb;
+        std::memcpy(&mb, &me, 8);
+        lo.push_back(make_uint4(rb(kk, i), rb(l, j), rb(l, i), rb(kk, j)));
+        lw.push_back(make_uint4(ab(i, j), ab(l, kk), (unsigned)mb, (unsigned)(mb >> 32)));
       };
       for (const VEntry &e : ls[k]) {
         const int i = e.idx & 255, j = (e.idx >> 8) & 255, kk = (e.idx >> 16) & 255, l = e.idx >> 24;
@@ -2489,13 +2488,13 @@ extern "C" int pnamp_create(const pnamp_basis *basis, const pnamp_interaction *i
         term(i, j, kk, l, e.me);
         if (kk + l != i + j) term(kk, l, i, j, e.me);
       }
-      while (lo.size() % 256) {
+      while (lo.size() % 256) {  // zero terms (offset 0, matrix element 0)
         lo.push_back(make_uint4(0u, 0u, 0u, 0u));
-        lm.push_back(0.0);
+        lw.push_back(make_uint4(0u, 0u, 0u, 0u));
       }
       c->nl[k] = (int)lo.size();
-      if ((r = dupload(c, &c->d_lofs[k], lo.data(), lo.size())) || (r = dupload(c, &c->d_lme[k], lm.data(), lm.size())))
-        return bail(r, c->err);
+      lo.insert(lo.end(), lw.begin(), lw.end());
+      if ((r = dupload(c, &c->d_lofs[k], lo.data(), lo.size()))) return bail(r, c->err);
     }
     // W_pn by rows x = c + a h, columns y = d + b h (species-local indices)
     const int hh = h * h;
@@ -2934,7 +2933,7 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
     fp.UfT = c->d_UfT; fp.VfT = c->d_VfT; fp.Z = c->d_Z; fp.t1b = c->d_t1b; fp.mtab = c->d_mtab;
     fp.jtab = (const double2 *)c->d_jtab;
     for (int q = 0; q < 2; ++q) {
-      fp.lofs[q] = c->d_lofs[q]; fp.lme[q] = c->d_lme[q]; fp.nl[q] = c->nl[q];
+      fp.lofs[q] = c->d_lofs[q]; fp.nl[q] = c->nl[q];
     }
     fp.out = c->d_fout; fp.rho = c->d_frho; fp.status = c->d_fstatus;
     for (int q = 0; q < 2; ++q) {
@@ -3528,7 +3527,7 @@ extern "C" void pnamp_destroy(pnamp_ctx *c) {
                   c->d_pk_a, c->d_pk_b, c->d_blk_off, c->d_blk_sz, c->d_blk_base, c->d_twoj, c->d_twom,
                   c->d_plofs, c->d_plme, c->d_pwork, c->d_t1b, c->d_jtab, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status, c->d_q0,
                   c->d_q1, c->d_pz, c->d_pn, c->d_zres, c->d_kern, c->d_status, c->d_pair_status,
-                  c->d_lofs[0], c->d_lofs[1], c->d_lme[0], c->d_lme[1], c->d_Uf, c->d_Vf,
+                  c->d_lofs[0], c->d_lofs[1], c->d_Uf, c->d_Vf,
                   c->d_UfT, c->d_VfT, c->d_mtab, c->d_fout, c->d_frho,
                   c->d_sbuf, c->d_fstatus, c->d_pn_rows,
                   c->d_pn_blk, c->d_pn_ycol, c->d_pn_w};
