@@ -38,6 +38,21 @@ __device__ __forceinline__ double2 crecip1(double2 a) {  // one division
   const double r = 1.0 / (a.x * a.x + a.y * a.y);
   return make_double2(a.x * r, -a.y * r);
 }
+// 1 / d for a positive normal d: the hardware reciprocal estimate (v_rcp_f64)
+// refined by two Newton-Raphson steps r <- r + r (1 - d r), each of which
+// about doubles the number of correct bits (a few instructions instead of the
+// scaled division sequence; within an ulp or two of the rounded quotient)
+__device__ __forceinline__ double rcp_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ double2 crecip_nr(double2 a) {
+  const double r = rcp_nr(a.x * a.x + a.y * a.y);
+  return make_double2(a.x * r, -a.y * r);
+}
 // Read-only view in the constant address space: wave-uniform reads through it
 // become scalar loads (a generic pointer that stores in the kernel might
 // alias is read with per-lane vector loads)
@@ -98,6 +113,25 @@ template <int KB>
 __device__ __forceinline__ unsigned pivot_key(double mag, int idx) {
   const float f = (mag >= 0.0) ? (float)mag : 0.0f;
   return (__float_as_uint(f) & ~((1u << KB) - 1u)) | ((1u << KB) - 1u - (unsigned)idx);
+}
+
+// Pivot key from the high word of a non-negative double (sign 0, 11 exponent
+// and 20 mantissa bits: unsigned order is numeric order) with the low KB bits
+// replaced by (2^KB - 1 - idx): the largest magnitude to 2^-(20-KB) relative,
+// ties to the smallest index.  A NaN or infinite magnitude wins.
+template <int KB>
+__device__ __forceinline__ unsigned pivot_key_hi(double mag, int idx) {
+  return ((unsigned)((unsigned long long)__double_as_longlong(mag) >> 32) & ~((1u << KB) - 1u)) |
+         ((1u << KB) - 1u - (unsigned)idx);
+}
+
+// Maximum over each aligned group of eight lanes, on all eight (quad
+// exchanges, then the half-row mirror).
+__device__ __forceinline__ unsigned oct_umax(unsigned v) {
+  v = dpp_umax<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+  v = dpp_umax<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+  v = dpp_umax<0x141, 0xF>(v);  // row_half_mirror
+  return v;
 }
 
 // Both halves of a wavefront's value, lanes 0..31 (lo) and 32..63 (hi), on

@@ -433,7 +433,7 @@ __device__ int wave_gauss_jordan(double2 (&col)[H], int *qmap) {
     // tested here (uniform) rather than at the INFO store: a test sunk to
     // the use keeps every pivot alive in registers
     if (__builtin_amdgcn_readfirstlane((pv.x == 0.0 && pv.y == 0.0) ? 1 : 0) && info == 0) info = k + 1;
-    const double2 prow = cmul(col[k], crecip1(pv));
+    const double2 prow = cmul(col[k], crecip_nr(pv));
 #pragma unroll
     for (int r = 0; r < H; ++r)
       if (r != k) cfms(col[r], make_double2(rl(col[r].x), rl(col[r].y)), prow);
@@ -584,18 +584,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
     if (zero || 4 * P >= H - 1) break;
     for (int t = 0; t < 4 && 4 * P + t < H - 1; ++t) {
       const int r = 8 * P + 2 * t;
-      // pivot: the largest |S(r, j)|, j > r (row r is slot row P of lanes a = 2t)
+      // pivot: the largest |S(r, j)|, j > r (row r is slot row P of lanes
+      // a = 2t, the eight lanes 16t .. 16t + 7: reduced among themselves)
       unsigned key = 0u;
       if (a == 2 * t) {
 #pragma unroll
         for (int q = P; q < T; ++q) {
           const int j = b + 8 * q;
           const double2 v = sv[SI(P, q)];
-          const unsigned kk = pivot_key<6>(v.x * v.x + v.y * v.y, j);
+          const unsigned kk = pivot_key_hi<6>(v.x * v.x + v.y * v.y, j);
           key = (j > r && j < N2) ? max(key, kk) : key;
         }
       }
-      key = wave_umax(key);
+      key = (unsigned)__builtin_amdgcn_readlane((int)oct_umax(key), 16 * t);
       const int c = 63 - (int)(key & 63u);
       const bool swap = c != r + 1;
       const int Q = c >> 3, c8 = c & 7;
@@ -630,7 +631,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
         zero = true;
         break;
       }
-      const double2 inv = crecip1(s);
+      const double2 inv = crecip_nr(s);
       pf = cmul(pf, swap ? cneg(s) : s);
       // The exchange c <-> r + 1 of the trailing indices: position c takes
       // index r + 1, so S'(x, c) = S(x, r + 1), S'(c, y) = S(r + 1, y) (= -A, A)
