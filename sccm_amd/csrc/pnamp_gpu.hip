@@ -299,6 +299,8 @@ __global__ __launch_bounds__(NTS) void k_fac_setup(SetupParams p) {
   for (int e = tid; e < hh; e += NTS) {
     const int i = e % h, j = e / h;
     double2 x = cz(), y = cz();
+    // (unrolled by four: four terms' loads in flight)
+#pragma unroll 4
     for (int k = 0; k < h; ++k) {
       rfma(x, V0[k + i * h], RV1[k + j * h]);
       rfma(y, U0[k + i * h], RcU1[k + j * h]);
@@ -403,9 +405,9 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
 // Gauss-Jordan on an H-row system held one column per lane of the wavefront:
 // col[r] is row r of this lane's column, lanes 0..H-1 hold the coefficient
 // matrix M, the other lanes right-hand sides.  Step k pivots on row k and the
-// unused coefficient column q with the largest |M(k, q)| (|re| + |im|, ties to
-// the smallest q): for M = T^T this is the row pivoting of T by ZGETRF's
-// IZAMAX.  The row index is static, so nothing is picked out of registers by
+// unused coefficient column q with the largest |M(k, q)| (|re| + |im| to
+// 2^-15 relative, ties to the smallest q): for M = T^T this is the row
+// pivoting of T by ZGETRF's IZAMAX.  The row index is static, so nothing is picked out of registers by
 // a run-time index; column q reaches every lane by v_readlane from lane q (a
 // wave-uniform operand of the update; through LDS the 64-lane broadcast reads
 // made the step LDS-bound).  On return the unknown q_k = qmap[k] is row k of
@@ -419,7 +421,7 @@ __device__ int wave_gauss_jordan(double2 (&col)[H], int *qmap) {
 #pragma unroll
   for (int k = 0; k < H; ++k) {
     unsigned key = 0u;
-    if (lane < H && !((used >> lane) & 1u)) key = pivot_key<5>(cabs1(col[k]), lane);
+    if (lane < H && !((used >> lane) & 1u)) key = pivot_key_hi<5>(cabs1(col[k]), lane);
     key = wave_umax(key);
     const int q = 31 - (int)(key & 31u);
     used |= 1u << q;
