@@ -220,12 +220,14 @@ struct VEntry {
 // (species lists) + E_pn(phi_p, phi_n), a bilinear form in rho_p and rho_n
 // evaluated for all nphi^2 gauge pairs by k_fac_combine.
 //
-// Three launches per chunk of pairs:
+// Four launches per chunk of pairs:
 //   k_fac_setup    one workgroup per (pair, Euler point, species): the
 //                  gauge-independent X, Y, R V1s, R* U1s, M0 = R Z1s R^T -> HBM;
-//   k_fac_phi<h>   one wavefront per (pair, Euler point, species, phi): the
-//                  Pfaffian and Gauss-Jordan matrices live in registers (no
-//                  workgroup barriers), rho / kappa in LDS for the list;
+//   k_fac_phi<h>   one wavefront per (pair, Euler point, species, phi), twice
+//                  over the same grid: the Pfaffian (the skew matrix in a 2D
+//                  block-cyclic register layout), then Gauss-Jordan (one
+//                  column per lane) and the densities, rho / kappa in LDS for
+//                  the list; no workgroup barriers;
 //   k_fac_combine  one workgroup per (pair, Euler point): the p-n bilinear form
 //                  and the nphi^2 gauge sum.
 constexpr int NTS = 256;     // threads of the setup workgroup
