@@ -1859,7 +1859,7 @@ __global__ __launch_bounds__(256) void k_project_sep(ProjParams p) {
 struct JacobiLds {
   double cs[512], sn[512];
   int pp[512], qq[512];
-  double red[256];
+  double red[1024];  // one partial sum per thread (up to 1024 threads)
   int done;
 };
 
@@ -1920,27 +1920,36 @@ __device__ int block_jacobi(double *A, double *Vv, int dim, JacobiLds &L) {
         L.pp[k] = P; L.qq[k] = Q; L.cs[k] = c; L.sn[k] = s;
       }
       __syncthreads();
-      // columns: A = A J, V = V J
-      for (int e = tid; e < (m / 2) * dim; e += nt) {
+      // A = J^T A J: one thread per 2 x 2 block (rows of pair a, columns of
+      // pair b).  The column and the row rotation of a block touch only its
+      // four entries, so every block updates in place on its own (columns
+      // first, then rows, as two separate passes over the matrix would);
+      // pairs with the dummy slot have c = 1, s = 0 and no dummy entries.
+      const int hp = m / 2;
+      for (int e = tid; e < hp * hp; e += nt) {
+        const int a = e % hp, b = e / hp;
+        const int Pa = L.pp[a], Qa = L.qq[a], Pb = L.pp[b], Qb = L.qq[b];
+        const bool qa = Qa < dim, qb = Qb < dim;
+        const double ca = L.cs[a], sa = L.sn[a], cb = L.cs[b], sb = L.sn[b];
+        const double xpp = A[Pa + Pb * dim];
+        const double xpq = qb ? A[Pa + Qb * dim] : 0.0;
+        const double xqp = qa ? A[Qa + Pb * dim] : 0.0;
+        const double xqq = (qa && qb) ? A[Qa + Qb * dim] : 0.0;
+        const double ypp = cb * xpp - sb * xpq, ypq = sb * xpp + cb * xpq;  // columns: A J
+        const double yqp = cb * xqp - sb * xqq, yqq = sb * xqp + cb * xqq;
+        A[Pa + Pb * dim] = ca * ypp - sa * yqp;  // rows: J^T (A J)
+        if (qb) A[Pa + Qb * dim] = ca * ypq - sa * yqq;
+        if (qa) A[Qa + Pb * dim] = sa * ypp + ca * yqp;
+        if (qa && qb) A[Qa + Qb * dim] = sa * ypq + ca * yqq;
+      }
+      // V = V J (columns P, Q of every pair; consecutive threads, consecutive rows)
+      for (int e = tid; e < hp * dim; e += nt) {
         const int k = e / dim, r = e % dim, P = L.pp[k], Q = L.qq[k];
         if (Q >= dim) continue;
         const double c = L.cs[k], s = L.sn[k];
-        const double ap = A[r + P * dim], aq = A[r + Q * dim];
-        A[r + P * dim] = c * ap - s * aq;
-        A[r + Q * dim] = s * ap + c * aq;
         const double vp = Vv[r + P * dim], vq = Vv[r + Q * dim];
         Vv[r + P * dim] = c * vp - s * vq;
         Vv[r + Q * dim] = s * vp + c * vq;
-      }
-      __syncthreads();
-      // rows: A = J^T A
-      for (int e = tid; e < (m / 2) * dim; e += nt) {
-        const int k = e / dim, r = e % dim, P = L.pp[k], Q = L.qq[k];
-        if (Q >= dim) continue;
-        const double c = L.cs[k], s = L.sn[k];
-        const double ap = A[P + r * dim], aq = A[Q + r * dim];
-        A[P + r * dim] = c * ap - s * aq;
-        A[Q + r * dim] = s * ap + c * aq;
       }
       __syncthreads();
     }
@@ -1948,7 +1957,9 @@ __device__ int block_jacobi(double *A, double *Vv, int dim, JacobiLds &L) {
   return sweep;
 }
 
-__global__ __launch_bounds__(256) void k_jacobi(double *A, double *Vv, int dim, int *nsweeps) {
+// (one workgroup of 1024 threads: the matrix streams through L2 every step,
+// so the workgroup's memory parallelism bounds it)
+__global__ __launch_bounds__(1024) void k_jacobi(double *A, double *Vv, int dim, int *nsweeps) {
   __shared__ JacobiLds L;
   const int sw = block_jacobi(A, Vv, dim, L);
   if (threadIdx.x == 0) *nsweeps = sw;
@@ -3319,7 +3330,7 @@ static int hw_prepare(pnamp_ctx *c, HwBufs &B, int dim, const double *N, const d
   HIPCHK(c, hipMemcpyAsync(dA, negN.data(), sizeof(double) * dd, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(*dN, N, sizeof(double) * dd, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(*dH, H, sizeof(double) * dd, hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(256), 0, c->stream, dA, dV, dim, dsw);
+  hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(1024), 0, c->stream, dA, dV, dim, dsw);
   HIPCHK(c, hipGetLastError());
   std::vector<double> A(dd), Vv(dd);
   HIPCHK(c, hipMemcpyAsync(A.data(), dA, sizeof(double) * dd, hipMemcpyDeviceToHost, c->stream));
@@ -3365,7 +3376,7 @@ extern "C" int pnamp_hw_solve(pnamp_ctx *c, int dim, const double *N, const doub
   if (M <= 0) return 1;
   if (!B.alloc(&dV, (size_t)M * M) || !B.alloc(&dsw, 1) || (F && !B.alloc(&dF, (size_t)dim * M)))
     return fail(c, PNAMP_E_DEVICE, "hipMalloc failed in pnamp_hw_solve");
-  hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(256), 0, c->stream, dG, dV, M, dsw);
+  hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(1024), 0, c->stream, dG, dV, M, dsw);
   HIPCHK(c, hipGetLastError());
   std::vector<double> G((size_t)M * M), gv((size_t)M * M);
   HIPCHK(c, hipMemcpyAsync(G.data(), dG, sizeof(double) * G.size(), hipMemcpyDeviceToHost, c->stream));
