@@ -455,11 +455,10 @@ __device__ int wave_gauss_jordan(double2 (&col)[H], int *qmap) {
 // Schur update removes r, r + 1.  ZPfaffianF (Codes/PNAMP/PNAMP_Proj.f90:218-350)
 // pivots on the largest entry of the whole remaining matrix; the Pfaffian is
 // the same polynomial, the rounding differs.  Its zero rule is kept: a pivot
-// of modulus <= 1e-13 gives Pf = 0.  The strict upper triangle is held in
-// registers, entry e <-> lane e % 64, slot e / 64, with entries numbered from
-// the bottom-right corner (rows descending), so the trailing matrix of step k
-// is entries e < C(2H - 2k, 2): the slot loop shrinks with k.  The columns
-// r, r + 1, c of the step go through LDS.
+// of modulus <= 1e-13 gives Pf = 0.  The matrix is held in registers in a 2D
+// block-cyclic layout (lane 8a + b holds S(a + 8p, b + 8q), slot rows p <=
+// slot columns q); rows r, r + 1 and row / column c of the step go through
+// LDS, and the update covers the slot rows that still hold trailing entries.
 //
 // T = e^{i phi} X + e^{-i phi} Y, W1 = e^{i phi} R V1s T^-1 and
 // W2 = e^{-i phi} R* U1s T^-1 solve T^T [W1 W2]^T = [e^{i phi} RV1, e^{-i phi} RcU1]^T
@@ -553,7 +552,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
   // nobody reads again (cheaper than masking them).
   constexpr int T = (N2 + 7) / 8, NS = T * (T + 1) / 2;
   // Holding the step's column values in registers halves the LDS reads of
-  // the update; at five or more waves per SIMD they are re-read per slot.
+  // the update; where they do not fit the variant's register budget (waves
+  // per SIMD WV) they are re-read per slot.
   constexpr bool HOLD = pf_hold_vgprs(H) <= 512 / WV;
   const int a = lane >> 3, b = lane & 7;
   auto SI = [](int pp, int qq) { return pp * T - pp * (pp - 1) / 2 + (qq - pp); };
