@@ -511,8 +511,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
   // B0 = sA[0, MS): rho (ld h + 1; transposed reads avoid bank conflicts);
   // B1 = sA[MS, MS + h^2) (ld h): Gauss-Jordan unknown map (one pass),
   // kappa01, kappa10, then the A-block A01 (strict upper) and A10 / 4
-  // (strict lower triangle); 13 KB at h = 20, twelve waves per CU
-  __shared__ double2 sA[MS + HH];
+  // (strict lower triangle); 13 KB at h = 20, twelve waves per CU.  The
+  // closing reduction reuses sA for 16 quad sums of 14 values (112 double2),
+  // more than MS + h^2 below h = 8: LDS stores past the allocation are
+  // dropped, so sA is never smaller than that.
+  constexpr int RED2 = 16 * 14 / 2;
+  __shared__ double2 sA[(MS + HH > RED2) ? MS + HH : RED2];
   // in two passes kappa01 of pass 0 holds B1 through pass 1
   constexpr bool GJ_IN_SA = (PASSES == 1);
   __shared__ int qmap_sep[GJ_IN_SA ? 1 : H];
@@ -895,6 +899,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
     v[q] += dpp_f64<0x4E, 0xF>(v[q]);  // quad_perm [2,3,0,1]
   }
   double *red = (double *)sA;
+  static_assert(sizeof(sA) >= 16 * 14 * sizeof(double), "sA holds the closing reduction");
   __syncthreads();
   if ((lane & 3) == 0) {
 #pragma unroll
@@ -1291,11 +1296,13 @@ struct VGemmParams {
 };
 
 // LDS bytes of k_vgemm<KP, MIX>: one k-step of A (all KP / 32 row blocks,
-// hi and for split BF16 lo), three buffers
+// hi and for split BF16 lo), three buffers, rounded up to whole 256-thread
+// passes; at least the epilogue's four 32 x 33 FP32 tiles, which reuse them
+// (LDS stores past the allocation are dropped)
 template <int KP, int MIX>
 constexpr size_t vgemm_lds() {
-  // three buffers of the k-step's fragments, rounded up to whole 256-thread passes
-  return (size_t)3 * (((MIX == MIX_BF16 ? 2 : 1) * (KP / 32) * 64 + 255) / 256 * 256) * 16;
+  return std::max((size_t)3 * (((MIX == MIX_BF16 ? 2 : 1) * (KP / 32) * 64 + 255) / 256 * 256) * 16,
+                  (size_t)4 * 32 * 33 * sizeof(float));
 }
 
 // A (W or Q) is stored k-step major: fragment (ks, rb, lane) at

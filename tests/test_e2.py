@@ -147,8 +147,8 @@ def _rel(a, b):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("space,zn,form", [("sd", (2, 2), 0), ("sd", (2, 2), 1), ("pf", (2, 4), 0), ("pf", (2, 4), 1),
-                                           ("f5pg9", (4, 2), 1)])
+@pytest.mark.parametrize("space,zn,form", [("p", (2, 2), 1), ("sd", (2, 2), 0), ("sd", (2, 2), 1), ("pf", (2, 4), 0),
+                                           ("pf", (2, 4), 1), ("f5pg9", (4, 2), 1)])
 def test_e2_kernels_on_gpu_match_oracle(oracle_mod, space, zn, form):
     from sccm_amd.mesh import make_mesh
     from sccm_amd.pnamp import PNAMP

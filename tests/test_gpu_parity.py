@@ -106,7 +106,8 @@ def test_ge76_goldens_on_gpu(oracle_mod):
     g.close()
 
 
-def test_exact_projection_on_gpu():
+@pytest.mark.parametrize("form", [0, 1])
+def test_exact_projection_on_gpu(form):
     from fock import Fock
     b = make_basis("p")
     inter = synthetic_interaction(b, 1.0, 11, 12)
@@ -119,6 +120,7 @@ def test_exact_projection_on_gpu():
     m = Mesh(alp, w, np.arccos(x), wb, alp.copy(), w.copy(), 7, jmax, 0, 0)
     g = _gpu(b, inter, m)
     g.set_states(np.array([U0, U1]), np.array([V0, V1]))
+    g.set_form(form)
     kern = g.kernel_pair(0, 1, 2, 2)
     F = Fock(b)
     v0, v1 = F.hfb_state(U0, V0), F.hfb_state(U1, V1)
@@ -166,6 +168,8 @@ def test_singular_state_is_reported():
 
 
 @pytest.mark.parametrize("space,zn,mesh_args", [
+    ("p32", (2, 2), (3, 4, 3, 3, 4)),
+    ("p", (2, 2), (3, 4, 3, 5, 4)),
     ("sd", (2, 2), (3, 4, 3, 5, 6)),
     ("pf", (2, 4), (2, 3, 3, 9, 8)),
     ("f5pg9", (4, 2), (2, 2, 2, 3, 4)),
@@ -377,6 +381,8 @@ MIX_TOL = {2: 2e-5, 3: 5e-4}
 
 @pytest.mark.parametrize("form", [2, 3])
 @pytest.mark.parametrize("space,zn,mesh_args", [
+    ("p32", (2, 2), (3, 4, 3, 3, 4)),
+    ("p", (2, 2), (3, 4, 3, 5, 4)),
     ("sd", (2, 2), (3, 4, 3, 5, 6)),
     ("pf", (2, 4), (2, 3, 3, 9, 8)),
     ("f5pg9", (4, 2), (2, 2, 2, 3, 4)),
