@@ -199,6 +199,31 @@ def test_factored_form_matches_plain_and_oracle(oracle_mod, space, zn, mesh_args
     g.close()
 
 
+@pytest.mark.parametrize("space,zn,nphi", [("sd", (2, 2), 2), ("sd", (2, 2), 4), ("sd", (2, 2), 12),
+                                            ("sd", (2, 2), 16), ("f5pg9", (4, 2), 16)])
+def test_factored_form_at_every_gauge_count_class(space, zn, nphi):
+    """The combine kernel is instantiated per gauge count (1..16; more than
+    nine angles run at two waves per SIMD): even counts and the largest,
+    against the plain form."""
+    from sccm_amd.pnamp import FORM_FACTORED, FORM_PLAIN
+    b = make_basis(space)
+    inter = synthetic_interaction(b, 1.2, 201, 202)
+    mesh = make_mesh(2, 3, 2, nphi, 4)
+    Us, Vs = zip(*[synthetic_state(b, *zn, seed=s) for s in (31, 32)])
+    g = _gpu(b, inter, mesh)
+    g.set_states(np.array(Us), np.array(Vs))
+    pairs = [(0, 1), (1, 1)]
+    g.set_form(FORM_PLAIN)
+    zp, kp = g.kernels_batched(pairs, *zn, want_zres=True)
+    g.set_form(FORM_FACTORED)
+    zf, kf = g.kernels_batched(pairs, *zn, want_zres=True)
+    for i in range(len(pairs)):
+        for io in range(zp.shape[2]):
+            assert _rel(zf[i][:, io], zp[i][:, io]) < REL, (space, nphi, pairs[i], io)
+        assert _blockwise(kf[i], kp[i]) < REL
+    g.close()
+
+
 def test_factored_form_rejects_mixed_states():
     from sccm_amd.pnamp import FORM_FACTORED, PnampError
     b, inter, U, V, betas, kern = ge76.load()
