@@ -396,6 +396,30 @@ def test_autotune_keeps_the_kernels():
     g.close()
 
 
+@pytest.mark.parametrize("space,zn", [("sd", (2, 2)), ("f5pg9", (4, 2))])
+def test_every_launch_variant_gives_the_same_kernels(space, zn):
+    """Each launch configuration pnamp_set_tune accepts (Pfaffian at 3, 4, 5
+    waves per SIMD, densities at 4 where h <= 16, chunk divisors 1, 2, 4)
+    computes the same kernels as the default: the variants differ only in
+    register budget and chunking."""
+    from sccm_amd.pnamp import FORM_FACTORED
+    b = make_basis(space)
+    inter = synthetic_interaction(b, 1.2, 201, 202)
+    Us, Vs = zip(*[synthetic_state(b, *zn, seed=s) for s in (41, 42, 43)])
+    g = _gpu(b, inter, make_mesh(2, 3, 2, 5, 4))
+    g.set_states(np.array(Us), np.array(Vs))
+    g.set_form(FORM_FACTORED)
+    pairs = [(0, 1), (1, 2), (2, 2)]
+    ref = g.kernels_batched(pairs, *zn)
+    for pf in (3, 4, 5):
+        for rest in (0, 4):
+            for div in (1, 2, 4):
+                g.set_tune(pf, rest, div)
+                k = g.kernels_batched(pairs, *zn)
+                assert np.abs(k - ref).max() <= 1e-13 * np.abs(ref).max(), (space, pf, rest, div)
+    g.close()
+
+
 # Mixed forms: the V contraction on MFMA in FP32 or split BF16.  Observables
 # without the two-body contraction (norm, Z, N, E_kin, <J^2>) stay FP64 and
 # match the factored form to 1e-10; the energy kernel carries the GEMM
