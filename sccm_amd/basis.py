@@ -77,8 +77,10 @@ class Basis:
                      for a in (self.t, self.nq, self.l, self.twoj, self.twom))
 
 
-def make_basis(space: str) -> Basis:
-    orbits = SPACES[space]
+def make_basis(space: str, orbits: list[tuple[int, int, int]] | None = None) -> Basis:
+    """Basis of a named space, or of ``orbits`` ((n, l, 2j) per orbit in .val
+    order) for any other valence space, which ``space`` then only names."""
+    orbits = SPACES[space] if orbits is None else [tuple(int(x) for x in o) for o in orbits]
     t, nq, l, tj, tm, orb = [], [], [], [], [], []
     for species in (-1, 1):
         for io, (n, ll, twoj) in enumerate(orbits):

@@ -224,6 +224,45 @@ def test_factored_form_at_every_gauge_count_class(space, zn, nphi):
     g.close()
 
 
+# f-p spaces of every species size the factored form and the plain size
+# classes 16..40 have between the named spaces (h = 8, 10, 14, 16, 18; the
+# Pfaffian's 2D layout at T = 2..5 slot rows, plain classes 16, 24, 32, 40)
+FP_SPACES = {
+    "f7": [(0, 3, 7)],
+    "f7p1": [(0, 3, 7), (1, 1, 1)],
+    "f7p": [(0, 3, 7), (1, 1, 3), (1, 1, 1)],
+    "f7f5p1": [(0, 3, 7), (0, 3, 5), (1, 1, 1)],
+    "f7f5p3": [(0, 3, 7), (0, 3, 5), (1, 1, 3)],
+}
+
+
+@pytest.mark.parametrize("space", list(FP_SPACES))
+def test_every_species_size_matches_oracle(oracle_mod, space):
+    from sccm_amd.pnamp import FORM_FACTORED, FORM_PLAIN
+    b = make_basis(space, FP_SPACES[space])
+    zn = (2, 4)
+    inter = synthetic_interaction(b, 1.2, 211, 212)
+    mesh = make_mesh(2, 3, 2, 5, 4)
+    Us, Vs = zip(*[synthetic_state(b, *zn, seed=s) for s in (51, 52)])
+    U, V = np.array(Us), np.array(Vs)
+    g = _gpu(b, inter, mesh)
+    g.set_states(U, V)
+    pairs = [(0, 1), (1, 1)]
+    g.set_form(FORM_PLAIN)
+    zp, kp = g.kernels_batched(pairs, *zn, want_zres=True)
+    g.set_form(FORM_FACTORED)
+    zf, kf = g.kernels_batched(pairs, *zn, want_zres=True)
+    o = oracle_mod.Oracle(b, inter, mesh, *zn)
+    for i, (p, q) in enumerate(pairs):
+        zo, ko = o.kernel_pair(U[p], V[p], U[q], V[q])
+        for io in range(zo.shape[1]):
+            assert _rel(zp[i][:, io], zo[:, io]) < REL, (space, "plain", pairs[i], io)
+            assert _rel(zf[i][:, io], zo[:, io]) < REL, (space, "factored", pairs[i], io)
+        assert _blockwise(kp[i], ko) < REL
+        assert _blockwise(kf[i], ko) < REL
+    g.close()
+
+
 def test_factored_form_rejects_mixed_states():
     from sccm_amd.pnamp import FORM_FACTORED, PnampError
     b, inter, U, V, betas, kern = ge76.load()
@@ -432,13 +471,14 @@ MIX_TOL = {2: 2e-5, 3: 5e-4}
 @pytest.mark.parametrize("space,zn,mesh_args", [
     ("p32", (2, 2), (3, 4, 3, 3, 4)),
     ("p", (2, 2), (3, 4, 3, 5, 4)),
+    *[(s, (2, 4), (2, 3, 2, 5, 4)) for s in FP_SPACES],
     ("sd", (2, 2), (3, 4, 3, 5, 6)),
     ("pf", (2, 4), (2, 3, 3, 9, 8)),
     ("f5pg9", (4, 2), (2, 2, 2, 3, 4)),
 ])
 def test_mixed_forms_match_factored(space, zn, mesh_args, form):
     from sccm_amd.pnamp import FORM_FACTORED
-    b = make_basis(space)
+    b = make_basis(space, FP_SPACES.get(space))
     inter = synthetic_interaction(b, 1.2, 201, 202)
     mesh = make_mesh(*mesh_args)
     Us, Vs = zip(*[synthetic_state(b, *zn, seed=s) for s in (21, 22, 23)])

@@ -126,7 +126,7 @@ def test_reference_pfaffian_pinned():
     assert abs(pf_ref(S) ** 2 - np.linalg.det(S)) < 1e-9 * abs(np.linalg.det(S))
 
 
-@pytest.mark.parametrize("n2", [4, 8, 12, 24, 36, 40, 44])
+@pytest.mark.parametrize("n2", [4, 8, 12, 16, 20, 24, 28, 32, 36, 40, 44])
 def test_layout_matches_reference(n2):
     rng = np.random.default_rng(100 + n2)
     X = rng.normal(size=(n2, n2)) + 1j * rng.normal(size=(n2, n2))
