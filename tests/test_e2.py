@@ -146,13 +146,20 @@ def _rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
+# f-p orbit sets between the named spaces: with f5pg9 they give the plain
+# size classes 16, 32, 48 with E2 (n = 16, 28, 44) and the factored form at
+# h = 8, 14
+FP_ORBITS = {"f7": [(0, 3, 7)], "f7p": [(0, 3, 7), (1, 1, 3), (1, 1, 1)]}
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("space,zn,form", [("p", (2, 2), 1), ("sd", (2, 2), 0), ("sd", (2, 2), 1), ("pf", (2, 4), 0),
-                                           ("pf", (2, 4), 1), ("f5pg9", (4, 2), 1)])
+                                           ("pf", (2, 4), 1), ("f5pg9", (4, 2), 0), ("f5pg9", (4, 2), 1),
+                                           *[(s, (2, 4), f) for s in FP_ORBITS for f in (0, 1)]])
 def test_e2_kernels_on_gpu_match_oracle(oracle_mod, space, zn, form):
     from sccm_amd.mesh import make_mesh
     from sccm_amd.pnamp import PNAMP
-    b = make_basis(space)
+    b = make_basis(space, FP_ORBITS.get(space))
     inter = synthetic_interaction(b, 1.2, 201, 202)
     Q = q2_matrices(b, b_osc=1.9)
     mesh = make_mesh(3, 4, 3, 5, 4)
