@@ -117,6 +117,12 @@ class PNAMP:
     """One device context: basis, interaction, mesh and HFB states live in HBM."""
 
     def __init__(self, basis, inter, mesh, device: int = 0):
+        # the norm is the Pfaffian overlap, the branch Kernel4Triaxial takes for
+        # shell-model interactions (Codes/PNAMP/PNAMP_Kernel3D.f90:140-143,
+        # 291-294); its Onishi branch (:286-289, HF states of other
+        # interactions) has no HIP form
+        if getattr(mesh, "norm_method", 0) != 0:
+            raise PnampError(-1, "norm_method 1 (Onishi) is not a HIP form: the norm is the Pfaffian overlap")
         L = load_library()
         self._L = L
         self._keep = []

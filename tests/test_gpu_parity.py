@@ -263,6 +263,35 @@ def test_every_species_size_matches_oracle(oracle_mod, space):
     g.close()
 
 
+@pytest.mark.parametrize("cscale,iscale", [(10.0, 1), (0.5, 0), (1.0, -1)])
+def test_overflow_scaling_matches_oracle(oracle_mod, cscale, iscale):
+    """Input%Cscale multiplies the Pfaffian's matrix (Codes/PNAMP/PNAMP_Proj.f90:206)
+    and Input%iscale the vacuum normalisations by 10^(-n iscale / 2)
+    (Codes/PNAMP/PNAMP_Kernel3D.f90:147-151, 291-294): both forms against the
+    oracle with the same scaling."""
+    from sccm_amd.pnamp import FORM_FACTORED, FORM_PLAIN
+    b = make_basis("sd")
+    zn = (2, 2)
+    inter = synthetic_interaction(b, 1.2, 201, 202)
+    mesh = make_mesh(2, 3, 2, 5, 4, cscale=cscale, iscale=iscale)
+    # well-conditioned states (cond(U) about 40): the vacuum normalisation
+    # det(1 + Z^+ Z), Z = V U^-1, carries cond(U)^2 eps, so two correct
+    # evaluations of it differ by 6e-10 already at cond(U) = 2.4e3 (seed 62)
+    Us, Vs = zip(*[synthetic_state(b, *zn, seed=s) for s in (21, 22)])
+    U, V = np.array(Us), np.array(Vs)
+    g = _gpu(b, inter, mesh)
+    g.set_states(U, V)
+    o = oracle_mod.Oracle(b, inter, mesh, *zn)
+    zo, ko = o.kernel_pair(U[0], V[0], U[1], V[1])
+    for form in (FORM_PLAIN, FORM_FACTORED):
+        g.set_form(form)
+        zr, k = g.kernel_pair(0, 1, *zn, want_zres=True)
+        for io in range(zo.shape[1]):
+            assert _rel(zr[:, io], zo[:, io]) < REL, (cscale, iscale, form, io)
+        assert _blockwise(k, ko) < REL
+    g.close()
+
+
 def test_factored_form_rejects_mixed_states():
     from sccm_amd.pnamp import FORM_FACTORED, PnampError
     b, inter, U, V, betas, kern = ge76.load()

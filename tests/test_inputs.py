@@ -67,3 +67,19 @@ def test_converter_reproduces_reference_mscheme_list():
     _, inter, _, _, _, _ = ge76.load()
     assert me.size == inter.me.size == 10330
     assert np.abs(me - inter.me).max() < 1e-9
+
+
+def test_onishi_norm_is_refused_before_any_device_call():
+    """The HIP path's norm is the Pfaffian overlap (the reference's branch for
+    shell-model interactions); a mesh asking for the Onishi norm is refused
+    with an error, not computed with the other formula."""
+    import dataclasses
+    import pytest
+    from sccm_amd.basis import make_basis
+    from sccm_amd.interaction import synthetic_interaction
+    from sccm_amd.mesh import make_mesh
+    from sccm_amd.pnamp import PNAMP, PnampError
+    b = make_basis("p")
+    mesh = dataclasses.replace(make_mesh(1, 1, 1, 1, 0), norm_method=1)
+    with pytest.raises(PnampError, match="Onishi"):
+        PNAMP(b, synthetic_interaction(b, 1.0, 1, 2), mesh)
