@@ -292,6 +292,31 @@ def test_overflow_scaling_matches_oracle(oracle_mod, cscale, iscale):
     g.close()
 
 
+@pytest.mark.parametrize("jmax", [9, 10, 12])
+def test_high_spin_projection_matches_oracle(oracle_mod, jmax):
+    """J up to the library's limit of 12: the separable projection kernel up
+    to jmax = 9, the general one above, both forms against the oracle."""
+    from sccm_amd.pnamp import FORM_FACTORED, FORM_PLAIN
+    b = make_basis("f5pg9")
+    zn = (4, 2)
+    inter = synthetic_interaction(b, 1.2, 201, 202)
+    mesh = make_mesh(3, 5, 3, 3, jmax)
+    Us, Vs = zip(*[synthetic_state(b, *zn, seed=s) for s in (21, 22)])
+    U, V = np.array(Us), np.array(Vs)
+    g = _gpu(b, inter, mesh)
+    g.set_states(U, V)
+    o = oracle_mod.Oracle(b, inter, mesh, *zn)
+    zo, ko = o.kernel_pair(U[0], V[0], U[1], V[1])
+    assert ko.shape[1] == jmax + 1
+    for form in (FORM_PLAIN, FORM_FACTORED):
+        g.set_form(form)
+        zr, k = g.kernel_pair(0, 1, *zn, want_zres=True)
+        for io in range(zo.shape[1]):
+            assert _rel(zr[:, io], zo[:, io]) < REL, (jmax, form, io)
+        assert _blockwise(k, ko) < REL, (jmax, form)
+    g.close()
+
+
 def test_factored_form_rejects_mixed_states():
     from sccm_amd.pnamp import FORM_FACTORED, PnampError
     b, inter, U, V, betas, kern = ge76.load()
