@@ -66,7 +66,8 @@ def plain_own_flops(n: int, nterms: int, nphi: int) -> float:
     return pf + 36.0 * n ** 3 + 82.0 * n ** 2 + 26.0 * nterms + 8.0 * n ** 3 / (nphi * nphi)
 
 
-def factored_flops(h: int, sum_b2: int, nphi: int, l_species: int, l_cross: int, nnz_pn: int) -> float:
+def factored_flops(h: int, sum_b2: int, nphi: int, l_species: int, l_cross: int, nnz_pn: int,
+                   schur: bool = False) -> float:
     """Algorithmic real flops of the factored form per (pair, Euler point),
     complex multiply-add = 8 flops.  Per species: R V1, R* U1, R Z1 (block-sparse,
     12 sum_b2 h), X, Y (8 h^3), M0 (8 sum_b2 h); then per gauge angle the
@@ -76,9 +77,11 @@ def factored_flops(h: int, sum_b2: int, nphi: int, l_species: int, l_cross: int,
     (28 flops per entry, 56 when its (cd, ab) partner term is present:
     ``l_species`` entries, ``l_cross`` of them with the partner).  Per Euler
     point once: G = W_pn rho_n (4 flops per CSR entry and phi_n), the nphi^2
-    dot products (8 h^2 each) and the gauge sums (100 per gauge pair)."""
+    dot products (8 h^2 each) and the gauge sums (100 per gauge pair).  With
+    ``schur`` the Pfaffian is the h x h one of the Schur complement (4 h^3 / 3)."""
     setup = 12.0 * sum_b2 * h + 8.0 * h ** 3 + 8.0 * sum_b2 * h
-    per_phi = (32.0 / 3.0 + 20.0 + 12.0) * h ** 3 + 40.0 * h * h + 28.0 * (l_species + l_cross)
+    pfaffian = (4.0 / 3.0 if schur else 32.0 / 3.0) * h ** 3
+    per_phi = pfaffian + (20.0 + 12.0) * h ** 3 + 40.0 * h * h + 28.0 * (l_species + l_cross)
     combine = 4.0 * nnz_pn * nphi + 8.0 * nphi * nphi * h * h + 100.0 * nphi * nphi
     return 2.0 * (setup + nphi * per_phi) + combine
 
@@ -229,6 +232,9 @@ def main():
     ap.add_argument("--levels", type=int, default=3, help="lowest HW levels per J compared")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="factored form: keep the default launch configuration")
+    ap.add_argument("--exact-pfaffian", action="store_true",
+                    help="factored forms: keep the 2h x 2h Pfaffian (by default the h x h Schur-complement "
+                         "Pfaffian is allowed for states with cond_1(Z_s) <= 1e6)")
     ap.add_argument("--tune", default="", help="factored form: 'pf,rest,div' launch configuration instead of "
                                                "autotuning (e.g. 6,3,2)")
     ap.add_argument("--mesh-check", type=int, default=0,
@@ -264,6 +270,10 @@ def main():
     form = {"factored": FORM_FACTORED, "mixed-fp32": FORM_MIXED_FP32, "mixed-bf16": FORM_MIXED_BF16,
             "plain": FORM_PLAIN}[args.form]
     ctx.set_form(form)
+    if form != FORM_PLAIN and not args.exact_pfaffian:
+        ctx.set_schur_pfaffian(True)
+    schur = form != FORM_PLAIN and ctx.schur_pfaffian_active
+    log(f"bench: Pfaffian route {'h x h Schur complement' if schur else '2h x 2h'}")
     npairs = len(w.pairs)
     P = args.pairs_per_step or npairs
 
@@ -330,7 +340,7 @@ def main():
             nz.add((c + a * h, dl + bl * h))
             if a + b != c + d:
                 nz.add((a + c * h, bl + dl * h))
-        fl = factored_flops(h, sum_b2, w.mesh.nphi, l_sp, l_cr, len(nz)) * w.mesh.n_euler * P
+        fl = factored_flops(h, sum_b2, w.mesh.nphi, l_sp, l_cr, len(nz), schur) * w.mesh.n_euler * P
         kname = "k_fac_setup + k_fac_phi + k_fac_combine"
         tform = {FORM_FACTORED: "factored_fp64", FORM_MIXED_FP32: "mixed_fp32", FORM_MIXED_BF16: "mixed_bf16"}[form]
         if form != FORM_FACTORED:
@@ -390,6 +400,14 @@ def main():
             "reference_sequence_equivalent_tflops": plain_equiv,
         },
     }
+    if form != FORM_PLAIN:
+        out["pfaffian"] = {
+            "route": "h x h Schur complement" if schur else "2h x 2h (ZPfaffianF's matrix)",
+            "note": ("Pf(S_s) = Pf(-Z_s) Pf(e^{2i phi} M0 - Z_s^-1), (-Z_s)^-1 and Pf(-Z_s) once per state; "
+                     "taken while cond_1(Z_s) <= 1e6 for every state, rounding up to about cond(Z_s) eps: "
+                     "see gcm.max_rel_kernel_diff against the plain FP64 path"
+                     if schur else "the reference's Pfaffian matrix, partial pivoting"),
+        }
     if tune is not None:
         out["autotune"] = tune
     if form == FORM_PLAIN:

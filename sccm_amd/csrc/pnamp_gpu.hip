@@ -322,6 +322,7 @@ struct PhiParams {
   const double2 *sbuf;              // k_fac_setup output
   const double *UfT, *VfT;          // [nstates][2][h*h], species blocks, row-major (U_s(j, k) at j h + k)
   const double *Z;                  // [nstates][n*n]
+  const double *binv, *pfb;         // Schur route: (-Z_s)^-1 [nstates][2][h*h], Pf(-cscale Z_s) [nstates][2]
   const double *t1b;                // n x n
   const double *mtab;               // [3][h]: m, <m+1|J_+|m> (0 at m = j), j(j+1) per species-local state
   const double2 *jtab;              // [2][h^2][4]: coefficients of rho(a,b) in Ekin and the <J^2> pieces (see pnamp_create)
@@ -487,6 +488,12 @@ __device__ int wave_gauss_jordan(double2 (&col)[H], int *qmap) {
 // steps), PART_REST the rest (13 KB of LDS for rho and one kappa / A block at
 // h = 20, so three waves per SIMD).
 constexpr int PART_PF = 1, PART_REST = 2;
+// PART_PFS: the Pfaffian by the Schur complement of the state block,
+//   Pf([[A, -1], [1, B]]) = Pf(B) Pf(A + B^-1),  A = e^{2i phi} M0, B = -Z_s,
+// an h x h Pfaffian per (pair, Euler point, species, phi) with B^-1 and
+// Pf(cscale B) computed once per state (for states whose Z_s is well
+// conditioned; the others take PART_PF)
+constexpr int PART_PFS = 3;
 
 // VGPRs of the Pfaffian part when it holds the step's column values in
 // registers: the T (T + 1) / 2 complex slots of the 2D layout (T = ceil(2h /
@@ -500,7 +507,9 @@ constexpr int pf_hold_vgprs(int h) {
 // Gauss-Jordan pass fits the wavefront, else two.
 constexpr int pf_default_waves(int h) { return pf_hold_vgprs(h) <= 128 ? 4 : 3; }
 template <int H, int PART>
-constexpr int fac_default_waves() { return PART == PART_PF ? pf_default_waves(H) : (3 * H <= 64 ? 3 : 2); }
+constexpr int fac_default_waves() {
+  return PART == PART_PF ? pf_default_waves(H) : PART == PART_PFS ? pf_default_waves(H / 2) : (3 * H <= 64 ? 3 : 2);
+}
 
 template <int H, int MIX, int PART, int WV = fac_default_waves<H, PART>()>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k_fac_phi(PhiParams p) {
@@ -542,8 +551,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
   const double phi = (nphi == 1) ? 0.0 : (2.0 * M_PI / nphi) * f;
   const double2 e1 = cexpi(phi), e2 = cexpi(2.0 * phi);
 
-  if constexpr (PART == PART_PF) {
-  // ---- Pfaffian
+  if constexpr (PART == PART_PF || PART == PART_PFS) {
+  // ---- Pfaffian (PART_PF: of the 2h x 2h S; PART_PFS: of A + B^-1, h x h)
   // 2D block-cyclic layout: lane = 8 a + b holds S(a + 8 p, b + 8 q) for
   // 0 <= p <= q < T in slot (p, q); the diagonal slots hold the whole 8 x 8
   // block (both triangles), so every row and every column of the matrix is
@@ -554,11 +563,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
   // column c (runtime) selects one slot column Q = c / 8 by a uniform switch.
   // Entries that left the trailing matrix keep being updated with values
   // nobody reads again (cheaper than masking them).
-  constexpr int T = (N2 + 7) / 8, NS = T * (T + 1) / 2;
+  constexpr int MN = (PART == PART_PFS) ? H : N2, MH = MN / 2;  // matrix size, elimination pairs
+  constexpr int T = (MN + 7) / 8, NS = T * (T + 1) / 2;
   // Holding the step's column values in registers halves the LDS reads of
   // the update; where they do not fit the variant's register budget (waves
   // per SIMD WV) they are re-read per slot.
-  constexpr bool HOLD = pf_hold_vgprs(H) <= 512 / WV;
+  constexpr bool HOLD = pf_hold_vgprs(MH) <= 512 / WV;
   const int a = lane >> 3, b = lane & 7;
   auto SI = [](int pp, int qq) { return pp * T - pp * (pp - 1) / 2 + (qq - pp); };
   // LDS vectors of the step, 8T entries each: R = row r, A = row r + 1,
@@ -576,12 +586,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
       for (int qq = pp; qq < T; ++qq) {
         const int i = a + 8 * pp, j = b + 8 * qq, lo = min(i, j), hi = max(i, j);
         const double2 m0v = M0[min(lo, H - 1) + min(hi, H - 1) * H];
-        const double z0v = Z0[(size_t)min(max(lo - H, 0), H - 1) + (size_t)min(max(hi - H, 0), H - 1) * n];
         double2 up;  // S(lo, hi)
-        if (hi < H) up = cmul(e2, m0v);
-        else if (lo < H) up = make_double2((hi - H == lo) ? -1.0 : 0.0, 0.0);
-        else up = make_double2(-z0v, 0.0);
-        up = (hi < N2 && lo != hi) ? cscale(p.cscale, up) : cz();
+        if constexpr (PART == PART_PFS) {
+          const double bv = p.binv[((size_t)q0 * 2 + sp) * HH + min(lo, H - 1) + min(hi, H - 1) * H];
+          up = cmul(e2, m0v);
+          up.x += bv;
+        } else {
+          const double z0v = Z0[(size_t)min(max(lo - H, 0), H - 1) + (size_t)min(max(hi - H, 0), H - 1) * n];
+          if (hi < H) up = cmul(e2, m0v);
+          else if (lo < H) up = make_double2((hi - H == lo) ? -1.0 : 0.0, 0.0);
+          else up = make_double2(-z0v, 0.0);
+        }
+        up = (hi < MN && lo != hi) ? cscale(p.cscale, up) : cz();
         sv[SI(pp, qq)] = (i < j) ? up : cneg(up);
       }
   }
@@ -589,8 +605,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
   bool zero = false;
 #pragma unroll
   for (int P = 0; P < T; ++P) {
-    if (zero || 4 * P >= H - 1) break;
-    for (int t = 0; t < 4 && 4 * P + t < H - 1; ++t) {
+    if (zero || 4 * P >= MH - 1) break;
+    for (int t = 0; t < 4 && 4 * P + t < MH - 1; ++t) {
       const int r = 8 * P + 2 * t;
       // pivot: the largest |S(r, j)|, j > r (row r is slot row P of lanes
       // a = 2t, the eight lanes 16t .. 16t + 7: reduced among themselves)
@@ -601,7 +617,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
           const int j = b + 8 * q;
           const double2 v = sv[SI(P, q)];
           const unsigned kk = pivot_key_hi<6>(v.x * v.x + v.y * v.y, j);
-          key = (j > r && j < N2) ? max(key, kk) : key;
+          key = (j > r && j < MN) ? max(key, kk) : key;
         }
       }
       key = (unsigned)__builtin_amdgcn_readlane((int)oct_umax(key), 16 * t);
@@ -661,7 +677,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
       // Schur update with the columns of the exchanged matrix,
       //   S'(i, j) = S(i, j) + S(i, r+1) S(j, r) / s - S(i, r) S(j, r+1) / s,
       // S(x, r) = -R[x] (R[c] <- R[r + 1]), S(x, r+1) / s -> C[x] (C[c] <- S(r + 1, c) / s)
-      if (lane >= r + 2 && lane < N2) {
+      if (lane >= r + 2 && lane < MN) {
         const bool xc = swap && lane == c;
         const bool low = lane < 8 * Q + 8;
         const double2 v = xc ? C[r + 1] : (low ? C[lane] : D[lane]);
@@ -703,12 +719,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
   if (zero) {
     pf = cz();
   } else {
-    // the last 2 x 2 block: S(N2 - 2, N2 - 1), one lane of slot (T - 1, T - 1)
-    constexpr int LL = ((N2 - 2) & 7) * 8 + ((N2 - 1) & 7);
-    const double2 v = sv[SI((N2 - 2) >> 3, (N2 - 1) >> 3)];
+    // the last 2 x 2 block: S(MN - 2, MN - 1), one lane of slot (T - 1, T - 1)
+    constexpr int LL = ((MN - 2) & 7) * 8 + ((MN - 1) & 7);
+    const double2 v = sv[SI((MN - 2) >> 3, (MN - 1) >> 3)];
     pf = cmul(pf, make_double2(__shfl(v.x, LL), __shfl(v.y, LL)));
   }
 
+  if constexpr (PART == PART_PFS) pf = cscale(p.pfb[(size_t)q0 * 2 + sp], pf);  // Pf(cscale B)
   if (lane == 0) p.out[obase * FAC_OUT] = pf;
   } else {
   // ---- Gauss-Jordan and the density products
@@ -1570,9 +1587,10 @@ struct FacTune {
 };
 
 template <int H>
-static void launch_phi(int mix, const FacTune &t, unsigned grid, hipStream_t s, const PhiParams &pp) {
+static void launch_phi(int mix, const FacTune &t, bool schur, unsigned grid, hipStream_t s, const PhiParams &pp) {
   const dim3 g(grid), b(64);
-  if (H >= 8 && mix == MIX_FP64 && t.pf_waves == 3) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF, 3>), g, b, 0, s, pp);
+  if (schur && t.pf_waves == 0) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PFS>), g, b, 0, s, pp);
+  else if (H >= 8 && mix == MIX_FP64 && t.pf_waves == 3) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF, 3>), g, b, 0, s, pp);
   else if (H >= 8 && mix == MIX_FP64 && t.pf_waves == 4) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF, 4>), g, b, 0, s, pp);
   else if (H >= 8 && mix == MIX_FP64 && t.pf_waves == 5) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF, 5>), g, b, 0, s, pp);
   else hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF>), g, b, 0, s, pp);
@@ -1586,8 +1604,9 @@ static void launch_phi(int mix, const FacTune &t, unsigned grid, hipStream_t s, 
 // Scratch (register spill) bytes per lane of the FP64 variant kernels a
 // tuning choice launches; the autotuner only takes spill-free variants.
 template <int H>
-static size_t phi_scratch(const FacTune &t) {
+static size_t phi_scratch(const FacTune &t, bool schur) {
   const void *pf = (const void *)k_fac_phi<H, MIX_FP64, PART_PF>;
+  if (schur && t.pf_waves == 0) pf = (const void *)k_fac_phi<H, MIX_FP64, PART_PFS>;
   if (H >= 8 && t.pf_waves == 3) pf = (const void *)k_fac_phi<H, MIX_FP64, PART_PF, 3>;
   if (H >= 8 && t.pf_waves == 4) pf = (const void *)k_fac_phi<H, MIX_FP64, PART_PF, 4>;
   if (H >= 8 && t.pf_waves == 5) pf = (const void *)k_fac_phi<H, MIX_FP64, PART_PF, 5>;
@@ -1598,38 +1617,39 @@ static size_t phi_scratch(const FacTune &t) {
   return a.localSizeBytes + b.localSizeBytes;
 }
 
-static size_t phi_scratch_h(int h, const FacTune &t) {
+static size_t phi_scratch_h(int h, const FacTune &t, bool schur) {
   switch (h) {
-    case 2: return phi_scratch<2>(t);
-    case 4: return phi_scratch<4>(t);
-    case 6: return phi_scratch<6>(t);
-    case 8: return phi_scratch<8>(t);
-    case 10: return phi_scratch<10>(t);
-    case 12: return phi_scratch<12>(t);
-    case 14: return phi_scratch<14>(t);
-    case 16: return phi_scratch<16>(t);
-    case 18: return phi_scratch<18>(t);
-    case 20: return phi_scratch<20>(t);
-    case 22: return phi_scratch<22>(t);
+    case 2: return phi_scratch<2>(t, schur);
+    case 4: return phi_scratch<4>(t, schur);
+    case 6: return phi_scratch<6>(t, schur);
+    case 8: return phi_scratch<8>(t, schur);
+    case 10: return phi_scratch<10>(t, schur);
+    case 12: return phi_scratch<12>(t, schur);
+    case 14: return phi_scratch<14>(t, schur);
+    case 16: return phi_scratch<16>(t, schur);
+    case 18: return phi_scratch<18>(t, schur);
+    case 20: return phi_scratch<20>(t, schur);
+    case 22: return phi_scratch<22>(t, schur);
     default: return (size_t)-1;
   }
 }
 
 // Species sizes h = n / 2 with an instantiated k_fac_phi (every h is even:
 // an orbit holds 2j + 1 states).
-static bool launch_phi_h(int h, int mix, const FacTune &t, unsigned grid, hipStream_t s, const PhiParams &pp) {
+static bool launch_phi_h(int h, int mix, const FacTune &t, bool schur, unsigned grid, hipStream_t s,
+                         const PhiParams &pp) {
   switch (h) {
-    case 2: launch_phi<2>(mix, t, grid, s, pp); return true;
-    case 4: launch_phi<4>(mix, t, grid, s, pp); return true;
-    case 6: launch_phi<6>(mix, t, grid, s, pp); return true;
-    case 8: launch_phi<8>(mix, t, grid, s, pp); return true;
-    case 10: launch_phi<10>(mix, t, grid, s, pp); return true;
-    case 12: launch_phi<12>(mix, t, grid, s, pp); return true;
-    case 14: launch_phi<14>(mix, t, grid, s, pp); return true;
-    case 16: launch_phi<16>(mix, t, grid, s, pp); return true;
-    case 18: launch_phi<18>(mix, t, grid, s, pp); return true;
-    case 20: launch_phi<20>(mix, t, grid, s, pp); return true;
-    case 22: launch_phi<22>(mix, t, grid, s, pp); return true;
+    case 2: launch_phi<2>(mix, t, schur, grid, s, pp); return true;
+    case 4: launch_phi<4>(mix, t, schur, grid, s, pp); return true;
+    case 6: launch_phi<6>(mix, t, schur, grid, s, pp); return true;
+    case 8: launch_phi<8>(mix, t, schur, grid, s, pp); return true;
+    case 10: launch_phi<10>(mix, t, schur, grid, s, pp); return true;
+    case 12: launch_phi<12>(mix, t, schur, grid, s, pp); return true;
+    case 14: launch_phi<14>(mix, t, schur, grid, s, pp); return true;
+    case 16: launch_phi<16>(mix, t, schur, grid, s, pp); return true;
+    case 18: launch_phi<18>(mix, t, schur, grid, s, pp); return true;
+    case 20: launch_phi<20>(mix, t, schur, grid, s, pp); return true;
+    case 22: launch_phi<22>(mix, t, schur, grid, s, pp); return true;
     default: return false;
   }
 }
@@ -2162,6 +2182,12 @@ struct pnamp_ctx {
   bool fac_list_ok = false;
   std::string fac_reason;
   double *d_Uf = nullptr, *d_Vf = nullptr, *d_UfT = nullptr, *d_VfT = nullptr;
+  // Schur route of the factored Pfaffian (set when every species block of
+  // every state has cond(Z_s) <= PFS_COND_MAX): (-Z_s)^-1 [nstates][2][h*h]
+  // and Pf(-cscale Z_s) [nstates][2]
+  double *d_binv = nullptr, *d_pfb = nullptr;
+  bool pf_schur = false;      // the states allow it
+  bool schur_allowed = false; // the caller asked for it (pnamp_set_schur_pfaffian)
   double *d_mtab = nullptr;         // [3][h]: m, <m+1|J_+|m>, j(j+1) per species-local state
   double2 *d_fout = nullptr, *d_frho = nullptr, *d_sbuf = nullptr;
   int *d_fstatus = nullptr;
@@ -2659,6 +2685,78 @@ static void host_jacobi(int n, std::vector<double> A, std::vector<double> &Q, st
   for (int i = 0; i < n; ++i) w[i] = A[i + (size_t)i * n];
 }
 
+// Largest cond_1(Z_s) = |Z_s|_1 |Z_s^-1|_1 of the Schur route of the factored
+// Pfaffian: its Pfaffians carry up to about cond(Z_s) eps of relative rounding
+// (at most 6.1e-11 over random rotations of states with cond_1 up to 5.6e5).
+constexpr double PFS_COND_MAX = 1.0e6;
+
+// Inverse of a real m x m matrix (column-major) by Gauss-Jordan with partial
+// pivoting; false if a pivot is exactly zero.
+static bool host_inverse(int m, std::vector<double> A, std::vector<double> &X) {
+  X.assign((size_t)m * m, 0.0);
+  for (int i = 0; i < m; ++i) X[i + (size_t)i * m] = 1.0;
+  for (int k = 0; k < m; ++k) {
+    int p = k;
+    for (int i = k + 1; i < m; ++i)
+      if (std::fabs(A[i + (size_t)k * m]) > std::fabs(A[p + (size_t)k * m])) p = i;
+    if (A[p + (size_t)k * m] == 0.0) return false;
+    for (int j = 0; j < m; ++j) {
+      std::swap(A[k + (size_t)j * m], A[p + (size_t)j * m]);
+      std::swap(X[k + (size_t)j * m], X[p + (size_t)j * m]);
+    }
+    const double inv = 1.0 / A[k + (size_t)k * m];
+    for (int j = 0; j < m; ++j) {
+      A[k + (size_t)j * m] *= inv;
+      X[k + (size_t)j * m] *= inv;
+    }
+    for (int i = 0; i < m; ++i) {
+      if (i == k) continue;
+      const double f = A[i + (size_t)k * m];
+      if (f == 0.0) continue;
+      for (int j = 0; j < m; ++j) {
+        A[i + (size_t)j * m] -= f * A[k + (size_t)j * m];
+        X[i + (size_t)j * m] -= f * X[k + (size_t)j * m];
+      }
+    }
+  }
+  return true;
+}
+
+// Pfaffian of a real skew m x m matrix (column-major, m even) by Parlett-Reid
+// with the pivot of largest modulus in row k.
+static double host_pfaffian(int m, std::vector<double> A) {
+  double pf = 1.0;
+  for (int k = 0; k + 1 < m; k += 2) {
+    int c = k + 1;
+    for (int j = k + 2; j < m; ++j)
+      if (std::fabs(A[k + (size_t)j * m]) > std::fabs(A[k + (size_t)c * m])) c = j;
+    if (c != k + 1) {
+      for (int i = 0; i < m; ++i) std::swap(A[i + (size_t)(k + 1) * m], A[i + (size_t)c * m]);
+      for (int j = 0; j < m; ++j) std::swap(A[(k + 1) + (size_t)j * m], A[c + (size_t)j * m]);
+      pf = -pf;
+    }
+    const double s = A[k + (size_t)(k + 1) * m];
+    if (s == 0.0) return 0.0;
+    pf *= s;
+    // S(i, j) += (S(i, k+1) S(j, k) - S(i, k) S(j, k+1)) / s for i, j > k + 1
+    for (int j = k + 2; j < m; ++j)
+      for (int i = k + 2; i < m; ++i)
+        A[i + (size_t)j * m] += (A[i + (size_t)(k + 1) * m] * A[j + (size_t)k * m] -
+                                 A[i + (size_t)k * m] * A[j + (size_t)(k + 1) * m]) / s;
+  }
+  return pf;
+}
+
+static double host_norm1(int m, const std::vector<double> &A) {
+  double r = 0.0;
+  for (int j = 0; j < m; ++j) {
+    double col = 0.0;
+    for (int i = 0; i < m; ++i) col += std::fabs(A[i + (size_t)j * m]);
+    r = std::max(r, col);
+  }
+  return r;
+}
+
 extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, const double *V) {
   if (!c || !U || !V || nstates <= 0) return PNAMP_E_ARG;
   const size_t nn = (size_t)c->n * c->n;
@@ -2698,7 +2796,8 @@ extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, cons
   std::vector<double> Zh(nn * nstates);
   HIPCHK(c, hipMemcpy(Zh.data(), c->d_Z, sizeof(double) * Zh.size(), hipMemcpyDeviceToHost));
   std::vector<double> Uf((size_t)nstates * 2 * h * h), Vf((size_t)nstates * 2 * h * h);
-  bool sep = true;
+  std::vector<double> Binv((size_t)nstates * 2 * h * h), pfb((size_t)nstates * 2);
+  bool sep = true, schur = true;
   for (int q = 0; q < nstates && sep; ++q) {
     const double *Zq = Zh.data() + (size_t)q * nn;
     double scale = 0.0, off = 0.0;
@@ -2737,9 +2836,30 @@ extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, cons
           for (int k = 0; k < h; ++k) acc += Zq[(spc * h + i) + (size_t)(spc * h + k) * n] * Us[k + (size_t)j * h];
           Vs[i + (size_t)j * h] = acc;
         }
+      // Schur route: B = -Z_s from its strict upper triangle (the entries the
+      // Pfaffian kernel reads), B^-1 and Pf(cscale B)
+      if (schur) {
+        std::vector<double> B((size_t)h * h, 0.0), X;
+        for (int j = 0; j < h; ++j)
+          for (int i = 0; i < j; ++i) {
+            const double z = Zq[(spc * h + i) + (size_t)(spc * h + j) * n];
+            B[i + (size_t)j * h] = -z;
+            B[j + (size_t)i * h] = z;
+          }
+        if (!host_inverse(h, B, X) || !(host_norm1(h, B) * host_norm1(h, X) <= PFS_COND_MAX)) {
+          schur = false;
+        } else {
+          std::copy(X.begin(), X.end(), Binv.begin() + ((size_t)q * 2 + spc) * h * h);
+          for (double &x : B) x *= c->cscale;
+          pfb[(size_t)q * 2 + spc] = host_pfaffian(h, B);
+        }
+      }
     }
   }
   c->separable = sep;
+  schur = schur && sep;
+  if (schur != c->pf_schur) drop_graph(c);  // the captured launches name the Pfaffian variant
+  c->pf_schur = schur;
   if (sep) {
     int r2 = 0;
     // row-major copies (U_s(j, k) at j h + k) for the wave-uniform reads of k_fac_phi
@@ -2756,10 +2876,13 @@ extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, cons
       HIPCHK(c, hipMemcpy(c->d_Vf, Vf.data(), b, hipMemcpyHostToDevice));
       HIPCHK(c, hipMemcpy(c->d_UfT, UfT.data(), b, hipMemcpyHostToDevice));
       HIPCHK(c, hipMemcpy(c->d_VfT, VfT.data(), b, hipMemcpyHostToDevice));
+      HIPCHK(c, hipMemcpy(c->d_binv, Binv.data(), b, hipMemcpyHostToDevice));
+      HIPCHK(c, hipMemcpy(c->d_pfb, pfb.data(), sizeof(double) * pfb.size(), hipMemcpyHostToDevice));
     } else {
       c->sep_cap = 0;
       if ((r2 = dupload(c, &c->d_Uf, Uf.data(), Uf.size())) || (r2 = dupload(c, &c->d_Vf, Vf.data(), Vf.size())) ||
-          (r2 = dupload(c, &c->d_UfT, UfT.data(), UfT.size())) || (r2 = dupload(c, &c->d_VfT, VfT.data(), VfT.size())))
+          (r2 = dupload(c, &c->d_UfT, UfT.data(), UfT.size())) || (r2 = dupload(c, &c->d_VfT, VfT.data(), VfT.size())) ||
+          (r2 = dupload(c, &c->d_binv, Binv.data(), Binv.size())) || (r2 = dupload(c, &c->d_pfb, pfb.data(), pfb.size())))
         return r2;
       c->sep_cap = nstates;
     }
@@ -2954,6 +3077,7 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
     PhiParams fp;
     fp.n = c->n; fp.nphi = c->nphi; fp.NE = c->NE; fp.cscale = c->cscale; fp.sbuf = c->d_sbuf;
     fp.UfT = c->d_UfT; fp.VfT = c->d_VfT; fp.Z = c->d_Z; fp.t1b = c->d_t1b; fp.mtab = c->d_mtab;
+    fp.binv = c->d_binv; fp.pfb = c->d_pfb;
     fp.jtab = (const double2 *)c->d_jtab;
     for (int q = 0; q < 2; ++q) {
       fp.lofs[q] = c->d_lofs[q]; fp.nl[q] = c->nl[q];
@@ -3014,7 +3138,7 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
         const size_t nwg = (size_t)np * c->NE * 2 * c->nphi;
         fp.nwg = (int)nwg;
         hipLaunchKernelGGL(k_fac_setup, dim3(c->NE, np, 2), dim3(NTS), setup_lds, c->stream, sp);
-        launch_phi_h(h, mix, c->tune, (unsigned)((nwg + 7) / 8 * 8), c->stream, fp);
+        launch_phi_h(h, mix, c->tune, c->pf_schur && c->schur_allowed, (unsigned)((nwg + 7) / 8 * 8), c->stream, fp);
         if (mix != MIX_FP64) {
           const int ncols = (int)(cols_pp * np);
           for (int q = 0; q < NVPROB; ++q) vp.prob[q].ncols = ncols;
@@ -3466,18 +3590,23 @@ extern "C" int pnamp_autotune(pnamp_ctx *c, int npairs, const int32_t *q0, const
   };
   std::vector<Cand> cands;
   const int h = c->h;
-  if (h >= 8) {
-    for (int pf : {3, 4, 5})
-      for (int rest : {0, 4}) {
-        if (rest == 4 && h > 16) continue;
-        FacTune t;
-        t.pf_waves = pf;
-        t.rest_waves = rest;
-        if (phi_scratch_h(h, t) != 0) continue;  // a variant that spills registers is no candidate
-        for (int div : {1, 2, 4}) cands.push_back({pf, rest, div});
-      }
-  }
+  // Pfaffian: the Schur route (0) where the states allow it, the 2h x 2h
+  // kernel at 3, 4, 5 waves per SIMD (h >= 8)
+  const bool schur = c->pf_schur && c->schur_allowed;
+  std::vector<int> pfs;
+  if (schur) pfs.push_back(0);
+  if (h >= 8) pfs.insert(pfs.end(), {3, 4, 5});
+  for (int pf : pfs)
+    for (int rest : {0, 4}) {
+      if (rest == 4 && (h > 16 || h < 8)) continue;
+      FacTune t;
+      t.pf_waves = pf;
+      t.rest_waves = rest;
+      if (phi_scratch_h(h, t, schur) != 0) continue;  // a variant that spills registers is no candidate
+      for (int div : {1, 2, 4}) cands.push_back({pf, rest, div});
+    }
   if (cands.empty()) cands.push_back({0, 0, 1});
+  const int pf_dflt = schur ? 0 : pf_default_waves(h);
   if ((int)cands.size() > PNAMP_TUNE_MAX) return fail(c, PNAMP_E_SIZE, "autotune: too many candidates");
   const FacTune keep = c->tune;
   float best = 0.0f;
@@ -3496,7 +3625,7 @@ extern "C" int pnamp_autotune(pnamp_ctx *c, int npairs, const int32_t *q0, const
     }
     if (ms) ms[i] = c->ms_points;
     if (cand) {
-      cand[3 * i] = cands[i].pf ? cands[i].pf : pf_default_waves(h);
+      cand[3 * i] = cands[i].pf ? cands[i].pf : pf_dflt;
       cand[3 * i + 1] = cands[i].rest ? cands[i].rest : (3 * h <= 64 ? 3 : 2);
       cand[3 * i + 2] = cands[i].div;
     }
@@ -3510,12 +3639,23 @@ extern "C" int pnamp_autotune(pnamp_ctx *c, int npairs, const int32_t *q0, const
   c->tune.chunk_div = cands[ib].div;
   drop_graph(c);
   if (choice) {
-    choice[0] = cands[ib].pf ? cands[ib].pf : pf_default_waves(h);
+    choice[0] = cands[ib].pf ? cands[ib].pf : pf_dflt;
     choice[1] = cands[ib].rest ? cands[ib].rest : (3 * h <= 64 ? 3 : 2);  // the default
     choice[2] = cands[ib].div;
   }
   if (ncand) *ncand = (int)cands.size();
   return PNAMP_OK;
+}
+
+extern "C" int pnamp_set_schur_pfaffian(pnamp_ctx *c, int enable) {
+  if (!c) return PNAMP_E_ARG;
+  if (c->schur_allowed != (enable != 0)) drop_graph(c);
+  c->schur_allowed = enable != 0;
+  return PNAMP_OK;
+}
+
+extern "C" int pnamp_schur_pfaffian_active(pnamp_ctx *c) {
+  return c ? ((c->pf_schur && c->schur_allowed) ? 1 : 0) : PNAMP_E_ARG;
 }
 
 extern "C" int pnamp_set_tune(pnamp_ctx *c, int pf_waves, int rest_waves, int chunk_div) {
@@ -3550,7 +3690,7 @@ extern "C" void pnamp_destroy(pnamp_ctx *c) {
                   c->d_pk_a, c->d_pk_b, c->d_blk_off, c->d_blk_sz, c->d_blk_base, c->d_twoj, c->d_twom,
                   c->d_plofs, c->d_plme, c->d_pwork, c->d_t1b, c->d_jtab, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status, c->d_q0,
                   c->d_q1, c->d_pz, c->d_pn, c->d_zres, c->d_kern, c->d_status, c->d_pair_status,
-                  c->d_lofs[0], c->d_lofs[1], c->d_Uf, c->d_Vf,
+                  c->d_lofs[0], c->d_lofs[1], c->d_Uf, c->d_Vf, c->d_binv, c->d_pfb,
                   c->d_UfT, c->d_VfT, c->d_mtab, c->d_fout, c->d_frho,
                   c->d_sbuf, c->d_fstatus, c->d_pn_rows,
                   c->d_pn_blk, c->d_pn_ycol, c->d_pn_w};

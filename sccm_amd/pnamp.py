@@ -99,6 +99,10 @@ def load_library(path: str = LIB_PATH):
     L.pnamp_autotune.restype = C.c_int
     L.pnamp_set_tune.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.pnamp_set_tune.restype = C.c_int
+    L.pnamp_set_schur_pfaffian.argtypes = [vp, C.c_int]
+    L.pnamp_set_schur_pfaffian.restype = C.c_int
+    L.pnamp_schur_pfaffian_active.argtypes = [vp]
+    L.pnamp_schur_pfaffian_active.restype = C.c_int
     L.pnamp_n_euler.argtypes = [vp]
     L.pnamp_n_euler.restype = C.c_int
     L.pnamp_last_error.argtypes = [vp]
@@ -309,6 +313,15 @@ class PNAMP:
         return {"pf_waves": int(ch[0]), "rest_waves": int(ch[1]), "chunk_div": int(ch[2]),
                 "candidates": [{"pf_waves": int(cand[3 * i]), "rest_waves": int(cand[3 * i + 1]),
                                 "chunk_div": int(cand[3 * i + 2]), "ms": float(ms[i])} for i in range(nc.value)]}
+
+    def set_schur_pfaffian(self, enable: bool):
+        """Allows the factored form's h x h Schur-complement Pfaffian (states
+        with cond_1(Z_s) <= 1e6; rounding up to about cond(Z_s) eps)."""
+        self._check(self._L.pnamp_set_schur_pfaffian(self._h, 1 if enable else 0))
+
+    @property
+    def schur_pfaffian_active(self) -> bool:
+        return bool(self._L.pnamp_schur_pfaffian_active(self._h) == 1)
 
     def set_tune(self, pf_waves: int, rest_waves: int, chunk_div: int):
         """Launch configuration of the factored form (a kept autotune choice)."""

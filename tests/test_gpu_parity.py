@@ -486,6 +486,10 @@ def test_autotune_keeps_the_kernels():
     assert (best["pf_waves"], best["chunk_div"]) == (t["pf_waves"], t["chunk_div"])
     after = g.kernels_batched(w.pairs, w.zval, w.nval)
     assert np.abs(after - before).max() <= 1e-14 * np.abs(before).max()
+    # with the Schur-complement Pfaffian allowed it is candidate 0 as well
+    g.set_schur_pfaffian(True)
+    t = g.autotune(w.pairs, w.zval, w.nval)
+    assert [c["pf_waves"] for c in t["candidates"]][:3] == [0, 0, 0]
     g.close()
 
 
@@ -510,6 +514,66 @@ def test_every_launch_variant_gives_the_same_kernels(space, zn):
                 g.set_tune(pf, rest, div)
                 k = g.kernels_batched(pairs, *zn)
                 assert np.abs(k - ref).max() <= 1e-13 * np.abs(ref).max(), (space, pf, rest, div)
+    g.close()
+
+
+def _states_from_z(b, zs, seed):
+    """A separable HFB state built from canonical pair amplitudes: per species
+    Z_s = O diag([[0, z_k], [-z_k, 0]]) O^T (O random orthogonal), U_s = (1 +
+    Z_s^T Z_s)^(-1/2), V_s = Z_s U_s; cond(Z_s) = max z / min z."""
+    rng = np.random.default_rng(seed)
+    h = b.n // 2
+    U = np.zeros((b.n, b.n))
+    V = np.zeros((b.n, b.n))
+    for s in range(2):
+        O, _ = np.linalg.qr(rng.normal(size=(h, h)))
+        D = np.zeros((h, h))
+        for k, z in enumerate(zs):
+            D[2 * k, 2 * k + 1], D[2 * k + 1, 2 * k] = z, -z
+        Z = O @ D @ O.T
+        w, Q = np.linalg.eigh(np.eye(h) + Z.T @ Z)
+        Us = Q @ np.diag(w ** -0.5) @ Q.T
+        U[s * h:(s + 1) * h, s * h:(s + 1) * h] = Us
+        V[s * h:(s + 1) * h, s * h:(s + 1) * h] = Z @ Us
+    return U, V
+
+
+@pytest.mark.parametrize("space,zn", [("sd", (2, 2)), ("pf", (2, 4)), ("f5pg9", (4, 2))])
+def test_schur_pfaffian_route(space, zn):
+    """The opt-in h x h Schur-complement Pfaffian: off by default (the kernels
+    of a context that never enabled it are the 2h x 2h route's), on for
+    well-conditioned states within its stated rounding of the 2h x 2h route
+    and of the oracle chain's inputs, off again (exact) for states with
+    cond_1(Z_s) above 1e6."""
+    from sccm_amd.pnamp import FORM_FACTORED
+    b = make_basis(space)
+    inter = synthetic_interaction(b, 1.2, 201, 202)
+    g = _gpu(b, inter, make_mesh(2, 3, 2, 5, 4))
+    Us, Vs = zip(*[synthetic_state(b, *zn, seed=s) for s in (41, 42, 43)])
+    g.set_states(np.array(Us), np.array(Vs))
+    g.set_form(FORM_FACTORED)
+    pairs = [(0, 1), (1, 2), (2, 2)]
+    full = g.kernels_batched(pairs, *zn)
+    assert not g.schur_pfaffian_active
+    g.set_schur_pfaffian(True)
+    assert g.schur_pfaffian_active
+    sch = g.kernels_batched(pairs, *zn)
+    d = np.abs(sch - full).max() / np.abs(full).max()
+    assert 0.0 < d < 1e-8, d
+    for i in range(len(pairs)):
+        assert _blockwise(sch[i], full[i]) < 2e-8
+    g.set_schur_pfaffian(False)
+    assert np.array_equal(g.kernels_batched(pairs, *zn), full)
+    # ill-conditioned Z_s (pair amplitudes 1e-7 .. 1): the 2h x 2h route stays
+    h = b.n // 2
+    zs = np.geomspace(1e-7, 1.0, h // 2)
+    U2, V2 = zip(*[_states_from_z(b, zs, seed=s) for s in (7, 8)])
+    g.set_schur_pfaffian(True)
+    g.set_states(np.array(U2), np.array(V2))
+    assert g.separable and not g.schur_pfaffian_active
+    k_on = g.kernels_batched([(0, 1)], 2, 2)
+    g.set_schur_pfaffian(False)
+    assert np.array_equal(g.kernels_batched([(0, 1)], 2, 2), k_on)
     g.close()
 
 
