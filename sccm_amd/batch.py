@@ -62,7 +62,8 @@ def instance_states(inst: Instance, basis):
 
 
 def run(n: int = 64, seed: int = 4001, nalp: int = 10, nbet: int = 20, nphi: int = 9, jmax: int = 8,
-        form: str = "factored", plain_pairs: int = 2, log=print, repeats: int = 2) -> dict:
+        form: str = "factored", plain_pairs: int = 2, log=print, repeats: int = 2,
+        schur_pfaffian: bool = False) -> dict:
     from .pnamp import FORM_FACTORED, FORM_MIXED_BF16, FORM_MIXED_FP32, FORM_PLAIN, PNAMP
     fcode = {"factored": FORM_FACTORED, "mixed-fp32": FORM_MIXED_FP32, "mixed-bf16": FORM_MIXED_BF16}[form]
     insts = draw_instances(n, seed)
@@ -92,6 +93,8 @@ def run(n: int = 64, seed: int = 4001, nalp: int = 10, nbet: int = 20, nphi: int
         zv, nv = np.array(zv, dtype=np.int32), np.array(nv, dtype=np.int32)
         ctx.set_states(np.concatenate(Us), np.concatenate(Vs))
         ctx.set_form(fcode)
+        ctx.set_schur_pfaffian(schur_pfaffian)  # taken only if every state's cond_1(Z_s) <= 1e6
+        schur = ctx.schur_pfaffian_active
         tune = None
         if form == "factored":  # launch configuration timed on a sample of this space's pairs
             t = time.perf_counter()
@@ -119,7 +122,8 @@ def run(n: int = 64, seed: int = 4001, nalp: int = 10, nbet: int = 20, nphi: int
                          "chosen_s": dt, "chosen_mesh_pts_per_s": pts / dt, "plain_sample_pairs": int(len(sample)),
                          "plain_s_extrapolated": plain_est, "rel_kernel_diff_vs_plain": rel,
                          "kernel_ms_device": dev_ms, "graph_captures": captures, "graph_replays": replays,
-                         "kernel_launches_per_call": nodes, "autotune": tune}
+                         "kernel_launches_per_call": nodes, "autotune": tune,
+                         "pfaffian": "h x h Schur complement" if schur else "2h x 2h"}
         tot_pts += pts
         t_chosen += dt
         t_plain_est += plain_est
@@ -145,9 +149,11 @@ def main():
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--form", default="factored", choices=("factored", "mixed-fp32", "mixed-bf16"))
     ap.add_argument("--plain-pairs", type=int, default=2)
+    ap.add_argument("--schur-pfaffian", action="store_true",
+                    help="allow the h x h Schur-complement Pfaffian (states with cond_1(Z_s) <= 1e6)")
     a = ap.parse_args()
     r = run(a.instances, form=a.form, plain_pairs=a.plain_pairs,
-            log=lambda *x: print(*x, file=sys.stderr, flush=True))
+            log=lambda *x: print(*x, file=sys.stderr, flush=True), schur_pfaffian=a.schur_pfaffian)
     print(json.dumps(r), flush=True)
 
 
