@@ -264,17 +264,18 @@ int pnamp_set_tune(pnamp_ctx *ctx, int pf_waves, int rest_waves, int chunk_div);
  * forms' Pfaffian by the Schur complement of the state block,
  *   Pf([[e^{2i phi} M0, -1], [1, -Z_s]]) = Pf(-Z_s) Pf(e^{2i phi} M0 - Z_s^-1),
  * an h x h Pfaffian per mesh point instead of ZPfaffianF's 2h x 2h one
- * (Codes/PNAMP/PNAMP_Proj.f90:142-392 on the matrix of Pfaf_Over).  It is
- * taken only while every species block of every state set has
- * cond_1(Z_s) <= 1e6; its Pfaffians then carry up to about cond(Z_s) eps of
+ * (Codes/PNAMP/PNAMP_Proj.f90:142-392 on the matrix of Pfaf_Over).  A pair
+ * takes it when both species blocks of its bra state have
+ * cond_1(Z_s) <= 1e6 (the others keep the 2h x 2h Pfaffian, both kernels
+ * then run over the grid); its Pfaffians carry up to about cond(Z_s) eps of
  * relative rounding (kernels within a few 1e-9 of the 2h x 2h route), so it
  * suits the factored form's GCM-energy target, not the 1e-10 drop-in.  With
  * it allowed, pnamp_autotune also times it (Pfaffian waves 0 in its output).
  * No counterpart in the reference. */
 int pnamp_set_schur_pfaffian(pnamp_ctx *ctx, int enable);
 
-/* 1 while the Schur-complement Pfaffian is allowed and the current states
- * admit it, else 0. */
+/* With the Schur-complement Pfaffian allowed: 2 if every current state
+ * takes it, 1 if some do, 0 if none; 0 while it is not allowed. */
 int pnamp_schur_pfaffian_active(pnamp_ctx *ctx);
 
 /* Launch-graph counters of the factored and mixed forms: graphs captured

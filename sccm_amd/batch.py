@@ -123,7 +123,9 @@ def run(n: int = 64, seed: int = 4001, nalp: int = 10, nbet: int = 20, nphi: int
                          "plain_s_extrapolated": plain_est, "rel_kernel_diff_vs_plain": rel,
                          "kernel_ms_device": dev_ms, "graph_captures": captures, "graph_replays": replays,
                          "kernel_launches_per_call": nodes, "autotune": tune,
-                         "pfaffian": "h x h Schur complement" if schur else "2h x 2h"}
+                         "pfaffian": {2: "h x h Schur complement (every state)",
+                                      1: "h x h Schur complement for the bra states that pass, 2h x 2h for the rest",
+                                      0: "2h x 2h"}[schur]}
         tot_pts += pts
         t_chosen += dt
         t_plain_est += plain_est

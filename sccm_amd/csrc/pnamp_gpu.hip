@@ -323,6 +323,7 @@ struct PhiParams {
   const double *UfT, *VfT;          // [nstates][2][h*h], species blocks, row-major (U_s(j, k) at j h + k)
   const double *Z;                  // [nstates][n*n]
   const double *binv, *pfb;         // Schur route: (-Z_s)^-1 [nstates][2][h*h], Pf(-cscale Z_s) [nstates][2]
+  const int *pfs_ok;                // both Pfaffian kernels launched: [nstates] route of bra state q0, else NULL
   const double *t1b;                // n x n
   const double *mtab;               // [3][h]: m, <m+1|J_+|m> (0 at m = j), j(j+1) per species-local state
   const double2 *jtab;              // [2][h^2][4]: coefficients of rho(a,b) in Ekin and the <J^2> pieces (see pnamp_create)
@@ -553,6 +554,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
 
   if constexpr (PART == PART_PF || PART == PART_PFS) {
   // ---- Pfaffian (PART_PF: of the 2h x 2h S; PART_PFS: of A + B^-1, h x h)
+  // With both kernels launched over the grid, each keeps the pairs whose bra
+  // state takes its route (one pair per wave: the test is wave-uniform)
+  if (p.pfs_ok && ((p.pfs_ok[q0] != 0) != (PART == PART_PFS))) return;
   // 2D block-cyclic layout: lane = 8 a + b holds S(a + 8 p, b + 8 q) for
   // 0 <= p <= q < T in slot (p, q); the diagonal slots hold the whole 8 x 8
   // block (both triangles), so every row and every column of the matrix is
@@ -1586,11 +1590,14 @@ struct FacTune {
   int pf_waves = 0, rest_waves = 0, chunk_div = 1;
 };
 
+// route: 0 the 2h x 2h Pfaffian, 1 the Schur route for every state, 2 both
+// kernels (pp.pfs_ok names each pair's); a pf_waves choice takes the 2h x 2h one
 template <int H>
-static void launch_phi(int mix, const FacTune &t, bool schur, unsigned grid, hipStream_t s, const PhiParams &pp) {
+static void launch_phi(int mix, const FacTune &t, int route, unsigned grid, hipStream_t s, const PhiParams &pp) {
   const dim3 g(grid), b(64);
-  if (schur && t.pf_waves == 0) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PFS>), g, b, 0, s, pp);
-  else if (H >= 8 && mix == MIX_FP64 && t.pf_waves == 3) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF, 3>), g, b, 0, s, pp);
+  if (route >= 1 && t.pf_waves == 0) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PFS>), g, b, 0, s, pp);
+  if (route == 1 && t.pf_waves == 0) {
+  } else if (H >= 8 && mix == MIX_FP64 && t.pf_waves == 3) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF, 3>), g, b, 0, s, pp);
   else if (H >= 8 && mix == MIX_FP64 && t.pf_waves == 4) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF, 4>), g, b, 0, s, pp);
   else if (H >= 8 && mix == MIX_FP64 && t.pf_waves == 5) hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF, 5>), g, b, 0, s, pp);
   else hipLaunchKernelGGL((k_fac_phi<H, MIX_FP64, PART_PF>), g, b, 0, s, pp);
@@ -1636,20 +1643,20 @@ static size_t phi_scratch_h(int h, const FacTune &t, bool schur) {
 
 // Species sizes h = n / 2 with an instantiated k_fac_phi (every h is even:
 // an orbit holds 2j + 1 states).
-static bool launch_phi_h(int h, int mix, const FacTune &t, bool schur, unsigned grid, hipStream_t s,
+static bool launch_phi_h(int h, int mix, const FacTune &t, int route, unsigned grid, hipStream_t s,
                          const PhiParams &pp) {
   switch (h) {
-    case 2: launch_phi<2>(mix, t, schur, grid, s, pp); return true;
-    case 4: launch_phi<4>(mix, t, schur, grid, s, pp); return true;
-    case 6: launch_phi<6>(mix, t, schur, grid, s, pp); return true;
-    case 8: launch_phi<8>(mix, t, schur, grid, s, pp); return true;
-    case 10: launch_phi<10>(mix, t, schur, grid, s, pp); return true;
-    case 12: launch_phi<12>(mix, t, schur, grid, s, pp); return true;
-    case 14: launch_phi<14>(mix, t, schur, grid, s, pp); return true;
-    case 16: launch_phi<16>(mix, t, schur, grid, s, pp); return true;
-    case 18: launch_phi<18>(mix, t, schur, grid, s, pp); return true;
-    case 20: launch_phi<20>(mix, t, schur, grid, s, pp); return true;
-    case 22: launch_phi<22>(mix, t, schur, grid, s, pp); return true;
+    case 2: launch_phi<2>(mix, t, route, grid, s, pp); return true;
+    case 4: launch_phi<4>(mix, t, route, grid, s, pp); return true;
+    case 6: launch_phi<6>(mix, t, route, grid, s, pp); return true;
+    case 8: launch_phi<8>(mix, t, route, grid, s, pp); return true;
+    case 10: launch_phi<10>(mix, t, route, grid, s, pp); return true;
+    case 12: launch_phi<12>(mix, t, route, grid, s, pp); return true;
+    case 14: launch_phi<14>(mix, t, route, grid, s, pp); return true;
+    case 16: launch_phi<16>(mix, t, route, grid, s, pp); return true;
+    case 18: launch_phi<18>(mix, t, route, grid, s, pp); return true;
+    case 20: launch_phi<20>(mix, t, route, grid, s, pp); return true;
+    case 22: launch_phi<22>(mix, t, route, grid, s, pp); return true;
     default: return false;
   }
 }
@@ -2186,8 +2193,9 @@ struct pnamp_ctx {
   // every state has cond(Z_s) <= PFS_COND_MAX): (-Z_s)^-1 [nstates][2][h*h]
   // and Pf(-cscale Z_s) [nstates][2]
   double *d_binv = nullptr, *d_pfb = nullptr;
-  bool pf_schur = false;      // the states allow it
-  bool schur_allowed = false; // the caller asked for it (pnamp_set_schur_pfaffian)
+  int *d_pfs_ok = nullptr;             // [nstates]: 1 if both species blocks of the state pass
+  bool pfs_any = false, pfs_all = false;  // some / every state passes
+  bool schur_allowed = false;          // the caller asked for it (pnamp_set_schur_pfaffian)
   double *d_mtab = nullptr;         // [3][h]: m, <m+1|J_+|m>, j(j+1) per species-local state
   double2 *d_fout = nullptr, *d_frho = nullptr, *d_sbuf = nullptr;
   int *d_fstatus = nullptr;
@@ -2797,7 +2805,8 @@ extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, cons
   HIPCHK(c, hipMemcpy(Zh.data(), c->d_Z, sizeof(double) * Zh.size(), hipMemcpyDeviceToHost));
   std::vector<double> Uf((size_t)nstates * 2 * h * h), Vf((size_t)nstates * 2 * h * h);
   std::vector<double> Binv((size_t)nstates * 2 * h * h), pfb((size_t)nstates * 2);
-  bool sep = true, schur = true;
+  std::vector<int> pfs_ok(nstates, 1);
+  bool sep = true;
   for (int q = 0; q < nstates && sep; ++q) {
     const double *Zq = Zh.data() + (size_t)q * nn;
     double scale = 0.0, off = 0.0;
@@ -2838,7 +2847,7 @@ extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, cons
         }
       // Schur route: B = -Z_s from its strict upper triangle (the entries the
       // Pfaffian kernel reads), B^-1 and Pf(cscale B)
-      if (schur) {
+      if (pfs_ok[q]) {
         std::vector<double> B((size_t)h * h, 0.0), X;
         for (int j = 0; j < h; ++j)
           for (int i = 0; i < j; ++i) {
@@ -2847,7 +2856,7 @@ extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, cons
             B[j + (size_t)i * h] = z;
           }
         if (!host_inverse(h, B, X) || !(host_norm1(h, B) * host_norm1(h, X) <= PFS_COND_MAX)) {
-          schur = false;
+          pfs_ok[q] = 0;
         } else {
           std::copy(X.begin(), X.end(), Binv.begin() + ((size_t)q * 2 + spc) * h * h);
           for (double &x : B) x *= c->cscale;
@@ -2857,9 +2866,14 @@ extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, cons
     }
   }
   c->separable = sep;
-  schur = schur && sep;
-  if (schur != c->pf_schur) drop_graph(c);  // the captured launches name the Pfaffian variant
-  c->pf_schur = schur;
+  bool any = false, all = sep;
+  for (int q = 0; q < nstates; ++q) {
+    any = any || (sep && pfs_ok[q]);
+    all = all && pfs_ok[q];
+  }
+  if (any != c->pfs_any || all != c->pfs_all) drop_graph(c);  // the captured launches name the Pfaffian kernels
+  c->pfs_any = any;
+  c->pfs_all = all;
   if (sep) {
     int r2 = 0;
     // row-major copies (U_s(j, k) at j h + k) for the wave-uniform reads of k_fac_phi
@@ -2878,11 +2892,13 @@ extern "C" int pnamp_set_states(pnamp_ctx *c, int nstates, const double *U, cons
       HIPCHK(c, hipMemcpy(c->d_VfT, VfT.data(), b, hipMemcpyHostToDevice));
       HIPCHK(c, hipMemcpy(c->d_binv, Binv.data(), b, hipMemcpyHostToDevice));
       HIPCHK(c, hipMemcpy(c->d_pfb, pfb.data(), sizeof(double) * pfb.size(), hipMemcpyHostToDevice));
+      HIPCHK(c, hipMemcpy(c->d_pfs_ok, pfs_ok.data(), sizeof(int) * pfs_ok.size(), hipMemcpyHostToDevice));
     } else {
       c->sep_cap = 0;
       if ((r2 = dupload(c, &c->d_Uf, Uf.data(), Uf.size())) || (r2 = dupload(c, &c->d_Vf, Vf.data(), Vf.size())) ||
           (r2 = dupload(c, &c->d_UfT, UfT.data(), UfT.size())) || (r2 = dupload(c, &c->d_VfT, VfT.data(), VfT.size())) ||
-          (r2 = dupload(c, &c->d_binv, Binv.data(), Binv.size())) || (r2 = dupload(c, &c->d_pfb, pfb.data(), pfb.size())))
+          (r2 = dupload(c, &c->d_binv, Binv.data(), Binv.size())) || (r2 = dupload(c, &c->d_pfb, pfb.data(), pfb.size())) ||
+          (r2 = dupload(c, &c->d_pfs_ok, pfs_ok.data(), pfs_ok.size())))
         return r2;
       c->sep_cap = nstates;
     }
@@ -3078,6 +3094,8 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
     fp.n = c->n; fp.nphi = c->nphi; fp.NE = c->NE; fp.cscale = c->cscale; fp.sbuf = c->d_sbuf;
     fp.UfT = c->d_UfT; fp.VfT = c->d_VfT; fp.Z = c->d_Z; fp.t1b = c->d_t1b; fp.mtab = c->d_mtab;
     fp.binv = c->d_binv; fp.pfb = c->d_pfb;
+    const bool pfs = c->schur_allowed && c->pfs_any, both = pfs && !c->pfs_all && c->tune.pf_waves == 0;
+    fp.pfs_ok = both ? c->d_pfs_ok : nullptr;
     fp.jtab = (const double2 *)c->d_jtab;
     for (int q = 0; q < 2; ++q) {
       fp.lofs[q] = c->d_lofs[q]; fp.nl[q] = c->nl[q];
@@ -3138,7 +3156,7 @@ static int kernels_impl(pnamp_ctx *c, int npairs, const int32_t *q0, const int32
         const size_t nwg = (size_t)np * c->NE * 2 * c->nphi;
         fp.nwg = (int)nwg;
         hipLaunchKernelGGL(k_fac_setup, dim3(c->NE, np, 2), dim3(NTS), setup_lds, c->stream, sp);
-        launch_phi_h(h, mix, c->tune, c->pf_schur && c->schur_allowed, (unsigned)((nwg + 7) / 8 * 8), c->stream, fp);
+        launch_phi_h(h, mix, c->tune, pfs ? (both ? 2 : 1) : 0, (unsigned)((nwg + 7) / 8 * 8), c->stream, fp);
         if (mix != MIX_FP64) {
           const int ncols = (int)(cols_pp * np);
           for (int q = 0; q < NVPROB; ++q) vp.prob[q].ncols = ncols;
@@ -3592,7 +3610,7 @@ extern "C" int pnamp_autotune(pnamp_ctx *c, int npairs, const int32_t *q0, const
   const int h = c->h;
   // Pfaffian: the Schur route (0) where the states allow it, the 2h x 2h
   // kernel at 3, 4, 5 waves per SIMD (h >= 8)
-  const bool schur = c->pf_schur && c->schur_allowed;
+  const bool schur = c->schur_allowed && c->pfs_any;
   std::vector<int> pfs;
   if (schur) pfs.push_back(0);
   if (h >= 8) pfs.insert(pfs.end(), {3, 4, 5});
@@ -3655,7 +3673,7 @@ extern "C" int pnamp_set_schur_pfaffian(pnamp_ctx *c, int enable) {
 }
 
 extern "C" int pnamp_schur_pfaffian_active(pnamp_ctx *c) {
-  return c ? ((c->pf_schur && c->schur_allowed) ? 1 : 0) : PNAMP_E_ARG;
+  return c ? ((c->pfs_any && c->schur_allowed) ? (c->pfs_all ? 2 : 1) : 0) : PNAMP_E_ARG;
 }
 
 extern "C" int pnamp_set_tune(pnamp_ctx *c, int pf_waves, int rest_waves, int chunk_div) {
@@ -3690,7 +3708,7 @@ extern "C" void pnamp_destroy(pnamp_ctx *c) {
                   c->d_pk_a, c->d_pk_b, c->d_blk_off, c->d_blk_sz, c->d_blk_base, c->d_twoj, c->d_twom,
                   c->d_plofs, c->d_plme, c->d_pwork, c->d_t1b, c->d_jtab, c->d_U, c->d_V, c->d_Z, c->d_det, c->d_state_status, c->d_q0,
                   c->d_q1, c->d_pz, c->d_pn, c->d_zres, c->d_kern, c->d_status, c->d_pair_status,
-                  c->d_lofs[0], c->d_lofs[1], c->d_Uf, c->d_Vf, c->d_binv, c->d_pfb,
+                  c->d_lofs[0], c->d_lofs[1], c->d_Uf, c->d_Vf, c->d_binv, c->d_pfb, c->d_pfs_ok,
                   c->d_UfT, c->d_VfT, c->d_mtab, c->d_fout, c->d_frho,
                   c->d_sbuf, c->d_fstatus, c->d_pn_rows,
                   c->d_pn_blk, c->d_pn_ycol, c->d_pn_w};

@@ -320,8 +320,12 @@ class PNAMP:
         self._check(self._L.pnamp_set_schur_pfaffian(self._h, 1 if enable else 0))
 
     @property
-    def schur_pfaffian_active(self) -> bool:
-        return bool(self._L.pnamp_schur_pfaffian_active(self._h) == 1)
+    def schur_pfaffian_active(self) -> int:
+        """2: every state takes the Schur-complement Pfaffian, 1: the pairs
+        of some bra states do, 0: none (not allowed, or no state passes)."""
+        r = self._L.pnamp_schur_pfaffian_active(self._h)
+        self._check(min(r, 0))
+        return r
 
     def set_tune(self, pf_waves: int, rest_waves: int, chunk_div: int):
         """Launch configuration of the factored form (a kept autotune choice)."""

@@ -554,9 +554,9 @@ def test_schur_pfaffian_route(space, zn):
     g.set_form(FORM_FACTORED)
     pairs = [(0, 1), (1, 2), (2, 2)]
     full = g.kernels_batched(pairs, *zn)
-    assert not g.schur_pfaffian_active
+    assert g.schur_pfaffian_active == 0
     g.set_schur_pfaffian(True)
-    assert g.schur_pfaffian_active
+    assert g.schur_pfaffian_active == 2
     sch = g.kernels_batched(pairs, *zn)
     d = np.abs(sch - full).max() / np.abs(full).max()
     assert 0.0 < d < 1e-8, d
@@ -570,10 +570,23 @@ def test_schur_pfaffian_route(space, zn):
     U2, V2 = zip(*[_states_from_z(b, zs, seed=s) for s in (7, 8)])
     g.set_schur_pfaffian(True)
     g.set_states(np.array(U2), np.array(V2))
-    assert g.separable and not g.schur_pfaffian_active
+    assert g.separable and g.schur_pfaffian_active == 0
     k_on = g.kernels_batched([(0, 1)], 2, 2)
     g.set_schur_pfaffian(False)
     assert np.array_equal(g.kernels_batched([(0, 1)], 2, 2), k_on)
+    # a mixed set: pairs whose bra state passes take the Schur route, the
+    # others keep the 2h x 2h Pfaffian (both kernels over one grid)
+    g.set_states(np.array([Us[0], U2[0], Us[1]]), np.array([Vs[0], V2[0], Vs[1]]))
+    mixed_pairs = [(0, 2), (1, 0), (2, 1), (1, 1)]
+    k_full = g.kernels_batched(mixed_pairs, *zn)
+    g.set_schur_pfaffian(True)
+    assert g.schur_pfaffian_active == 1
+    k_mix = g.kernels_batched(mixed_pairs, *zn)
+    for i, (p, q) in enumerate(mixed_pairs):
+        if p == 1:  # ill-conditioned bra: the 2h x 2h route, bit for bit
+            assert np.array_equal(k_mix[i], k_full[i]), (p, q)
+        else:
+            assert 0.0 < _blockwise(k_mix[i], k_full[i]) < 2e-8, (p, q)
     g.close()
 
 
