@@ -146,3 +146,24 @@ def test_planted_mistakes_are_caught(plant):
     X = rng.normal(size=(24, 24)) + 1j * rng.normal(size=(24, 24))
     S = X - X.T
     assert abs(emulate(S, plant) - pf_ref(S)) > 1e-6 * abs(pf_ref(S))
+
+
+@pytest.mark.parametrize("h", [2, 4, 10, 20, 22])
+def test_schur_complement_pfaffian_identity(h):
+    """The factored form's opt-in Pfaffian route: for the species matrix
+    S = [[A, -1], [1, B]] (A = e^{2i phi} M0 complex skew, B = -Z_s real skew
+    and invertible), Pf(S) = Pf(B) Pf(A + B^-1), with the cscale factor split
+    as Pf(c S) = Pf(c B) Pf(c (A + B^-1)); the h x h Pfaffian through the
+    kernel's 2D layout emulation.  A dropped sign (A - B^-1) is caught."""
+    rng = np.random.default_rng(300 + h)
+    X = rng.normal(size=(h, h)) + 1j * rng.normal(size=(h, h))
+    A = np.exp(0.8j) * (X - X.T)
+    Y = rng.normal(size=(h, h))
+    B = Y - Y.T
+    S = np.block([[A, -np.eye(h)], [np.eye(h), B]])
+    full = pf_ref(S)
+    Binv = np.linalg.inv(B)
+    assert abs(pf_ref(B) * pf_ref(A + Binv) - full) <= 1e-10 * abs(full)
+    c = 1.7
+    assert abs(pf_ref(c * B) * emulate(c * (A + Binv)) - pf_ref(c * S)) <= 1e-10 * abs(pf_ref(c * S))
+    assert abs(pf_ref(B) * pf_ref(A - Binv) - full) > 1e-6 * abs(full)
